@@ -1,0 +1,140 @@
+/* lgca.h -- C ABI of the MI355X FHP lattice-gas engine.
+ *
+ * A drop-in for the reference's GPU backend: every entry point maps to one
+ * method of `GpuLattice` (src/gpu.rs:453-755), which `Sim` (src/sim.rs:19-175)
+ * drives, so a Rust `Backend::Gpu` can call these through `extern "C"` instead
+ * of Metal. Unlike the reference's Metal kernels, this engine reproduces the
+ * reference CPU path (src/lattice.rs:228-285) bit for bit, including its
+ * random streams, which depend on the CPU thread count: that count is the
+ * `bands` argument of lg_create.
+ *
+ * Conventions
+ *   - Every call is synchronous (returns after the device work finished), as
+ *     the reference's methods are (src/gpu.rs:705,720,752).
+ *   - Functions returning int give 0 on success and a negative code on error;
+ *     the message is then available from lg_last_error() (thread-local). The
+ *     library never aborts on bad input; where the reference asserts
+ *     (src/gpu.rs:480-481,573,631,727-731) it returns LG_EINVAL.
+ *   - A handle owns all its device memory and its own HIP stream; use one
+ *     handle from one thread at a time. Distinct handles may run concurrently.
+ *   - Host arrays are borrowed for the duration of the call only.
+ *   - Cells are one byte per site in the reference's layout (src/hex.rs:7-23):
+ *     bits 0-5 moving directions, bit 6 rest, bit 7 solid; row-major, y * w + x.
+ */
+#ifndef LGCA_H
+#define LGCA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LG_OK 0
+#define LG_EINVAL (-1)   /* bad argument (the reference would assert) */
+#define LG_EDEVICE (-2)  /* HIP error, or no device */
+#define LG_ESTATE (-3)   /* call out of order, e.g. read_field before attach_field */
+
+typedef struct lg_lattice lg_t;
+
+/* GpuLattice::new (src/gpu.rs:479-515). h must be even and w, h > 4
+ * (src/lattice.rs:137-138). `bands` is the CPU thread count the run is to be
+ * bit-identical with (src/lattice.rs:232-242); it is new here because the
+ * reference's GPU path did not reproduce the CPU streams. Returns NULL on
+ * error (see lg_last_error). */
+lg_t* lg_create(size_t w, size_t h, int rest, uint64_t seed, size_t bands);
+
+/* Drop for GpuLattice. NULL is accepted. */
+void lg_destroy(lg_t* g);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* lg_last_error(void);
+
+/* GpuLattice::device_name (src/gpu.rs:517-519). */
+int lg_device_name(lg_t* g, char* buf, size_t len);
+
+/* GpuLattice::set_inlet (src/gpu.rs:525-528) given the seven 24-bit
+ * thresholds of Equilibrium::thresholds (src/lattice.rs:87-92), rest last. */
+int lg_set_inlet(lg_t* g, size_t cols, const uint32_t thresh[7]);
+
+/* GpuLattice::set_inlet with the equilibrium computed here exactly as
+ * Equilibrium::new (src/lattice.rs:59-82) computes it, in f32. */
+int lg_set_inlet_flow(lg_t* g, size_t cols, float density, float ux, float uy);
+
+/* GpuLattice::attach_field (src/gpu.rs:532-545): a device-resident coarse
+ * field of (w / bx) x (h / by) blocks, zero-initialised. */
+int lg_attach_field(lg_t* g, size_t bx, size_t by);
+
+/* GpuLattice::load (src/gpu.rs:572-610). n must be w * h. */
+int lg_load(lg_t* g, const uint8_t* cells, size_t n);
+
+/* GpuLattice::restart (src/gpu.rs:616-620): load, set the seed, steps = 0. */
+int lg_restart(lg_t* g, const uint8_t* cells, size_t n, uint64_t seed);
+
+/* GpuLattice::store (src/gpu.rs:630-652). n must be w * h. */
+int lg_store(lg_t* g, uint8_t* cells, size_t n);
+
+/* GpuLattice::advance (src/gpu.rs:657-659). */
+int lg_advance(lg_t* g, uint64_t n);
+
+/* GpuLattice::advance_sampling (src/gpu.rs:668-706): fold the lattice into
+ * the attached field every `every` steps and after a ragged last chunk;
+ * every == 0 skips sampling. */
+int lg_advance_sampling(lg_t* g, uint64_t n, uint64_t every, float alpha);
+
+/* GpuLattice::sample_field (src/gpu.rs:710-721). No-op without a field. */
+int lg_sample_field(lg_t* g, float alpha);
+
+/* GpuLattice::read_field (src/gpu.rs:725-736): copy the four f32 arrays out.
+ * nblocks must be (w / bx) * (h / by). */
+int lg_read_field(lg_t* g, float* ux, float* uy, float* rho, float* solid, size_t nblocks);
+
+/* New: the exact integer block sums behind the last sample (mass, px2, py2 in
+ * the units of src/hex.rs:29-30, and solid count), the quantities
+ * Field::sample derives its f32 values from (src/render.rs:93-116). */
+int lg_read_block_sums(lg_t* g, int64_t* mass, int64_t* px2, int64_t* py2, int64_t* nsolid,
+                       size_t nblocks);
+
+/* GpuLattice::total_particles (src/gpu.rs:740-754), as a 64-bit count. */
+int lg_total_particles(lg_t* g, uint64_t* out);
+
+/* Public fields w, h, steps of GpuLattice (src/gpu.rs:467-471). */
+int lg_dims(lg_t* g, size_t* w, size_t* h);
+uint64_t lg_steps(lg_t* g);
+
+/* New: set the step counter, so that cells + seed + steps + bands resume a
+ * run exactly (the reference can only restart at step 0). */
+int lg_set_steps(lg_t* g, uint64_t steps);
+
+/* New: advance n steps and report the device time of the step work, measured
+ * with HIP events on the handle's stream around the step launches. */
+int lg_advance_timed(lg_t* g, uint64_t n, float* ms);
+
+/* New: lg_advance_sampling with the same device-time report. */
+int lg_advance_sampling_timed(lg_t* g, uint64_t n, uint64_t every, float alpha, float* ms);
+
+/* Lattice::add_solid (src/lattice.rs:210-220) on the device, for the two
+ * obstacle shapes the reference builds: the plate box
+ * x0 <= px <= x1 && y0 <= py <= y1 (src/main.rs:584-588, tests/golden.rs:78-83)
+ * and the disc (px-cx)^2 + (py-cy)^2 <= r^2 (src/main.rs:589-592), over the
+ * f32 site positions of src/hex.rs:52-55. Marked sites lose their particles. */
+int lg_add_solid_box(lg_t* g, float x0, float x1, float y0, float y1);
+int lg_add_solid_disc(lg_t* g, float cx, float cy, float r);
+
+/* Lattice::init_equilibrium (src/lattice.rs:197-208) on the device: every
+ * fluid site drawn from Equilibrium(density, ux, uy) in index order from the
+ * stream seeded with seed ^ 0xA5A51234, bit-identical with the CPU. */
+int lg_init_equilibrium(lg_t* g, float density, float ux, float uy);
+
+/* Diagnostics. The generated collision circuit as the device evaluates it,
+ * for every state and every draw index k < stride, into out[state * stride +
+ * k] (0xFF where the lanes of a word disagreed). Returns the stride (30 with
+ * rest particles, 6 without) or a negative error. The same exhaustive check
+ * as the reference's lgca_verify kernel (src/gpu.rs:399-417, 775-820). */
+int lg_debug_collide_table(int rest, uint8_t* out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LGCA_H */

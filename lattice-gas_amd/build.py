@@ -1,0 +1,41 @@
+#!/usr/bin/env python3
+"""Build the HIP engine for gfx950: generate the collision circuit, then
+compile lattice-gas_amd/csrc/lgca_engine.hip into liblgca_hip.so (in-tree)."""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "liblgca_hip.so")
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # no fused multiply-add anywhere: the f32 field must round like the CPU's
+    "-ffp-contract=off",
+    # packed fp32 ops are kept out of every HIP build on these boxes
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+
+
+def build(verbose: bool = False) -> str:
+    subprocess.run([sys.executable, os.path.join(HERE, "gen_collide.py"),
+                    "--out", os.path.join(CSRC, "lgca_collide.gen.h")], check=True)
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = LIB + ".tmp"
+    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp, os.path.join(CSRC, "lgca_engine.hip")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed for lgca_engine.hip")
+    if verbose:
+        sys.stderr.write(r.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,786 @@
+// MI355X FHP lattice-gas engine: HIP kernels and the C ABI of include/lgca.h.
+//
+// Hot path per step (one launch of lg_step_kernel, one thread per u64 word):
+// gather six direction planes with in-word shifts and cross-word carries,
+// jump each word's position in its CPU band stream, draw the 64 collision
+// indices, evaluate the generated collision circuit, bounce back on solid
+// sites, re-seed the inflow columns, mask the ragged tail. The per-step
+// stream bases come from lg_basis_kernel, one launch per batch of steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lgca.h"
+#include "lgca_device.h"
+#include "lgca_jump.hpp"
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Stream bases for `nsteps` consecutive steps from t0: for step t0 + k and
+// stream s (s < nbands: collision band s of src/lattice.rs:242; s == nbands:
+// the inflow stream of src/lattice.rs:276, seeded with the incremented step)
+// write the states T^0 s0 .. T^63 s0 of the freshly seeded generator.
+__global__ void __launch_bounds__(256) lg_basis_kernel(uint64_t seed, uint64_t t0, uint32_t nsteps,
+                                                       uint32_t nbands, uint64_t* __restrict basis) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t per = nbands + 1;
+  if (g >= nsteps * per) return;
+  uint32_t k = g / per, s = g - k * per;
+  uint64_t t = t0 + k;
+  uint64_t st = lg_rng_init(s < nbands ? lg_band_seed(seed, t, s) : lg_inlet_seed(seed, t + 1));
+  uint64_t* out = basis + (size_t)g * 64;
+  for (int i = 0; i < 64; ++i) {
+    out[i] = st;
+    st = lg_xs(st);
+  }
+}
+
+// One step. Threads are mapped with the inflow words first (as the reference
+// does, src/gpu.rs:212-228) so the waves that run the serial re-seed hold
+// nothing else: g < h * iw -> word (g / iw, g % iw); otherwise row-major over
+// the remaining wpr - iw words of each row.
+__global__ void __launch_bounds__(256) lg_step_kernel(LgGeom G, const uint64_t* __restrict src,
+                                                      uint64_t* __restrict dst,
+                                                      const uint64_t* __restrict solid,
+                                                      const uint64_t* __restrict cjump,
+                                                      const uint32_t* __restrict row_band,
+                                                      const uint64_t* __restrict ijump,
+                                                      const uint64_t* __restrict basis) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G.total) return;
+  uint32_t y, j;
+  uint32_t ninlet = G.h * G.iw;
+  if (g < ninlet) {
+    y = g / G.iw;
+    j = g - y * G.iw;
+  } else {
+    uint32_t per = G.wpr - G.iw;
+    uint32_t q = g - ninlet;
+    y = q / per;
+    j = G.iw + (q - y * per);
+  }
+  lg_step_word(G, src, dst, solid, cjump, row_band, ijump, basis, y, j);
+}
+
+// Bytes -> planes (GpuLattice::load, src/gpu.rs:572-610), one thread per word.
+__global__ void __launch_bounds__(256) lg_pack_kernel(LgGeom G, const uint8_t* __restrict cells,
+                                                      uint64_t* __restrict planes,
+                                                      uint64_t* __restrict solid) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G.total) return;
+  uint32_t y = g / G.wpr, j = g - y * G.wpr;
+  uint32_t x0 = j * 64, n = min(64u, G.w - x0);
+  const uint8_t* row = cells + (size_t)y * G.w + x0;
+  uint64_t p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t x = 0; x < n; ++x) {
+    uint32_t c = row[x];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) p[d] |= (uint64_t)((c >> d) & 1u) << x;
+  }
+#pragma unroll
+  for (int d = 0; d < 7; ++d) planes[(size_t)d * G.total + g] = p[d];
+  solid[g] = p[7];
+}
+
+// Planes -> bytes (GpuLattice::store, src/gpu.rs:630-652).
+__global__ void __launch_bounds__(256) lg_unpack_kernel(LgGeom G, const uint64_t* __restrict planes,
+                                                        const uint64_t* __restrict solid,
+                                                        uint8_t* __restrict cells) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G.total) return;
+  uint32_t y = g / G.wpr, j = g - y * G.wpr;
+  uint32_t x0 = j * 64, n = min(64u, G.w - x0);
+  uint64_t p[8];
+#pragma unroll
+  for (int d = 0; d < 7; ++d) p[d] = planes[(size_t)d * G.total + g];
+  p[7] = solid[g];
+  uint8_t* row = cells + (size_t)y * G.w + x0;
+  for (uint32_t x = 0; x < n; ++x) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) c |= (uint32_t)((p[d] >> x) & 1u) << d;
+    row[x] = (uint8_t)c;
+  }
+}
+
+// Coarse-graining, one thread per block of the field grid: exact integer sums
+// over fluid cells (src/moments.rs:37-65 FLUID, src/render.rs:93-106), then
+// the f32 running average of src/render.rs:107-115 with every operation
+// rounded on its own (no fused multiply-add), as the CPU computes it.
+__global__ void __launch_bounds__(256) lg_sample_kernel(
+    LgGeom G, const uint64_t* __restrict planes, const uint64_t* __restrict solid, uint32_t bx,
+    uint32_t by, uint32_t bw, uint32_t bh, float alpha, float* __restrict ux, float* __restrict uy,
+    float* __restrict rho, float* __restrict sol, int64_t* __restrict sums) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= bw * bh) return;
+  uint32_t jb = g / bw, ib = g - jb * bw;
+  const int CX2[6] = {2, 1, -1, -2, -1, 1};
+  const int CY2[6] = {0, 1, 1, 0, -1, -1};
+  uint32_t x0 = ib * bx, x1 = x0 + bx;
+  uint32_t w0 = x0 >> 6, w1 = (x1 - 1) >> 6;
+  int64_t mass = 0, px = 0, py = 0, ns = 0;
+  for (uint32_t y = jb * by; y < (jb + 1) * by; ++y) {
+    size_t base = (size_t)y * G.wpr;
+    for (uint32_t wi = w0; wi <= w1; ++wi) {
+      uint32_t lo = max(x0, wi << 6) - (wi << 6);
+      uint32_t hi = min(x1, (wi + 1) << 6) - (wi << 6);
+      uint64_t m = (hi - lo >= 64) ? ~0ull : (((1ull << (hi - lo)) - 1ull) << lo);
+      uint64_t sm = solid[base + wi] & m;
+      ns += __popcll(sm);
+      uint64_t fm = m & ~sm;
+      for (int d = 0; d < 6; ++d) {
+        int64_t c = __popcll(planes[(size_t)d * G.total + base + wi] & fm);
+        mass += c;
+        px += CX2[d] * c;
+        py += CY2[d] * c;
+      }
+      mass += __popcll(planes[(size_t)6 * G.total + base + wi] & fm);
+    }
+  }
+  sums[(size_t)g * 4 + 0] = mass;
+  sums[(size_t)g * 4 + 1] = px;
+  sums[(size_t)g * 4 + 2] = py;
+  sums[(size_t)g * 4 + 3] = ns;
+  // src/moments.rs:91-97 and src/render.rs:107-115
+  const float SQRT3_2 = 0.8660254f;
+  float n = (float)(bx * by);
+  float vx = 0.0f, vy = 0.0f;
+  if (mass != 0) {
+    float mf = (float)mass;
+    vx = __fdiv_rn(__fmul_rn((float)px, 0.5f), mf);
+    vy = __fdiv_rn(__fmul_rn((float)py, SQRT3_2), mf);
+  }
+  ux[g] = __fadd_rn(ux[g], __fmul_rn(alpha, __fsub_rn(vx, ux[g])));
+  uy[g] = __fadd_rn(uy[g], __fmul_rn(alpha, __fsub_rn(vy, uy[g])));
+  rho[g] = __fadd_rn(rho[g], __fmul_rn(alpha, __fsub_rn(__fdiv_rn((float)mass, n), rho[g])));
+  sol[g] = __fdiv_rn((float)ns, n);
+}
+
+// Obstacles (src/lattice.rs:210-220): every site whose f32 position
+// (src/hex.rs:52-55) satisfies the predicate becomes SOLID_BIT with no
+// particles. kind 0: box x0 <= px <= x1 && y0 <= py <= y1 (src/main.rs:584-588,
+// tests/golden.rs:78-83); kind 1: disc (px-a)^2 + (py-b)^2 <= r^2
+// (src/main.rs:589-592) with a = x0, b = x1, r = y0.
+__global__ void __launch_bounds__(256) lg_solid_kernel(LgGeom G, int kind, float x0, float x1,
+                                                       float y0, float y1,
+                                                       uint64_t* __restrict planes,
+                                                       uint64_t* __restrict solid) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G.total) return;
+  uint32_t y = g / G.wpr, j = g - y * G.wpr;
+  const float SQRT3_2 = 0.8660254f;
+  float py = __fmul_rn((float)y, SQRT3_2);
+  uint64_t m = 0;
+  uint32_t n = min(64u, G.w - j * 64);
+  for (uint32_t b = 0; b < n; ++b) {
+    float px = __fadd_rn((float)(j * 64 + b), (y & 1u) ? 0.5f : 0.0f);
+    bool in;
+    if (kind == 0) {
+      in = px >= x0 && px <= x1 && py >= y0 && py <= y1;
+    } else {
+      float dx = __fsub_rn(px, x0), dy = __fsub_rn(py, x1);
+      in = __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)) <= __fmul_rn(y0, y0);
+    }
+    m |= (uint64_t)in << b;
+  }
+  if (!m) return;
+  solid[g] |= m;
+#pragma unroll
+  for (int d = 0; d < 7; ++d) planes[(size_t)d * G.total + g] &= ~m;
+}
+
+// Lattice::init_equilibrium (src/lattice.rs:197-208): every fluid cell in
+// index order takes Equilibrium::sample from the stream seeded with
+// seed ^ 0xA5A51234; the word's start in that stream is given by `cinit`.
+__global__ void __launch_bounds__(256) lg_init_kernel(LgGeom G, const uint64_t* __restrict cinit,
+                                                      const uint64_t* __restrict basis,
+                                                      uint64_t* __restrict planes,
+                                                      const uint64_t* __restrict solid) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G.total) return;
+  uint32_t j = g % G.wpr;
+  uint64_t tail = (j == G.lastword) ? G.tailmask : ~0ull;
+  uint64_t fm = ~solid[g] & tail;
+  uint64_t res[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (fm) lg_inlet_word(G, lg_jump(cinit[g], basis), fm, res);
+#pragma unroll
+  for (int d = 0; d < 7; ++d) planes[(size_t)d * G.total + g] = res[d];
+}
+
+// Total particles, obstacle sites included (src/lattice.rs:299-312): one
+// thread per word, a 64-bit atomic per wave.
+__global__ void __launch_bounds__(256) lg_count_kernel(LgGeom G, const uint64_t* __restrict planes,
+                                                       unsigned long long* __restrict out) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long c = 0;
+  if (g < G.total) {
+#pragma unroll
+    for (int d = 0; d < 7; ++d) c += __popcll(planes[(size_t)d * G.total + g]);
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+
+// Exhaustive check of the generated circuit as the device evaluates it: one
+// thread per (state, k), k < stride; lanes must agree (out 0xFF otherwise).
+__global__ void __launch_bounds__(256) lg_verify_kernel(int rest, uint8_t* __restrict out) {
+  const int ns = rest ? 128 : 64;
+  const int stride = rest ? LG_COLLIDE_REST_STRIDE : LG_COLLIDE_NOREST_STRIDE;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns * stride) return;
+  int s = i / stride, k = i - s * stride;
+  uint64_t n[7], o[7];
+  for (int d = 0; d < 7; ++d) n[d] = ((s >> d) & 1) ? ~0ull : 0ull;
+  if (rest) {
+    uint64_t K[LG_COLLIDE_REST_KBITS];
+    for (int b = 0; b < LG_COLLIDE_REST_KBITS; ++b) K[b] = ((k >> b) & 1) ? ~0ull : 0ull;
+    lg_collide_rest(n, K, o);
+  } else {
+    uint64_t K[LG_COLLIDE_NOREST_KBITS];
+    for (int b = 0; b < LG_COLLIDE_NOREST_KBITS; ++b) K[b] = ((k >> b) & 1) ? ~0ull : 0ull;
+    lg_collide_norest(n, K, o);
+  }
+  int r = 0, bad = 0;
+  for (int d = 0; d < 7; ++d) {
+    if (o[d] == ~0ull) r |= 1 << d;
+    else if (o[d] != 0) bad = 1;
+  }
+  out[i] = bad ? 0xFF : (uint8_t)r;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(LG_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)&p, count * sizeof(T));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+// Equilibrium thresholds exactly as src/lattice.rs:59-82 computes them: each
+// f32 operation rounded on its own, in source order.
+uint32_t threshold_f32(float p) {
+  const float ONE = 16777216.0f;
+  volatile float prod = p * ONE;
+  float t = std::ceil((float)prod);
+  if (t >= ONE) return (uint32_t)ONE;
+  if (t > 0.0f) return (uint32_t)t;
+  return 0;
+}
+
+void equilibrium_thresholds(float density, float ux, float uy, bool rest, uint32_t out[7]) {
+  const float SQRT3_2 = 0.8660254f;
+  const float CXF[6] = {1.0f, 0.5f, -0.5f, -1.0f, -0.5f, 0.5f};
+  const float CYF[6] = {0.0f, SQRT3_2, SQRT3_2, 0.0f, -SQRT3_2, -SQRT3_2};
+  const float bias = rest ? (2.0f * 7.0f / 6.0f) : 2.0f;
+  for (int d = 0; d < 6; ++d) {
+    volatile float a = CXF[d] * ux;
+    volatile float b = CYF[d] * uy;
+    volatile float dot = a + b;
+    volatile float sc = bias * dot;
+    volatile float one_plus = 1.0f + sc;
+    volatile float p = density * one_plus;
+    out[d] = threshold_f32(p);
+  }
+  out[6] = rest ? threshold_f32(density) : 0;
+}
+
+}  // namespace
+
+struct lg_lattice {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  LgGeom G{};
+  uint64_t seed = 0, steps = 0;
+  uint32_t bands = 1;
+  bool loaded = false;
+  // planes: a (current), b (next); solid; static tables
+  DevBuf<uint64_t> a, b, solid, cjump, ijump, basis;
+  DevBuf<uint32_t> row_band;
+  DevBuf<uint8_t> bytes;
+  uint32_t basis_steps = 0;  // capacity of `basis` in steps
+  // field
+  bool has_field = false;
+  uint32_t bx = 0, by = 0, bw = 0, bh = 0;
+  DevBuf<float> fux, fuy, frho, fsol;
+  DevBuf<int64_t> fsums;
+  DevBuf<unsigned long long> counter;
+};
+
+namespace {
+
+uint32_t blocks_for(size_t n, uint32_t t = 256) { return (uint32_t)((n + t - 1) / t); }
+
+int rebuild_tables(lg_t* g, const std::vector<uint64_t>& solid_host) {
+  std::vector<uint64_t> cj, ij;
+  std::vector<uint32_t> rb;
+  lgca::build_tables(g->G.w, g->G.h, g->bands, g->G.inlet_cols, g->G.rest != 0, solid_host, cj, rb,
+                     ij);
+  if (lgca::charpoly_low() == 0) return fail(LG_EDEVICE, "jump polynomial not found");
+  HIP_TRY(g->cjump.alloc(cj.size()));
+  HIP_TRY(g->row_band.alloc(rb.size()));
+  HIP_TRY(g->ijump.alloc(std::max<size_t>(ij.size(), 1)));
+  HIP_TRY(hipMemcpyAsync(g->cjump.p, cj.data(), cj.size() * 8, hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipMemcpyAsync(g->row_band.p, rb.data(), rb.size() * 4, hipMemcpyHostToDevice, g->stream));
+  if (!ij.empty())
+    HIP_TRY(hipMemcpyAsync(g->ijump.p, ij.data(), ij.size() * 8, hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return LG_OK;
+}
+
+// The solid plane of a byte lattice, for the jump tables (host side).
+std::vector<uint64_t> solid_plane(const LgGeom& G, const uint8_t* cells) {
+  std::vector<uint64_t> s((size_t)G.total, 0);
+  for (uint32_t y = 0; y < G.h; ++y) {
+    const uint8_t* row = cells + (size_t)y * G.w;
+    for (uint32_t x = 0; x < G.w; ++x)
+      if (row[x] & 0x80u) s[(size_t)y * G.wpr + x / 64] |= 1ull << (x % 64);
+  }
+  return s;
+}
+
+int download_solid(lg_t* g, std::vector<uint64_t>& s) {
+  s.resize((size_t)g->G.total);
+  HIP_TRY(hipMemcpyAsync(s.data(), g->solid.p, s.size() * 8, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return LG_OK;
+}
+
+int add_solid(lg_t* g, int kind, float a, float b, float c, float d) {
+  hipLaunchKernelGGL(lg_solid_kernel, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream, g->G,
+                     kind, a, b, c, d, g->a.p, g->solid.p);
+  HIP_TRY(hipGetLastError());
+  std::vector<uint64_t> s;
+  int rc = download_solid(g, s);
+  if (rc) return rc;
+  g->loaded = true;
+  return rebuild_tables(g, s);
+}
+
+int sample_now(lg_t* g, float alpha) {
+  if (!g->has_field) return LG_OK;
+  hipLaunchKernelGGL(lg_sample_kernel, dim3(blocks_for((size_t)g->bw * g->bh)), dim3(256), 0,
+                     g->stream, g->G, g->a.p, g->solid.p, g->bx, g->by, g->bw, g->bh, alpha,
+                     g->fux.p, g->fuy.p, g->frho.p, g->fsol.p, g->fsums.p);
+  HIP_TRY(hipGetLastError());
+  return LG_OK;
+}
+
+// Enqueue `n` steps (and samples every `every`), no synchronisation.
+int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) {
+  uint64_t left = n, since = 0;
+  const uint32_t per = g->G.nbands + 1;
+  bool first = true;
+  while (left > 0) {
+    uint64_t chunk = std::min<uint64_t>(left, g->basis_steps);
+    if (every > 0) chunk = std::min<uint64_t>(chunk, every - since);
+    hipLaunchKernelGGL(lg_basis_kernel, dim3(blocks_for((size_t)chunk * per)), dim3(256), 0,
+                       g->stream, g->seed, g->steps, (uint32_t)chunk, g->G.nbands, g->basis.p);
+    HIP_TRY(hipGetLastError());
+    if (timed && first) HIP_TRY(hipEventRecord(g->ev0, g->stream));
+    first = false;
+    for (uint64_t k = 0; k < chunk; ++k) {
+      hipLaunchKernelGGL(lg_step_kernel, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream,
+                         g->G, g->a.p, g->b.p, g->solid.p, g->cjump.p, g->row_band.p, g->ijump.p,
+                         g->basis.p + (size_t)k * per * 64);
+      std::swap(g->a.p, g->b.p);
+      g->steps += 1;
+    }
+    HIP_TRY(hipGetLastError());
+    left -= chunk;
+    since += chunk;
+    if (every > 0 && (since >= every || left == 0)) {
+      since = 0;
+      int rc = sample_now(g, alpha);
+      if (rc) return rc;
+    }
+  }
+  if (timed) HIP_TRY(hipEventRecord(g->ev1, g->stream));
+  return LG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lg_last_error(void) { return g_err.c_str(); }
+
+lg_t* lg_create(size_t w, size_t h, int rest, uint64_t seed, size_t bands) {
+  if (h % 2 != 0) {
+    fail(LG_EINVAL, "row count must be even for the lattice to wrap cleanly in y");
+    return nullptr;
+  }
+  if (w <= 4 || h <= 4) {
+    fail(LG_EINVAL, "lattice is too small");
+    return nullptr;
+  }
+  if (w > (1u << 30) || h > (1u << 30) || ((w + 63) / 64) * h >= (1ull << 32)) {
+    fail(LG_EINVAL, "lattice is too large for 32-bit word indices");
+    return nullptr;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    fail(LG_EDEVICE, "no HIP device on this machine");
+    return nullptr;
+  }
+  auto* g = new lg_lattice();
+  auto bail = [&](const std::string& m) -> lg_t* {
+    fail(LG_EDEVICE, m);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return nullptr;
+  };
+  if (hipGetDevice(&g->device) != hipSuccess) return bail("hipGetDevice failed");
+  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail("hipStreamCreate failed");
+  if (hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess)
+    return bail("hipEventCreate failed");
+  LgGeom& G = g->G;
+  G.w = (uint32_t)w;
+  G.h = (uint32_t)h;
+  G.wpr = (uint32_t)((w + 63) / 64);
+  G.total = G.wpr * G.h;
+  G.lastword = (uint32_t)((w - 1) / 64);
+  G.lastbit = (uint32_t)((w - 1) % 64);
+  G.tailmask = G.lastbit == 63 ? ~0ull : ((1ull << (G.lastbit + 1)) - 1);
+  G.iw = 0;
+  G.inlet_cols = 0;
+  G.rest = rest ? 1u : 0u;
+  g->bands = (uint32_t)std::max<size_t>(1, std::min<size_t>(bands, h));
+  G.nbands = lgca::n_bands(G.h, g->bands);
+  g->seed = seed;
+  const size_t per = G.nbands + 1;
+  g->basis_steps = (uint32_t)std::max<size_t>(1, std::min<size_t>(512, (64u << 20) / (per * 512)));
+  if (g->a.alloc((size_t)7 * G.total) || g->b.alloc((size_t)7 * G.total) ||
+      g->solid.alloc(G.total) || g->basis.alloc((size_t)g->basis_steps * per * 64) ||
+      g->counter.alloc(1))
+    return bail("hipMalloc failed for the lattice planes");
+  if (hipMemsetAsync(g->a.p, 0, g->a.n * 8, g->stream) ||
+      hipMemsetAsync(g->b.p, 0, g->b.n * 8, g->stream) ||
+      hipMemsetAsync(g->solid.p, 0, g->solid.n * 8, g->stream) ||
+      hipStreamSynchronize(g->stream))
+    return bail("hipMemset failed");
+  std::vector<uint64_t> empty((size_t)G.total, 0);
+  if (rebuild_tables(g, empty) != LG_OK) return bail(g_err);
+  return g;
+}
+
+void lg_destroy(lg_t* g) {
+  if (!g) return;
+  (void)hipStreamSynchronize(g->stream);
+  g->a.release();
+  g->b.release();
+  g->solid.release();
+  g->cjump.release();
+  g->ijump.release();
+  g->basis.release();
+  g->row_band.release();
+  g->bytes.release();
+  g->fux.release();
+  g->fuy.release();
+  g->frho.release();
+  g->fsol.release();
+  g->fsums.release();
+  g->counter.release();
+  if (g->ev0) (void)hipEventDestroy(g->ev0);
+  if (g->ev1) (void)hipEventDestroy(g->ev1);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+int lg_device_name(lg_t* g, char* buf, size_t len) {
+  if (!g || !buf || len == 0) return fail(LG_EINVAL, "null argument");
+  hipDeviceProp_t p;
+  HIP_TRY(hipGetDeviceProperties(&p, g->device));
+  std::snprintf(buf, len, "%s (%s)", p.name, p.gcnArchName);
+  return LG_OK;
+}
+
+int lg_set_inlet(lg_t* g, size_t cols, const uint32_t thresh[7]) {
+  if (!g || (!thresh && cols)) return fail(LG_EINVAL, "null argument");
+  for (int d = 0; d < 7; ++d) {
+    uint32_t t = thresh ? thresh[d] : 0;
+    if (t > (1u << 24)) return fail(LG_EINVAL, "threshold above 2^24");
+    g->G.thresh[d] = t;
+  }
+  uint32_t c = (uint32_t)std::min<size_t>(cols, g->G.w);
+  g->G.inlet_cols = c;
+  g->G.iw = (c + 63) / 64;
+  // inflow words move to the front of the grid, which needs wpr > iw or
+  // all words inflow; both are handled by the mapping.
+  if (!g->loaded) {
+    std::vector<uint64_t> empty((size_t)g->G.total, 0);
+    return rebuild_tables(g, empty);
+  }
+  // rebuild the inflow tables from the device's solid plane
+  std::vector<uint64_t> s;
+  int rc = download_solid(g, s);
+  if (rc) return rc;
+  return rebuild_tables(g, s);
+}
+
+int lg_set_inlet_flow(lg_t* g, size_t cols, float density, float ux, float uy) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  uint32_t t[7];
+  equilibrium_thresholds(density, ux, uy, g->G.rest != 0, t);
+  return lg_set_inlet(g, cols, t);
+}
+
+int lg_attach_field(lg_t* g, size_t bx, size_t by) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  if (bx == 0 || by == 0) return fail(LG_EINVAL, "block size must be positive");
+  g->bx = (uint32_t)bx;
+  g->by = (uint32_t)by;
+  g->bw = g->G.w / g->bx;
+  g->bh = g->G.h / g->by;
+  size_t nb = std::max<size_t>((size_t)g->bw * g->bh, 1);
+  HIP_TRY(g->fux.alloc(nb));
+  HIP_TRY(g->fuy.alloc(nb));
+  HIP_TRY(g->frho.alloc(nb));
+  HIP_TRY(g->fsol.alloc(nb));
+  HIP_TRY(g->fsums.alloc(nb * 4));
+  HIP_TRY(hipMemsetAsync(g->fux.p, 0, nb * 4, g->stream));
+  HIP_TRY(hipMemsetAsync(g->fuy.p, 0, nb * 4, g->stream));
+  HIP_TRY(hipMemsetAsync(g->frho.p, 0, nb * 4, g->stream));
+  HIP_TRY(hipMemsetAsync(g->fsol.p, 0, nb * 4, g->stream));
+  HIP_TRY(hipMemsetAsync(g->fsums.p, 0, nb * 32, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->has_field = true;
+  return LG_OK;
+}
+
+int lg_load(lg_t* g, const uint8_t* cells, size_t n) {
+  if (!g || !cells) return fail(LG_EINVAL, "null argument");
+  if (n != (size_t)g->G.w * g->G.h) return fail(LG_EINVAL, "wrong number of cells");
+  if (g->bytes.n < n) HIP_TRY(g->bytes.alloc(n));
+  HIP_TRY(hipMemcpyAsync(g->bytes.p, cells, n, hipMemcpyHostToDevice, g->stream));
+  hipLaunchKernelGGL(lg_pack_kernel, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream, g->G,
+                     g->bytes.p, g->a.p, g->solid.p);
+  HIP_TRY(hipGetLastError());
+  int rc = rebuild_tables(g, solid_plane(g->G, cells));
+  if (rc) return rc;
+  g->loaded = true;
+  return LG_OK;
+}
+
+int lg_restart(lg_t* g, const uint8_t* cells, size_t n, uint64_t seed) {
+  int rc = lg_load(g, cells, n);
+  if (rc) return rc;
+  g->seed = seed;
+  g->steps = 0;
+  return LG_OK;
+}
+
+int lg_store(lg_t* g, uint8_t* cells, size_t n) {
+  if (!g || !cells) return fail(LG_EINVAL, "null argument");
+  if (n != (size_t)g->G.w * g->G.h) return fail(LG_EINVAL, "wrong number of cells");
+  if (g->bytes.n < n) HIP_TRY(g->bytes.alloc(n));
+  hipLaunchKernelGGL(lg_unpack_kernel, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream,
+                     g->G, g->a.p, g->solid.p, g->bytes.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(cells, g->bytes.p, n, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return LG_OK;
+}
+
+int lg_advance(lg_t* g, uint64_t n) { return lg_advance_sampling(g, n, 0, 0.0f); }
+
+int lg_advance_sampling(lg_t* g, uint64_t n, uint64_t every, float alpha) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  int rc = enqueue_steps(g, n, every, alpha, false);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return LG_OK;
+}
+
+int lg_advance_sampling_timed(lg_t* g, uint64_t n, uint64_t every, float alpha, float* ms) {
+  if (!g || !ms) return fail(LG_EINVAL, "null argument");
+  if (n == 0) {
+    *ms = 0.0f;
+    return LG_OK;
+  }
+  int rc = enqueue_steps(g, n, every, alpha, true);
+  if (rc) return rc;
+  HIP_TRY(hipEventSynchronize(g->ev1));
+  HIP_TRY(hipEventElapsedTime(ms, g->ev0, g->ev1));
+  return LG_OK;
+}
+
+int lg_advance_timed(lg_t* g, uint64_t n, float* ms) {
+  return lg_advance_sampling_timed(g, n, 0, 0.0f, ms);
+}
+
+int lg_sample_field(lg_t* g, float alpha) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  int rc = sample_now(g, alpha);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return LG_OK;
+}
+
+int lg_read_field(lg_t* g, float* ux, float* uy, float* rho, float* solid, size_t nblocks) {
+  if (!g || !ux || !uy || !rho || !solid) return fail(LG_EINVAL, "null argument");
+  if (!g->has_field) return fail(LG_ESTATE, "no field attached to this lattice");
+  if (nblocks != (size_t)g->bw * g->bh)
+    return fail(LG_EINVAL, "the field on the device has a different block grid");
+  if (nblocks == 0) return LG_OK;
+  HIP_TRY(hipMemcpyAsync(ux, g->fux.p, nblocks * 4, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(uy, g->fuy.p, nblocks * 4, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(rho, g->frho.p, nblocks * 4, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(solid, g->fsol.p, nblocks * 4, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return LG_OK;
+}
+
+int lg_read_block_sums(lg_t* g, int64_t* mass, int64_t* px2, int64_t* py2, int64_t* nsolid,
+                       size_t nblocks) {
+  if (!g || !mass || !px2 || !py2 || !nsolid) return fail(LG_EINVAL, "null argument");
+  if (!g->has_field) return fail(LG_ESTATE, "no field attached to this lattice");
+  if (nblocks != (size_t)g->bw * g->bh)
+    return fail(LG_EINVAL, "the field on the device has a different block grid");
+  std::vector<int64_t> s(nblocks * 4);
+  if (nblocks) {
+    HIP_TRY(hipMemcpyAsync(s.data(), g->fsums.p, s.size() * 8, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+  }
+  for (size_t i = 0; i < nblocks; ++i) {
+    mass[i] = s[i * 4 + 0];
+    px2[i] = s[i * 4 + 1];
+    py2[i] = s[i * 4 + 2];
+    nsolid[i] = s[i * 4 + 3];
+  }
+  return LG_OK;
+}
+
+int lg_total_particles(lg_t* g, uint64_t* out) {
+  if (!g || !out) return fail(LG_EINVAL, "null argument");
+  HIP_TRY(hipMemsetAsync(g->counter.p, 0, 8, g->stream));
+  hipLaunchKernelGGL(lg_count_kernel, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream,
+                     g->G, g->a.p, g->counter.p);
+  HIP_TRY(hipGetLastError());
+  unsigned long long v = 0;
+  HIP_TRY(hipMemcpyAsync(&v, g->counter.p, 8, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  *out = v;
+  return LG_OK;
+}
+
+int lg_dims(lg_t* g, size_t* w, size_t* h) {
+  if (!g || !w || !h) return fail(LG_EINVAL, "null argument");
+  *w = g->G.w;
+  *h = g->G.h;
+  return LG_OK;
+}
+
+uint64_t lg_steps(lg_t* g) { return g ? g->steps : 0; }
+
+int lg_set_steps(lg_t* g, uint64_t steps) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  g->steps = steps;
+  return LG_OK;
+}
+
+int lg_add_solid_box(lg_t* g, float x0, float x1, float y0, float y1) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  return add_solid(g, 0, x0, x1, y0, y1);
+}
+
+int lg_add_solid_disc(lg_t* g, float cx, float cy, float r) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  return add_solid(g, 1, cx, cy, r, 0.0f);
+}
+
+int lg_init_equilibrium(lg_t* g, float density, float ux, float uy) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  std::vector<uint64_t> s;
+  int rc = download_solid(g, s);
+  if (rc) return rc;
+  const LgGeom& G0 = g->G;
+  // per-word start in the init stream: 4 (3) draws per fluid cell before it
+  const uint64_t p = lgca::charpoly_low();
+  const uint32_t per = G0.rest ? 4 : 3;
+  std::vector<uint64_t> cinit((size_t)G0.total);
+  uint64_t c = 1;
+  for (uint32_t y = 0; y < G0.h; ++y)
+    for (uint32_t j = 0; j < G0.wpr; ++j) {
+      size_t word = (size_t)y * G0.wpr + j;
+      cinit[word] = c;
+      uint64_t tail = (j == G0.lastword) ? G0.tailmask : ~0ull;
+      c = lgca::poly_mulz(c, (uint32_t)__builtin_popcountll(~s[word] & tail) * per, p);
+    }
+  uint64_t basis[64];
+  uint64_t st = lg_rng_init(g->seed ^ 0xA5A51234ull);
+  for (int i = 0; i < 64; ++i) {
+    basis[i] = st;
+    st = lg_xs(st);
+  }
+  LgGeom G = G0;
+  equilibrium_thresholds(density, ux, uy, G.rest != 0, G.thresh);
+  DevBuf<uint64_t> dc, db;
+  HIP_TRY(dc.alloc(cinit.size()));
+  HIP_TRY(db.alloc(64));
+  HIP_TRY(hipMemcpyAsync(dc.p, cinit.data(), cinit.size() * 8, hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipMemcpyAsync(db.p, basis, 64 * 8, hipMemcpyHostToDevice, g->stream));
+  hipLaunchKernelGGL(lg_init_kernel, dim3(blocks_for(G.total)), dim3(256), 0, g->stream, G, dc.p,
+                     db.p, g->a.p, g->solid.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->loaded = true;
+  return LG_OK;
+}
+
+// Test hook: the generated circuit evaluated on the device for every state
+// and every k < stride (out: n_states * stride bytes, 0xFF where lanes of a
+// word disagreed). Returns the stride.
+int lg_debug_collide_table(int rest, uint8_t* out, size_t n) {
+  const size_t ns = rest ? 128 : 64;
+  const size_t stride = rest ? LG_COLLIDE_REST_STRIDE : LG_COLLIDE_NOREST_STRIDE;
+  if (!out || n < ns * stride) return fail(LG_EINVAL, "output too small");
+  uint8_t* d = nullptr;
+  HIP_TRY(hipMalloc((void**)&d, ns * stride));
+  hipLaunchKernelGGL(lg_verify_kernel, dim3(blocks_for(ns * stride)), dim3(256), 0, 0, rest, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(out, d, ns * stride, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(LG_EDEVICE, hipGetErrorString(e));
+  return (int)stride;
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+// Host-side jump-ahead tables for the exact CPU random streams.
+//
+// The reference draws one xorshift64* value per fluid cell, sequentially per
+// row band and per step (src/lattice.rs:232-256), and 4 (3) per fluid inflow
+// cell from one serial stream (src/lattice.rs:270-285). A device lane that
+// owns a 64-cell word must start at the right place in its stream. xorshift's
+// state update is linear over GF(2)^64, s -> T s, so
+//     T^n s = c(T) s,  c(z) = z^n mod P(z),
+// where P is T's characteristic polynomial (Cayley-Hamilton). P is found once
+// by Berlekamp-Massey on one output bit of the state sequence; the jump is
+// then a 64-term XOR over the stream's first 64 states. This is the method of
+// Haramoto, Matsumoto, Nishimura, Panneton and L'Ecuyer, "Efficient Jump Ahead
+// for F2-Linear Random Number Generators", INFORMS J. Computing 20(3), 2008.
+//
+// The polynomials depend only on the obstacle layout (which cells draw), so
+// they are built once per load and reused by every step.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+namespace lgca {
+
+inline uint64_t xs_step(uint64_t x) {
+  x ^= x << 13;
+  x ^= x >> 7;
+  x ^= x << 17;
+  return x;
+}
+
+// Low 64 coefficients of P(z) = z^64 + sum_{i<64} p_i z^i.
+inline uint64_t charpoly_low() {
+  static uint64_t cached = 0;
+  static bool done = false;
+  if (done) return cached;
+  // Bit 0 of the state sequence from an arbitrary non-zero start.
+  const int N = 256;
+  uint8_t a[N];
+  uint64_t s = 0x0123456789ABCDEFull;
+  for (int i = 0; i < N; ++i) {
+    a[i] = (uint8_t)(s & 1u);
+    s = xs_step(s);
+  }
+  // Berlekamp-Massey over GF(2): connection polynomial C(z) = 1 + c_1 z + ...
+  std::vector<uint8_t> C(N + 1, 0), B(N + 1, 0), T;
+  C[0] = B[0] = 1;
+  int L = 0, m = 1;
+  for (int n = 0; n < N; ++n) {
+    uint8_t d = a[n];
+    for (int i = 1; i <= L; ++i) d ^= (uint8_t)(C[i] & a[n - i]);
+    if (d == 0) {
+      ++m;
+    } else if (2 * L <= n) {
+      T = C;
+      for (int i = 0; i + m <= N; ++i) C[i + m] ^= B[i];
+      L = n + 1 - L;
+      B = T;
+      m = 1;
+    } else {
+      for (int i = 0; i + m <= N; ++i) C[i + m] ^= B[i];
+      ++m;
+    }
+  }
+  // P(z) = z^L C(1/z): coefficient of z^(L-j) is c_j.
+  uint64_t p = 0;
+  if (L == 64)
+    for (int j = 1; j <= 64; ++j)
+      if (C[j]) p |= 1ull << (64 - j);
+  cached = p;
+  done = true;
+  return p;
+}
+
+// c(z) * z^f mod P(z)
+inline uint64_t poly_mulz(uint64_t c, uint32_t f, uint64_t p) {
+  for (uint32_t i = 0; i < f; ++i) {
+    uint64_t carry = c >> 63;
+    c <<= 1;
+    if (carry) c ^= p;
+  }
+  return c;
+}
+
+// a(z) * b(z) mod P(z)
+inline uint64_t poly_mulmod(uint64_t a, uint64_t b, uint64_t p) {
+  uint64_t r = 0;
+  for (int i = 63; i >= 0; --i) {
+    uint64_t carry = r >> 63;
+    r <<= 1;
+    if (carry) r ^= p;
+    if ((b >> i) & 1u) r ^= a;
+  }
+  return r;
+}
+
+// z^n mod P(z)
+inline uint64_t poly_pow(uint64_t n, uint64_t p) {
+  uint64_t r = 1, base = 2;  // z
+  while (n) {
+    if (n & 1u) r = poly_mulmod(r, base, p);
+    base = poly_mulmod(base, base, p);
+    n >>= 1;
+  }
+  return r;
+}
+
+// T^n s, evaluated through the polynomial (for tests and the host tables).
+inline uint64_t jump_state(uint64_t s, uint64_t n) {
+  uint64_t c = poly_pow(n, charpoly_low());
+  uint64_t st = 0, b = s;
+  for (int i = 0; i < 64; ++i) {
+    if ((c >> i) & 1u) st ^= b;
+    b = xs_step(b);
+  }
+  return st;
+}
+
+// Band geometry of src/lattice.rs:232-240.
+inline uint32_t band_rows(uint32_t h, uint32_t threads) {
+  uint32_t nt = threads < 1 ? 1 : threads;
+  if (nt > h) nt = h;
+  return (h + nt - 1) / nt;
+}
+inline uint32_t n_bands(uint32_t h, uint32_t threads) {
+  uint32_t b = band_rows(h, threads);
+  return (h + b - 1) / b;
+}
+
+// Static per-load tables.
+//   cjump[y * wpr + j]: z^n mod P, n = fluid cells of the row band before
+//                       word (y, j) in row-major order.
+//   row_band[y]:        the band (collision stream) of row y.
+//   ijump[y * iw + j]:  z^n mod P, n = draws of the inflow stream before the
+//                       first inlet cell of word (y, j).
+// `solid` is the solid plane, wpr words per row; tail bits must be zero.
+inline void build_tables(uint32_t w, uint32_t h, uint32_t bands, uint32_t inlet_cols, bool rest,
+                         const std::vector<uint64_t>& solid, std::vector<uint64_t>& cjump,
+                         std::vector<uint32_t>& row_band, std::vector<uint64_t>& ijump) {
+  const uint64_t p = charpoly_low();
+  const uint32_t wpr = (w + 63) / 64;
+  const uint32_t band = band_rows(h, bands);
+  cjump.assign((size_t)wpr * h, 0);
+  row_band.assign(h, 0);
+  uint64_t c = 1;
+  for (uint32_t y = 0; y < h; ++y) {
+    if (y % band == 0) c = 1;
+    row_band[y] = y / band;
+    for (uint32_t j = 0; j < wpr; ++j) {
+      size_t word = (size_t)y * wpr + j;
+      cjump[word] = c;
+      uint32_t ncell = (j + 1) * 64 <= w ? 64 : w - j * 64;
+      uint64_t cells = ncell == 64 ? ~0ull : ((1ull << ncell) - 1);
+      uint32_t fluid = (uint32_t)__builtin_popcountll(cells & ~solid[word]);
+      c = poly_mulz(c, fluid, p);
+    }
+  }
+  const uint32_t cols = inlet_cols < w ? inlet_cols : w;
+  const uint32_t iw = (cols + 63) / 64;
+  const uint32_t per = rest ? 4 : 3;
+  ijump.assign((size_t)iw * h, 0);
+  c = 1;
+  for (uint32_t y = 0; y < h; ++y)
+    for (uint32_t j = 0; j < iw; ++j) {
+      ijump[(size_t)y * iw + j] = c;
+      uint32_t span = cols - j * 64;
+      uint64_t m = span >= 64 ? ~0ull : ((1ull << span) - 1);
+      uint32_t fluid = (uint32_t)__builtin_popcountll(m & ~solid[(size_t)y * wpr + j]);
+      c = poly_mulz(c, fluid * per, p);
+    }
+}
+
+}  // namespace lgca

@@ -1,0 +1,259 @@
+"""Python host side of the MI355X FHP engine, over the C ABI of include/lgca.h.
+
+``GpuLattice`` mirrors the reference's ``GpuLattice`` (src/gpu.rs:453-755)
+method for method, and ``Sim`` mirrors ``Sim`` (src/sim.rs:19-175) for its GPU
+backend. Every call goes to liblgca_hip.so; there is no CPU fallback in this
+module: if the library or a HIP device is missing, construction raises.
+
+Cells are one byte per site in the reference's layout (src/hex.rs:7-23):
+bits 0-5 moving directions, bit 6 rest, bit 7 solid, row-major ``y * w + x``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblgca_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "lgca.h")
+
+_lib = None
+
+
+class LgcaError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """Load liblgca_hip.so and declare its C signatures (no device calls)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise LgcaError(f"{path} is missing: build it with lattice-gas_amd/build.py")
+    L = C.CDLL(path)
+    vp, sz, u64, f32, i32 = C.c_void_p, C.c_size_t, C.c_uint64, C.c_float, C.c_int
+    P = C.POINTER
+    sig = {
+        "lg_create": (vp, [sz, sz, i32, u64, sz]),
+        "lg_destroy": (None, [vp]),
+        "lg_last_error": (C.c_char_p, []),
+        "lg_device_name": (i32, [vp, C.c_char_p, sz]),
+        "lg_set_inlet": (i32, [vp, sz, P(C.c_uint32)]),
+        "lg_set_inlet_flow": (i32, [vp, sz, f32, f32, f32]),
+        "lg_attach_field": (i32, [vp, sz, sz]),
+        "lg_load": (i32, [vp, P(C.c_uint8), sz]),
+        "lg_restart": (i32, [vp, P(C.c_uint8), sz, u64]),
+        "lg_store": (i32, [vp, P(C.c_uint8), sz]),
+        "lg_advance": (i32, [vp, u64]),
+        "lg_advance_sampling": (i32, [vp, u64, u64, f32]),
+        "lg_sample_field": (i32, [vp, f32]),
+        "lg_read_field": (i32, [vp, P(f32), P(f32), P(f32), P(f32), sz]),
+        "lg_read_block_sums": (i32, [vp] + [P(C.c_int64)] * 4 + [sz]),
+        "lg_total_particles": (i32, [vp, P(u64)]),
+        "lg_dims": (i32, [vp, P(sz), P(sz)]),
+        "lg_steps": (u64, [vp]),
+        "lg_set_steps": (i32, [vp, u64]),
+        "lg_advance_timed": (i32, [vp, u64, P(f32)]),
+        "lg_advance_sampling_timed": (i32, [vp, u64, u64, f32, P(f32)]),
+        "lg_debug_collide_table": (i32, [i32, P(C.c_uint8), sz]),
+        "lg_add_solid_box": (i32, [vp, f32, f32, f32, f32]),
+        "lg_add_solid_disc": (i32, [vp, f32, f32, f32]),
+        "lg_init_equilibrium": (i32, [vp, f32, f32, f32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc < 0:
+        raise LgcaError(load_library().lg_last_error().decode())
+    return rc
+
+
+def _u8(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+class GpuLattice:
+    """src/gpu.rs:453-755, exact with the CPU path for the given band count."""
+
+    def __init__(self, w: int, h: int, rest_particles: bool, seed: int, bands: int = 1):
+        L = load_library()
+        self._p = L.lg_create(int(w), int(h), int(bool(rest_particles)), int(seed) & (2**64 - 1),
+                              max(int(bands), 1))
+        if not self._p:
+            raise LgcaError(L.lg_last_error().decode())
+        self.w, self.h = int(w), int(h)
+        self.rest = bool(rest_particles)
+        self.bands = max(int(bands), 1)
+        self._field = None
+
+    def close(self):
+        p = getattr(self, "_p", None)
+        if p:
+            load_library().lg_destroy(p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # src/gpu.rs:517-519
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        _check(load_library().lg_device_name(self._p, buf, 256))
+        return buf.value.decode()
+
+    # src/gpu.rs:525-528
+    def set_inlet(self, cols: int, density: float, inflow, rest: bool | None = None):
+        if rest is not None and bool(rest) != self.rest:
+            raise LgcaError("rest-particle setting differs from the lattice's")
+        _check(load_library().lg_set_inlet_flow(self._p, int(cols), float(density),
+                                                float(inflow[0]), float(inflow[1])))
+
+    def set_inlet_thresholds(self, cols: int, thresh7):
+        t = np.ascontiguousarray(thresh7, dtype=np.uint32)
+        _check(load_library().lg_set_inlet(self._p, int(cols), t.ctypes.data_as(C.POINTER(C.c_uint32))))
+
+    # src/gpu.rs:532-545
+    def attach_field(self, bx: int, by: int):
+        _check(load_library().lg_attach_field(self._p, int(bx), int(by)))
+        self._field = (int(bx), int(by), self.w // int(bx), self.h // int(by))
+
+    # src/lattice.rs:210-220 (box / disc obstacles), on the device
+    def add_solid_box(self, x0, x1, y0, y1):
+        _check(load_library().lg_add_solid_box(self._p, float(x0), float(x1), float(y0), float(y1)))
+
+    def add_solid_disc(self, cx, cy, r):
+        _check(load_library().lg_add_solid_disc(self._p, float(cx), float(cy), float(r)))
+
+    # src/lattice.rs:197-208, on the device
+    def init_equilibrium(self, density, ux, uy):
+        _check(load_library().lg_init_equilibrium(self._p, float(density), float(ux), float(uy)))
+
+    # src/gpu.rs:572-610
+    def load(self, cells: np.ndarray):
+        a = np.ascontiguousarray(cells, dtype=np.uint8).ravel()
+        _check(load_library().lg_load(self._p, _u8(a), a.size))
+
+    # src/gpu.rs:616-620
+    def restart(self, cells: np.ndarray, seed: int):
+        a = np.ascontiguousarray(cells, dtype=np.uint8).ravel()
+        _check(load_library().lg_restart(self._p, _u8(a), a.size, int(seed) & (2**64 - 1)))
+
+    # src/gpu.rs:630-652
+    def store(self, out: np.ndarray | None = None) -> np.ndarray:
+        if out is None:
+            out = np.empty(self.w * self.h, dtype=np.uint8)
+        _check(load_library().lg_store(self._p, _u8(out), out.size))
+        return out
+
+    # src/gpu.rs:657-659
+    def advance(self, n: int):
+        _check(load_library().lg_advance(self._p, int(n)))
+
+    # src/gpu.rs:668-706
+    def advance_sampling(self, n: int, sample_every: int, alpha: float):
+        _check(load_library().lg_advance_sampling(self._p, int(n), int(sample_every), float(alpha)))
+
+    def advance_timed(self, n: int, sample_every: int = 0, alpha: float = 0.0) -> float:
+        """Advance and return the device milliseconds between HIP events
+        recorded on the engine's stream around the step launches."""
+        ms = C.c_float()
+        _check(load_library().lg_advance_sampling_timed(self._p, int(n), int(sample_every),
+                                                        float(alpha), C.byref(ms)))
+        return ms.value
+
+    # src/gpu.rs:710-721
+    def sample_field(self, alpha: float):
+        _check(load_library().lg_sample_field(self._p, float(alpha)))
+
+    def _nblocks(self):
+        if self._field is None:
+            raise LgcaError("no field attached to this lattice")
+        return self._field[2] * self._field[3]
+
+    # src/gpu.rs:725-736 (returns the four arrays instead of filling a Field)
+    def read_field(self):
+        n = self._nblocks()
+        arrs = [np.zeros(n, dtype=np.float32) for _ in range(4)]
+        _check(load_library().lg_read_field(self._p, *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in arrs], n))
+        return tuple(arrs)
+
+    def read_block_sums(self):
+        n = self._nblocks()
+        arrs = [np.zeros(n, dtype=np.int64) for _ in range(4)]
+        _check(load_library().lg_read_block_sums(self._p, *[a.ctypes.data_as(C.POINTER(C.c_int64)) for a in arrs], n))
+        return tuple(arrs)
+
+    # src/gpu.rs:740-754
+    def total_particles(self) -> int:
+        v = C.c_uint64()
+        _check(load_library().lg_total_particles(self._p, C.byref(v)))
+        return int(v.value)
+
+    @property
+    def steps(self) -> int:
+        return int(load_library().lg_steps(self._p))
+
+    @steps.setter
+    def steps(self, v: int):
+        _check(load_library().lg_set_steps(self._p, int(v)))
+
+
+def collide_table_on_device(rest: bool) -> np.ndarray:
+    """The generated circuit evaluated on the GPU for every (state, k)."""
+    ns, stride = (128, 30) if rest else (64, 6)
+    out = np.zeros(ns * stride, dtype=np.uint8)
+    _check(load_library().lg_debug_collide_table(int(rest), _u8(out), out.size))
+    return out.reshape(ns, stride)
+
+
+class Sim:
+    """src/sim.rs:19-175 with the GPU backend only.
+
+    Built from the same arguments the reference's ``Sim::new`` reads off a
+    prepared ``Lattice`` (src/sim.rs:60-67): size, rest flag, seed and
+    thread count, the inflow, and the initial cells."""
+
+    def __init__(self, w, h, rest, seed, threads, inlet_cols, density, inflow, cells):
+        self.gpu = GpuLattice(w, h, rest, seed, threads)
+        self.gpu.set_inlet(inlet_cols, density, inflow)
+        self.gpu.load(cells)
+        self.w, self.h = w, h
+        self._field = None
+
+    def attach_field(self, bx, by):                     # src/sim.rs:80-86
+        self.gpu.attach_field(bx, by)
+
+    def restart(self, cells, seed):                     # src/sim.rs:91-97
+        self.gpu.restart(cells, seed)
+
+    def cells(self) -> np.ndarray:                      # src/sim.rs:102-114
+        return self.gpu.store()
+
+    def advance_sampling(self, steps, sample_every, alpha):   # src/sim.rs:119-143
+        self.gpu.advance_sampling(steps, max(int(sample_every), 1), alpha)
+        return self.gpu.read_field()
+
+    def sample(self, alpha):                            # src/sim.rs:148-157
+        self.gpu.sample_field(alpha)
+        return self.gpu.read_field()
+
+    def advance(self, steps):                           # src/sim.rs:159-165
+        self.gpu.advance(steps)
+
+    def total_particles(self) -> int:                   # src/sim.rs:168-174
+        return self.gpu.total_particles()

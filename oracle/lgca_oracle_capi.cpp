@@ -24,6 +24,12 @@ void orc_add_solid_box(void* p, float x0, float x1, float y0, float y1) {
 void orc_add_solid_disc(void* p, float cx, float cy, float r) {
   static_cast<Lattice*>(p)->add_solid_disc(cx, cy, r);
 }
+// The public `inflow` field of Lattice (src/lattice.rs:18), for workloads
+// that initialise at one velocity and drive the inflow at another.
+void orc_set_inflow(void* p, float ux, float uy) {
+  static_cast<Lattice*>(p)->inflow_x = ux;
+  static_cast<Lattice*>(p)->inflow_y = uy;
+}
 void orc_init_equilibrium(void* p) { static_cast<Lattice*>(p)->init_equilibrium(); }
 int orc_restart(void* p, const uint8_t* cells, size_t n, uint64_t seed) {
   auto* L = static_cast<Lattice*>(p);

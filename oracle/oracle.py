@@ -44,6 +44,7 @@ def lib():
             "orc_add_solid_box": (None, [vp, f32, f32, f32, f32]),
             "orc_add_solid_disc": (None, [vp, f32, f32, f32]),
             "orc_init_equilibrium": (None, [vp]),
+            "orc_set_inflow": (None, [vp, f32, f32]),
             "orc_restart": (i32, [vp, P(C.c_uint8), sz, u64]),
             "orc_advance": (None, [vp, u64]),
             "orc_steps": (u64, [vp]),
@@ -181,6 +182,10 @@ class Lattice:
 
     def init_equilibrium(self):
         lib().orc_init_equilibrium(self._p)
+
+    def set_inflow(self, ux, uy):
+        lib().orc_set_inflow(self._p, ux, uy)
+        self.inflow = (float(ux), float(uy))
 
     def restart(self, cells: np.ndarray, seed: int):
         a = np.ascontiguousarray(cells, dtype=np.uint8).ravel()

@@ -1,0 +1,136 @@
+// Test harness: the HIP engine's per-word step (lgca_device.h), compiled for
+// the host and driven over a whole lattice, so CPU tests can compare the
+// exact device algorithm with the oracle without a GPU. Not part of the
+// product: nothing outside tests/ loads it.
+#include <cstring>
+#include <vector>
+
+#include "../../lattice-gas_amd/csrc/lgca_device.h"
+#include "../../lattice-gas_amd/csrc/lgca_jump.hpp"
+
+namespace {
+struct Sim {
+  LgGeom G;
+  uint64_t seed, steps = 0;
+  uint32_t bands;
+  std::vector<uint64_t> a, b, solid, cjump, ijump, basis;
+  std::vector<uint32_t> row_band;
+};
+}  // namespace
+
+extern "C" {
+
+void* hs_new(uint32_t w, uint32_t h, int rest, uint64_t seed, uint32_t bands, uint32_t inlet_cols,
+             const uint32_t* thresh7) {
+  auto* S = new Sim();
+  LgGeom& G = S->G;
+  std::memset(&G, 0, sizeof(G));
+  G.w = w;
+  G.h = h;
+  G.wpr = (w + 63) / 64;
+  G.total = G.wpr * h;
+  G.lastword = (w - 1) / 64;
+  G.lastbit = (w - 1) % 64;
+  G.tailmask = G.lastbit == 63 ? ~0ull : ((1ull << (G.lastbit + 1)) - 1);
+  G.inlet_cols = inlet_cols < w ? inlet_cols : w;
+  G.iw = (G.inlet_cols + 63) / 64;
+  G.rest = rest ? 1u : 0u;
+  G.nbands = lgca::n_bands(h, bands);
+  for (int d = 0; d < 7; ++d) G.thresh[d] = thresh7 ? thresh7[d] : 0;
+  S->seed = seed;
+  S->bands = bands;
+  S->a.assign((size_t)7 * G.total, 0);
+  S->b.assign((size_t)7 * G.total, 0);
+  S->solid.assign(G.total, 0);
+  S->basis.assign((size_t)(G.nbands + 1) * 64, 0);
+  return S;
+}
+
+void hs_free(void* p) { delete static_cast<Sim*>(p); }
+
+void hs_load(void* p, const uint8_t* cells) {
+  auto* S = static_cast<Sim*>(p);
+  LgGeom& G = S->G;
+  std::fill(S->a.begin(), S->a.end(), 0);
+  std::fill(S->solid.begin(), S->solid.end(), 0);
+  for (uint32_t y = 0; y < G.h; ++y)
+    for (uint32_t x = 0; x < G.w; ++x) {
+      uint8_t c = cells[(size_t)y * G.w + x];
+      size_t word = (size_t)y * G.wpr + x / 64;
+      uint64_t bit = 1ull << (x % 64);
+      for (int d = 0; d < 7; ++d)
+        if (c & (1u << d)) S->a[(size_t)d * G.total + word] |= bit;
+      if (c & 0x80u) S->solid[word] |= bit;
+    }
+  lgca::build_tables(G.w, G.h, S->bands, G.inlet_cols, G.rest != 0, S->solid, S->cjump,
+                     S->row_band, S->ijump);
+}
+
+void hs_store(void* p, uint8_t* cells) {
+  auto* S = static_cast<Sim*>(p);
+  LgGeom& G = S->G;
+  for (uint32_t y = 0; y < G.h; ++y)
+    for (uint32_t x = 0; x < G.w; ++x) {
+      size_t word = (size_t)y * G.wpr + x / 64;
+      uint32_t k = x % 64;
+      uint8_t c = 0;
+      for (int d = 0; d < 7; ++d) c |= (uint8_t)(((S->a[(size_t)d * G.total + word] >> k) & 1u) << d);
+      c |= (uint8_t)(((S->solid[word] >> k) & 1u) << 7);
+      cells[(size_t)y * G.w + x] = c;
+    }
+}
+
+void hs_advance(void* p, uint64_t n) {
+  auto* S = static_cast<Sim*>(p);
+  LgGeom& G = S->G;
+  for (uint64_t it = 0; it < n; ++it) {
+    uint64_t t = S->steps;
+    for (uint32_t s = 0; s <= G.nbands; ++s) {
+      uint64_t st = lg_rng_init(s < G.nbands ? lg_band_seed(S->seed, t, s)
+                                             : lg_inlet_seed(S->seed, t + 1));
+      for (int i = 0; i < 64; ++i) {
+        S->basis[(size_t)s * 64 + i] = st;
+        st = lg_xs(st);
+      }
+    }
+    for (uint32_t y = 0; y < G.h; ++y)
+      for (uint32_t j = 0; j < G.wpr; ++j)
+        lg_step_word(G, S->a.data(), S->b.data(), S->solid.data(), S->cjump.data(),
+                     S->row_band.data(), S->ijump.data(), S->basis.data(), y, j);
+    std::swap(S->a, S->b);
+    S->steps += 1;
+  }
+}
+
+// Exhaustive check of the generated circuit: for every state and every k,
+// the collided state (stride 30 with rest, 6 without) into out[s * stride + k].
+void hs_collide_table(int rest, uint8_t* out) {
+  const int ns = rest ? 128 : 64;
+  const int stride = rest ? LG_COLLIDE_REST_STRIDE : LG_COLLIDE_NOREST_STRIDE;
+  for (int s = 0; s < ns; ++s)
+    for (int k = 0; k < stride; ++k) {
+      uint64_t n[7], o[7];
+      for (int d = 0; d < 7; ++d) n[d] = ((s >> d) & 1) ? ~0ull : 0ull;
+      uint8_t r = 0;
+      if (rest) {
+        uint64_t K[LG_COLLIDE_REST_KBITS];
+        for (int b = 0; b < LG_COLLIDE_REST_KBITS; ++b) K[b] = ((k >> b) & 1) ? ~0ull : 0ull;
+        lg_collide_rest(n, K, o);
+      } else {
+        uint64_t K[LG_COLLIDE_NOREST_KBITS];
+        for (int b = 0; b < LG_COLLIDE_NOREST_KBITS; ++b) K[b] = ((k >> b) & 1) ? ~0ull : 0ull;
+        lg_collide_norest(n, K, o);
+      }
+      int bad = 0;
+      for (int d = 0; d < 7; ++d) {
+        if (o[d] == ~0ull) r |= (uint8_t)(1u << d);
+        else if (o[d] != 0) bad = 1;
+      }
+      out[s * stride + k] = bad ? 0xFF : r;
+    }
+}
+
+uint64_t hs_charpoly(void) { return lgca::charpoly_low(); }
+uint64_t hs_jump_state(uint64_t s, uint64_t n) { return lgca::jump_state(s, n); }
+
+}  // extern "C"
