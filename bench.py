@@ -94,7 +94,22 @@ def main():
     ap.add_argument("--bands", type=int, default=0, help="CPU bands (default: host threads, <= 16)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run a plain device copy of MB megabytes (to scale rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20)
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n, "launches": 5}), flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this engine runs on exactly one GPU; --gpus {args.gpus} is not "

@@ -19,52 +19,44 @@
 #define LG_HD
 #endif
 
-typedef uint64_t lg_word;
+#if defined(__HIP_DEVICE_COMPILE__)
+#define LG_UNROLL _Pragma("unroll")
+#else
+#define LG_UNROLL
+#endif
 
 // ---------------------------------------------------------------------------
-// Three-input lookup: result bit = tt[(a << 2) | (b << 1) | c]. On gfx950 this
-// is v_bitop3_b32 (src0 <-> 0xF0, src1 <-> 0xCC, src2 <-> 0xAA), once per half.
+// Three-input lookup: result bit = TT[(a << 2) | (b << 1) | c]. On gfx950 this
+// is v_bitop3_b32 (src0 <-> 0xF0, src1 <-> 0xCC, src2 <-> 0xAA).
 // ---------------------------------------------------------------------------
 template <uint32_t TT>
-LG_HD inline uint64_t lg_op3(uint64_t a, uint64_t b, uint64_t c) {
+LG_HD inline uint32_t lg_op3(uint32_t a, uint32_t b, uint32_t c) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, (uint32_t)c, TT);
-  uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32),
-                                            (uint32_t)(c >> 32), TT);
-  return ((uint64_t)hi << 32) | lo;
+  return __builtin_amdgcn_bitop3_b32(a, b, c, TT);
 #else
-  uint64_t r = 0;
+  uint32_t r = 0;
   for (uint32_t idx = 0; idx < 8; ++idx)
     if ((TT >> idx) & 1u)
       r |= ((idx & 4) ? a : ~a) & ((idx & 2) ? b : ~b) & ((idx & 1) ? c : ~c);
   return r;
 #endif
 }
-#define LG_OP3(a, b, c, tt) lg_op3<(tt)>((a), (b), (c))
+template <uint32_t TT>
+LG_HD inline uint64_t lg_op3(uint64_t a, uint64_t b, uint64_t c) {
+  return ((uint64_t)lg_op3<TT>((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32)) << 32) |
+         lg_op3<TT>((uint32_t)a, (uint32_t)b, (uint32_t)c);
+}
 
 #include "lgca_collide.gen.h"
 
 // ---------------------------------------------------------------------------
-// Geometry constants (src/hex.rs:36-41)
-// ---------------------------------------------------------------------------
-#define LG_OPP(d) (((d) + 3) % 6)
-// Source column offset of the gather for direction d at a destination row of
-// the given parity: DX_{EVEN,ODD}[OPP[d]] (src/lattice.rs:333-342).
-LG_HD inline int lg_src_dx(int d, uint32_t odd) {
-  const int DXE[6] = {1, 0, -1, -1, -1, 0};
-  const int DXO[6] = {1, 1, 0, -1, 0, 1};
-  return odd ? DXO[LG_OPP(d)] : DXE[LG_OPP(d)];
-}
-// Source row offset DY[OPP[d]]
-LG_HD inline int lg_src_dy(int d) {
-  const int DY[6] = {0, 1, 1, 0, -1, -1};
-  return DY[LG_OPP(d)];
-}
-
-// ---------------------------------------------------------------------------
-// xorshift64* (src/rng.rs:8-29)
+// xorshift64* (src/rng.rs:8-29) and the per-step stream seeds
+// (src/lattice.rs:233,242,276)
 // ---------------------------------------------------------------------------
 #define LG_RNG_MUL 0x2545F4914F6CDD1Dull
+#define LG_STEP_MIX 0x9E3779B97F4A7C15ull
+#define LG_INLET_MIX 0xD1B54A32D192ED03ull
+
 LG_HD inline uint64_t lg_rng_init(uint64_t seed) {
   uint64_t x = seed ^ 0x9E3779B97F4A7C15ull;
   x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -79,10 +71,10 @@ LG_HD inline uint64_t lg_xs(uint64_t x) {
 }
 // next_u32 of the state just produced: the high half of x * M.
 LG_HD inline uint32_t lg_out32(uint64_t x) { return (uint32_t)((x * LG_RNG_MUL) >> 32); }
-
-// Per-step stream seeds (src/lattice.rs:233,242,276).
-#define LG_STEP_MIX 0x9E3779B97F4A7C15ull
-#define LG_INLET_MIX 0xD1B54A32D192ED03ull
+// The collision table's column k = (r * stride) >> 32 (src/collision.rs:115-119).
+LG_HD inline uint32_t lg_kcol(uint32_t r, uint32_t stride) {
+  return (uint32_t)(((uint64_t)r * stride) >> 32);
+}
 LG_HD inline uint64_t lg_band_seed(uint64_t seed, uint64_t t, uint32_t band) {
   return (seed ^ (t * LG_STEP_MIX)) ^ ((uint64_t)(band + 1) << 32);
 }
@@ -92,15 +84,40 @@ LG_HD inline uint64_t lg_inlet_seed(uint64_t seed, uint64_t t_after) {
 
 // Jump-ahead: the state n draws into a stream whose start is s is
 // T^n s = sum_i c_i T^i s with c(z) = z^n mod P(z), P the characteristic
-// polynomial of the xorshift matrix T. B[i] = T^i s is the stream's basis,
-// computed once per stream per step; c is static per word.
+// polynomial of the xorshift matrix T; c is static per word.
+//
+// lg_jump: against the 64 states B[i] = T^i s.
 LG_HD inline uint64_t lg_jump(uint64_t c, const uint64_t* B) {
   uint64_t st = 0;
-#if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll
-#endif
   for (int i = 0; i < 64; ++i) st ^= B[i] & (0ull - ((c >> i) & 1ull));
   return st;
+}
+// lg_jump_tab: against 4-bit window tables, tab[g * 16 + v] = XOR of
+// T^(4g + b) s over the set bits b of v: 16 lookups instead of 64 terms.
+#define LG_WTAB 256  // entries per stream per step
+LG_HD inline uint64_t lg_jump_tab(uint64_t c, const uint64_t* __restrict tab) {
+  uint64_t a = 0, b = 0;
+  LG_UNROLL
+  for (int g = 0; g < 16; g += 2) {
+    a ^= tab[g * 16 + (uint32_t)((c >> (4 * g)) & 15u)];
+    b ^= tab[(g + 1) * 16 + (uint32_t)((c >> (4 * g + 4)) & 15u)];
+  }
+  return a ^ b;
+}
+// Fill one stream's window tables from its seeded start state.
+LG_HD inline void lg_build_tab(uint64_t s, uint64_t* __restrict tab) {
+  uint64_t B[64];
+  for (int i = 0; i < 64; ++i) {
+    B[i] = s;
+    s = lg_xs(s);
+  }
+  for (int g = 0; g < 16; ++g)
+    for (uint32_t v = 0; v < 16; ++v) {
+      uint64_t e = 0;
+      for (int b = 0; b < 4; ++b)
+        if ((v >> b) & 1u) e ^= B[4 * g + b];
+      tab[g * 16 + v] = e;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -118,94 +135,75 @@ struct LgGeom {
   uint32_t thresh[7];         // inflow thresholds (src/lattice.rs:87-92)
 };
 
-// Gather plane d for destination word (y, j): bit x takes the bit of the
-// source site (x + dx, y + dy), periodic in x and y (src/lattice.rs:345-356).
-LG_HD inline uint64_t lg_gather(const LgGeom& G, const uint64_t* __restrict plane, uint32_t y,
-                                uint32_t j, int dx, int dy) {
-  int sy = (int)y + dy;
-  if (sy < 0) sy += (int)G.h;
-  if (sy >= (int)G.h) sy -= (int)G.h;
-  const uint64_t* row = plane + (size_t)sy * G.wpr;
-  uint64_t w0 = row[j];
-  if (dx == 0) return w0;
-  if (dx > 0) {  // bit x takes bit x + 1
-    uint64_t hi = (j + 1 < G.wpr) ? row[j + 1] : 0ull;
-    uint64_t v = (w0 >> 1) | (hi << 63);
-    if (j == G.lastword)  // the last cell takes cell 0 of the row
-      v = (v & ~(1ull << G.lastbit)) | ((row[0] & 1ull) << G.lastbit);
-    return v;
-  }
-  // bit x takes bit x - 1
-  uint64_t lo = (j > 0) ? row[j - 1] : 0ull;
-  uint64_t v = (w0 << 1) | (lo >> 63);
-  if (j == 0) v = (v & ~1ull) | ((row[G.lastword] >> G.lastbit) & 1ull);
-  return v;
-}
-
-// Deposit the binary digits of k (k < 32) for cell x into the digit planes:
-// the digit word is rotated so digit b lands on bit (x + b) % 32 of a 32-bit
-// accumulator per digit; lg_k_finish rotates each accumulator back by b.
+// ---------------------------------------------------------------------------
+// Collision draws for one word: the binary digits of k for the 64 cells.
+// ---------------------------------------------------------------------------
+// Fast path (every cell draws). Cell x's digits are OR-ed into accumulator
+// x % 8 at bits x..x+4 (rotated past bit 31), one shift-or per cell; the
+// digit planes are then gathered back with a rotate and a mask per
+// accumulator. Cells sharing an accumulator are 8 apart, so their 5-bit
+// windows never overlap.
 template <int NK>
-LG_HD inline void lg_k_deposit(uint32_t acc[NK], uint32_t k, uint32_t x) {
-  uint32_t xr = x & 31u;
-  uint32_t rk = (k << xr) | (xr ? (k >> (32u - xr)) : 0u);
-#if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll
-#endif
+LG_HD inline uint64_t lg_draw_half(uint64_t st, uint32_t stride, uint32_t P[NK]) {
+  uint32_t R[8];
+  LG_UNROLL
+  for (int m = 0; m < 8; ++m) R[m] = 0;
+  LG_UNROLL
+  for (uint32_t x = 0; x < 32; ++x) {
+    st = lg_xs(st);
+    uint32_t k = lg_kcol(lg_out32(st), stride);
+    if (x + NK <= 32)
+      R[x & 7] |= k << x;
+    else
+      R[x & 7] |= (k << x) | (k >> (32 - x));
+  }
+  LG_UNROLL
   for (int b = 0; b < NK; ++b) {
-    uint32_t m = 1u << ((xr + (uint32_t)b) & 31u);
-    acc[b] = (rk & m) | (acc[b] & ~m);
+    uint32_t acc = 0;
+    LG_UNROLL
+    for (int m = 0; m < 8; ++m) {
+      uint32_t r = b ? ((R[m] >> b) | (R[m] << (32 - b))) : R[m];
+      acc |= r & (0x01010101u << m);
+    }
+    P[b] = acc;
   }
-}
-template <int NK>
-LG_HD inline uint32_t lg_k_finish_one(uint32_t a, int b) {
-  return b ? ((a >> b) | (a << (32 - b))) : a;
-}
-
-// Collision draws for one word: K[b] holds digit b of k for every cell that
-// draws (fluid cells inside the row), zeros elsewhere. `st` is the stream
-// state before the word's first draw; fm the cells that draw, in order.
-template <int NK>
-LG_HD inline uint64_t lg_draw_word(uint64_t st, uint64_t fm, uint32_t stride, uint64_t K[NK]) {
-  uint32_t lo[NK], hi[NK];
-  for (int b = 0; b < NK; ++b) lo[b] = hi[b] = 0;
-  if (fm == ~0ull) {
-#if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll
-#endif
-    for (uint32_t x = 0; x < 32; ++x) {
-      st = lg_xs(st);
-      uint32_t k = (uint32_t)(((uint64_t)lg_out32(st) * stride) >> 32);
-      lg_k_deposit<NK>(lo, k, x);
-    }
-#if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll
-#endif
-    for (uint32_t x = 32; x < 64; ++x) {
-      st = lg_xs(st);
-      uint32_t k = (uint32_t)(((uint64_t)lg_out32(st) * stride) >> 32);
-      lg_k_deposit<NK>(hi, k, x);
-    }
-  } else {
-    for (uint32_t x = 0; x < 64; ++x) {
-      uint32_t take = (uint32_t)((fm >> x) & 1ull);
-      uint64_t nx = lg_xs(st);
-      st = take ? nx : st;
-      uint32_t k = take ? (uint32_t)(((uint64_t)lg_out32(st) * stride) >> 32) : 0u;
-      if (x < 32)
-        lg_k_deposit<NK>(lo, k, x);
-      else
-        lg_k_deposit<NK>(hi, k, x);
-    }
-  }
-  for (int b = 0; b < NK; ++b)
-    K[b] = ((uint64_t)lg_k_finish_one<NK>(hi[b], b) << 32) | lg_k_finish_one<NK>(lo[b], b);
   return st;
 }
 
-// Inflow re-seed for the inlet cells `apply` of one word (src/lattice.rs:102-115,
-// 270-285): each applied cell takes 4 draws (3 without rest) in order, and
-// its seven bits replace res[] at that cell.
+// General path: only the cells in `fm` draw, in order (solid sites and the
+// cells past the row's end take no draw, src/lattice.rs:358-362).
+template <int NK>
+LG_HD inline void lg_draw_masked(uint64_t st, uint64_t fm, uint32_t stride, uint32_t lo[NK],
+                                 uint32_t hi[NK]) {
+  for (int b = 0; b < NK; ++b) lo[b] = hi[b] = 0;
+  for (uint32_t x = 0; x < 64; ++x) {
+    if (!((fm >> x) & 1ull)) continue;
+    st = lg_xs(st);
+    uint32_t k = lg_kcol(lg_out32(st), stride);
+    for (int b = 0; b < NK; ++b) {
+      uint32_t bit = (k >> b) & 1u;
+      if (x < 32)
+        lo[b] |= bit << x;
+      else
+        hi[b] |= bit << (x - 32);
+    }
+  }
+}
+
+template <int NK>
+LG_HD inline void lg_draw_word(uint64_t st, uint64_t fm, uint32_t stride, uint32_t lo[NK],
+                               uint32_t hi[NK]) {
+  if (fm == ~0ull) {
+    st = lg_draw_half<NK>(st, stride, lo);
+    lg_draw_half<NK>(st, stride, hi);
+  } else {
+    lg_draw_masked<NK>(st, fm, stride, lo, hi);
+  }
+}
+
+// Inflow re-seed / equilibrium sampling for the cells `apply` of one word
+// (src/lattice.rs:102-115, 197-208, 270-285): each applied cell takes 4 draws
+// (3 without rest) in order, and its seven bits replace res[] at that cell.
 LG_HD inline void lg_inlet_word(const LgGeom& G, uint64_t st, uint64_t apply, uint64_t res[7]) {
   uint64_t bits[7] = {0, 0, 0, 0, 0, 0, 0};
   for (uint32_t x = 0; x < 64; ++x) {
@@ -227,6 +225,25 @@ LG_HD inline void lg_inlet_word(const LgGeom& G, uint64_t st, uint64_t apply, ui
   for (int d = 0; d < 7; ++d) res[d] = (res[d] & ~apply) | bits[d];
 }
 
+// ---------------------------------------------------------------------------
+// Propagation. Bit d of site (x, y) comes from the site (x + dx, y + dy)
+// with (dx, dy) = (DX_{EVEN,ODD}, DY)[OPP[d]] for the destination row's
+// parity, periodic in x and y (src/lattice.rs:333-356):
+//   d:        0   1   2   3   4   5
+//   dy:       0  -1  -1   0  +1  +1
+//   dx even: -1  -1   0  +1   0  -1
+//   dx odd:  -1   0  +1  +1  +1   0
+// dx = -1: bit x takes bit x-1, with the carry from the word on the left
+// (the row's last cell for j = 0); dx = +1: bit x takes bit x+1, with the
+// carry from the word on the right (cell 0 for the last word).
+// ---------------------------------------------------------------------------
+LG_HD inline uint64_t lg_shl_in(uint64_t w0, uint64_t wl, uint32_t csh) {
+  return (w0 << 1) | ((wl >> csh) & 1ull);
+}
+LG_HD inline uint64_t lg_shr_in(uint64_t w0, uint64_t wr, uint32_t csh) {
+  return (w0 >> 1) | ((wr & 1ull) << csh);
+}
+
 // One word of the fused step: gather, exact-stream collision, bounce-back on
 // solid sites, inflow re-seed, tail mask (src/lattice.rs:228-285).
 //   src/dst:  7 planes each (ping-pong)
@@ -234,54 +251,96 @@ LG_HD inline void lg_inlet_word(const LgGeom& G, uint64_t st, uint64_t apply, ui
 //   cjump:    per-word collision jump polynomial
 //   row_band: collision stream of each row
 //   ijump:    per inlet word (y * iw + j) inlet jump polynomial
-//   basis:    (nbands + 1) x 64 stream bases for this step; the last is the
+//   tabs:     (nbands + 1) window tables for this step; the last is the
 //             inflow stream of the step's end
 LG_HD inline void lg_step_word(const LgGeom& G, const uint64_t* __restrict src,
                                uint64_t* __restrict dst, const uint64_t* __restrict solid,
                                const uint64_t* __restrict cjump, const uint32_t* __restrict row_band,
-                               const uint64_t* __restrict ijump, const uint64_t* __restrict basis,
+                               const uint64_t* __restrict ijump, const uint64_t* __restrict tabs,
                                uint32_t y, uint32_t j) {
   const size_t word = (size_t)y * G.wpr + j;
   const uint32_t odd = y & 1u;
-  uint64_t n[7];
-#if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll
-#endif
-  for (int d = 0; d < 6; ++d)
-    n[d] = lg_gather(G, src + (size_t)d * G.total, y, j, lg_src_dx(d, odd), lg_src_dy(d));
-  n[6] = src[(size_t)6 * G.total + word];
+  const size_t T = G.total;
+  const uint32_t ym = (y == 0) ? G.h - 1 : y - 1;
+  const uint32_t yp = (y + 1 == G.h) ? 0 : y + 1;
+  const uint32_t jl = (j == 0) ? G.lastword : j - 1;
+  const uint32_t jr = (j == G.lastword) ? 0 : j + 1;
+  const uint32_t cl = (j == 0) ? G.lastbit : 63u;         // carry bit taken from the left word
+  const uint32_t cr = (j == G.lastword) ? G.lastbit : 63u;  // where the right carry lands
+  const size_t r0 = (size_t)y * G.wpr, rm = (size_t)ym * G.wpr, rp = (size_t)yp * G.wpr;
 
+  // issue every load first
+  const uint64_t c = cjump[word];
+  const uint32_t band = row_band[y];
   const uint64_t sm = solid[word];
+  const uint64_t a0 = src[0 * T + r0 + j], a0l = src[0 * T + r0 + jl];
+  const uint64_t a1 = src[1 * T + rm + j], a1l = src[1 * T + rm + jl];
+  const uint64_t a2 = src[2 * T + rm + j], a2r = src[2 * T + rm + jr];
+  const uint64_t a3 = src[3 * T + r0 + j], a3r = src[3 * T + r0 + jr];
+  const uint64_t a4 = src[4 * T + rp + j], a4r = src[4 * T + rp + jr];
+  const uint64_t a5 = src[5 * T + rp + j], a5l = src[5 * T + rp + jl];
+  const uint64_t a6 = src[6 * T + word];
+
   const uint64_t tail = (j == G.lastword) ? G.tailmask : ~0ull;
   const uint64_t fm = ~sm & tail;
+  const uint64_t st = lg_jump_tab(c, tabs + (size_t)band * LG_WTAB);
 
-  uint64_t st = lg_jump(cjump[word], basis + (size_t)row_band[y] * 64);
+  uint64_t n[7];
+  n[0] = lg_shl_in(a0, a0l, cl);
+  n[1] = odd ? a1 : lg_shl_in(a1, a1l, cl);
+  n[2] = odd ? lg_shr_in(a2, a2r, cr) : a2;
+  n[3] = lg_shr_in(a3, a3r, cr);
+  n[4] = odd ? lg_shr_in(a4, a4r, cr) : a4;
+  n[5] = odd ? a5 : lg_shl_in(a5, a5l, cl);
+  n[6] = a6;
+
   uint64_t o[7];
   if (G.rest) {
-    uint64_t K[LG_COLLIDE_REST_KBITS];
-    lg_draw_word<LG_COLLIDE_REST_KBITS>(st, fm, LG_COLLIDE_REST_STRIDE, K);
-    lg_collide_rest(n, K, o);
+    uint32_t klo[LG_COLLIDE_REST_KBITS], khi[LG_COLLIDE_REST_KBITS];
+    lg_draw_word<LG_COLLIDE_REST_KBITS>(st, fm, LG_COLLIDE_REST_STRIDE, klo, khi);
+    uint32_t nh[7], oh[7];
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)n[d];
+    lg_collide_rest<uint32_t>(nh, klo, oh);
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) o[d] = oh[d];
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)(n[d] >> 32);
+    lg_collide_rest<uint32_t>(nh, khi, oh);
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) o[d] |= (uint64_t)oh[d] << 32;
   } else {
-    uint64_t K[LG_COLLIDE_NOREST_KBITS];
-    lg_draw_word<LG_COLLIDE_NOREST_KBITS>(st, fm, LG_COLLIDE_NOREST_STRIDE, K);
-    lg_collide_norest(n, K, o);
+    uint32_t klo[LG_COLLIDE_NOREST_KBITS], khi[LG_COLLIDE_NOREST_KBITS];
+    lg_draw_word<LG_COLLIDE_NOREST_KBITS>(st, fm, LG_COLLIDE_NOREST_STRIDE, klo, khi);
+    uint32_t nh[7], oh[7];
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)n[d];
+    lg_collide_norest<uint32_t>(nh, klo, oh);
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) o[d] = oh[d];
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)(n[d] >> 32);
+    lg_collide_norest<uint32_t>(nh, khi, oh);
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) o[d] |= (uint64_t)oh[d] << 32;
   }
 
   // A wall sends every particle back the way it came and never collides;
   // the rest particle stays (src/hex.rs:45-48, src/lattice.rs:358-359).
   uint64_t res[7];
-  for (int d = 0; d < 6; ++d) res[d] = (sm & n[LG_OPP(d)]) | (~sm & o[d]);
+  LG_UNROLL
+  for (int d = 0; d < 6; ++d) res[d] = (sm & n[(d + 3) % 6]) | (~sm & o[d]);
   res[6] = (sm & n[6]) | (~sm & o[6]);
 
   if (j < G.iw) {
-    uint32_t x0 = j * 64u;
-    uint32_t span = G.inlet_cols - x0;  // > 0 since j < iw
+    uint32_t span = G.inlet_cols - j * 64u;  // > 0 since j < iw
     uint64_t imask = span >= 64 ? ~0ull : ((1ull << span) - 1ull);
     uint64_t apply = imask & fm;
     if (apply) {
-      uint64_t ist = lg_jump(ijump[(size_t)y * G.iw + j], basis + (size_t)G.nbands * 64);
+      uint64_t ist = lg_jump_tab(ijump[(size_t)y * G.iw + j], tabs + (size_t)G.nbands * LG_WTAB);
       lg_inlet_word(G, ist, apply, res);
     }
   }
-  for (int d = 0; d < 7; ++d) dst[(size_t)d * G.total + word] = res[d] & tail;
+  LG_UNROLL
+  for (int d = 0; d < 7; ++d) dst[(size_t)d * T + word] = res[d] & tail;
 }

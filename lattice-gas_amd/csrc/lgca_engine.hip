@@ -5,7 +5,8 @@
 // jump each word's position in its CPU band stream, draw the 64 collision
 // indices, evaluate the generated collision circuit, bounce back on solid
 // sites, re-seed the inflow columns, mask the ragged tail. The per-step
-// stream bases come from lg_basis_kernel, one launch per batch of steps.
+// stream jump tables come from lg_tables_kernel, one launch per batch of
+// steps.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,22 +24,31 @@
 // Kernels
 // ---------------------------------------------------------------------------
 
-// Stream bases for `nsteps` consecutive steps from t0: for step t0 + k and
+// Stream tables for `nsteps` consecutive steps from t0: for step t0 + k and
 // stream s (s < nbands: collision band s of src/lattice.rs:242; s == nbands:
 // the inflow stream of src/lattice.rs:276, seeded with the incremented step)
-// write the states T^0 s0 .. T^63 s0 of the freshly seeded generator.
-__global__ void __launch_bounds__(256) lg_basis_kernel(uint64_t seed, uint64_t t0, uint32_t nsteps,
-                                                       uint32_t nbands, uint64_t* __restrict basis) {
+// write the 4-bit window tables of the freshly seeded generator. One thread
+// per (step, stream, window group g): it walks to T^(4g) s0 and fills the
+// 16 entries of its group.
+__global__ void __launch_bounds__(256) lg_tables_kernel(uint64_t seed, uint64_t t0, uint32_t nsteps,
+                                                        uint32_t nbands, uint64_t* __restrict tabs) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t per = nbands + 1;
-  if (g >= nsteps * per) return;
-  uint32_t k = g / per, s = g - k * per;
+  if (g >= nsteps * per * 16) return;
+  uint32_t grp = g & 15u, ks = g >> 4;
+  uint32_t k = ks / per, s = ks - k * per;
   uint64_t t = t0 + k;
   uint64_t st = lg_rng_init(s < nbands ? lg_band_seed(seed, t, s) : lg_inlet_seed(seed, t + 1));
-  uint64_t* out = basis + (size_t)g * 64;
-  for (int i = 0; i < 64; ++i) {
-    out[i] = st;
-    st = lg_xs(st);
+  for (uint32_t i = 0; i < 4 * grp; ++i) st = lg_xs(st);
+  uint64_t b0 = st, b1 = lg_xs(b0), b2 = lg_xs(b1), b3 = lg_xs(b2);
+  uint64_t* out = tabs + (size_t)ks * LG_WTAB + grp * 16;
+  for (uint32_t v = 0; v < 16; ++v) {
+    uint64_t e = 0;
+    if (v & 1u) e ^= b0;
+    if (v & 2u) e ^= b1;
+    if (v & 4u) e ^= b2;
+    if (v & 8u) e ^= b3;
+    out[v] = e;
   }
 }
 
@@ -52,7 +62,7 @@ __global__ void __launch_bounds__(256) lg_step_kernel(LgGeom G, const uint64_t* 
                                                       const uint64_t* __restrict cjump,
                                                       const uint32_t* __restrict row_band,
                                                       const uint64_t* __restrict ijump,
-                                                      const uint64_t* __restrict basis) {
+                                                      const uint64_t* __restrict tabs) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= G.total) return;
   uint32_t y, j;
@@ -66,7 +76,7 @@ __global__ void __launch_bounds__(256) lg_step_kernel(LgGeom G, const uint64_t* 
     y = q / per;
     j = G.iw + (q - y * per);
   }
-  lg_step_word(G, src, dst, solid, cjump, row_band, ijump, basis, y, j);
+  lg_step_word(G, src, dst, solid, cjump, row_band, ijump, tabs, y, j);
 }
 
 // Bytes -> planes (GpuLattice::load, src/gpu.rs:572-610), one thread per word.
@@ -241,11 +251,11 @@ __global__ void __launch_bounds__(256) lg_verify_kernel(int rest, uint8_t* __res
   if (rest) {
     uint64_t K[LG_COLLIDE_REST_KBITS];
     for (int b = 0; b < LG_COLLIDE_REST_KBITS; ++b) K[b] = ((k >> b) & 1) ? ~0ull : 0ull;
-    lg_collide_rest(n, K, o);
+    lg_collide_rest<uint64_t>(n, K, o);
   } else {
     uint64_t K[LG_COLLIDE_NOREST_KBITS];
     for (int b = 0; b < LG_COLLIDE_NOREST_KBITS; ++b) K[b] = ((k >> b) & 1) ? ~0ull : 0ull;
-    lg_collide_norest(n, K, o);
+    lg_collide_norest<uint64_t>(n, K, o);
   }
   int r = 0, bad = 0;
   for (int d = 0; d < 7; ++d) {
@@ -411,7 +421,7 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
   while (left > 0) {
     uint64_t chunk = std::min<uint64_t>(left, g->basis_steps);
     if (every > 0) chunk = std::min<uint64_t>(chunk, every - since);
-    hipLaunchKernelGGL(lg_basis_kernel, dim3(blocks_for((size_t)chunk * per)), dim3(256), 0,
+    hipLaunchKernelGGL(lg_tables_kernel, dim3(blocks_for((size_t)chunk * per * 16)), dim3(256), 0,
                        g->stream, g->seed, g->steps, (uint32_t)chunk, g->G.nbands, g->basis.p);
     HIP_TRY(hipGetLastError());
     if (timed && first) HIP_TRY(hipEventRecord(g->ev0, g->stream));
@@ -419,7 +429,7 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
     for (uint64_t k = 0; k < chunk; ++k) {
       hipLaunchKernelGGL(lg_step_kernel, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream,
                          g->G, g->a.p, g->b.p, g->solid.p, g->cjump.p, g->row_band.p, g->ijump.p,
-                         g->basis.p + (size_t)k * per * 64);
+                         g->basis.p + (size_t)k * per * LG_WTAB);
       std::swap(g->a.p, g->b.p);
       g->steps += 1;
     }
@@ -487,9 +497,10 @@ lg_t* lg_create(size_t w, size_t h, int rest, uint64_t seed, size_t bands) {
   G.nbands = lgca::n_bands(G.h, g->bands);
   g->seed = seed;
   const size_t per = G.nbands + 1;
-  g->basis_steps = (uint32_t)std::max<size_t>(1, std::min<size_t>(512, (64u << 20) / (per * 512)));
+  g->basis_steps = (uint32_t)std::max<size_t>(
+      1, std::min<size_t>(256, (32u << 20) / (per * LG_WTAB * 8)));
   if (g->a.alloc((size_t)7 * G.total) || g->b.alloc((size_t)7 * G.total) ||
-      g->solid.alloc(G.total) || g->basis.alloc((size_t)g->basis_steps * per * 64) ||
+      g->solid.alloc(G.total) || g->basis.alloc((size_t)g->basis_steps * per * LG_WTAB) ||
       g->counter.alloc(1))
     return bail("hipMalloc failed for the lattice planes");
   if (hipMemsetAsync(g->a.p, 0, g->a.n * 8, g->stream) ||

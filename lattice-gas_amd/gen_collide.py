@@ -428,11 +428,11 @@ def emit(rest: bool, fname: str):
         else:
             a3, tt3 = a, tt
         names[i] = f"t{i}"
-        lines.append(f"  const lg_word t{i} = LG_OP3({a3[0]}, {a3[1]}, {a3[2]}, 0x{tt3:02x});")
+        lines.append(f"  const W t{i} = lg_op3<0x{tt3:02x}>({a3[0]}, {a3[1]}, {a3[2]});")
     for d in range(7):
         lines.append(f"  o[{d}] = {names[outs[d]]};")
     body = "\n".join(lines)
-    sig = (f"LG_HD inline void {fname}(const lg_word n[7], const lg_word K[{nk}], lg_word o[7])")
+    sig = (f"template <typename W>\nLG_HD inline void {fname}(const W n[7], const W K[{nk}], W o[7])")
     return (f"// {len(order)} three-input nodes; stride {st}; k has {nk} binary digits.\n"
             f"{sig} {{\n{body}\n}}\n"), len(order), st, nk
 
@@ -449,7 +449,7 @@ def main():
         info.append((fname, nops, st, nk))
     hdr = ["// GENERATED by lattice-gas_amd/gen_collide.py -- do not edit.",
            "// The FHP collision rule of src/collision.rs:37-119 as bitplane code.",
-           "// Requires: lg_word, LG_HD and LG_OP3(a, b, c, table) from lgca_device.h.",
+           "// Requires LG_HD and lg_op3<table>(a, b, c) (32- or 64-bit words) from lgca_device.h.",
            "#pragma once", ""]
     for fname, nops, st, nk in info:
         hdr.append(f"#define {fname.upper()}_NODES {nops}")

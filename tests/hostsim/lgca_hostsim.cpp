@@ -42,7 +42,7 @@ void* hs_new(uint32_t w, uint32_t h, int rest, uint64_t seed, uint32_t bands, ui
   S->a.assign((size_t)7 * G.total, 0);
   S->b.assign((size_t)7 * G.total, 0);
   S->solid.assign(G.total, 0);
-  S->basis.assign((size_t)(G.nbands + 1) * 64, 0);
+  S->basis.assign((size_t)(G.nbands + 1) * LG_WTAB, 0);
   return S;
 }
 
@@ -88,10 +88,7 @@ void hs_advance(void* p, uint64_t n) {
     for (uint32_t s = 0; s <= G.nbands; ++s) {
       uint64_t st = lg_rng_init(s < G.nbands ? lg_band_seed(S->seed, t, s)
                                              : lg_inlet_seed(S->seed, t + 1));
-      for (int i = 0; i < 64; ++i) {
-        S->basis[(size_t)s * 64 + i] = st;
-        st = lg_xs(st);
-      }
+      lg_build_tab(st, S->basis.data() + (size_t)s * LG_WTAB);
     }
     for (uint32_t y = 0; y < G.h; ++y)
       for (uint32_t j = 0; j < G.wpr; ++j)
@@ -115,11 +112,11 @@ void hs_collide_table(int rest, uint8_t* out) {
       if (rest) {
         uint64_t K[LG_COLLIDE_REST_KBITS];
         for (int b = 0; b < LG_COLLIDE_REST_KBITS; ++b) K[b] = ((k >> b) & 1) ? ~0ull : 0ull;
-        lg_collide_rest(n, K, o);
+        lg_collide_rest<uint64_t>(n, K, o);
       } else {
         uint64_t K[LG_COLLIDE_NOREST_KBITS];
         for (int b = 0; b < LG_COLLIDE_NOREST_KBITS; ++b) K[b] = ((k >> b) & 1) ? ~0ull : 0ull;
-        lg_collide_norest(n, K, o);
+        lg_collide_norest<uint64_t>(n, K, o);
       }
       int bad = 0;
       for (int d = 0; d < 7; ++d) {

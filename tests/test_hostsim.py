@@ -1,0 +1,126 @@
+"""The device algorithm (lattice-gas_amd/csrc/lgca_device.h) compiled for the
+host and compared with the oracle, cell for cell, every step. This checks the
+GPU path's logic -- propagation shifts, parity, wrap, ragged tails, jump
+offsets into the band streams, the generated circuit, inflow order -- on the
+CPU; the gpu tests run the same source on the device."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import build_hostsim
+
+
+@pytest.fixture(scope="module")
+def hs():
+    L = C.CDLL(build_hostsim())
+    P = C.POINTER
+    L.hs_new.restype = C.c_void_p
+    L.hs_new.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, P(C.c_uint32)]
+    L.hs_free.argtypes = [C.c_void_p]
+    L.hs_load.argtypes = [C.c_void_p, P(C.c_uint8)]
+    L.hs_store.argtypes = [C.c_void_p, P(C.c_uint8)]
+    L.hs_advance.argtypes = [C.c_void_p, C.c_uint64]
+    L.hs_collide_table.argtypes = [C.c_int, P(C.c_uint8)]
+    L.hs_charpoly.restype = C.c_uint64
+    L.hs_jump_state.restype = C.c_uint64
+    L.hs_jump_state.argtypes = [C.c_uint64, C.c_uint64]
+    return L
+
+
+def u8p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+class HostSim:
+    def __init__(self, L, w, h, rest, seed, bands, inlet, thresh):
+        self.L, self.w, self.h = L, w, h
+        t = np.ascontiguousarray(thresh, dtype=np.uint32)
+        self.p = L.hs_new(w, h, int(rest), seed, bands, inlet, t.ctypes.data_as(C.POINTER(C.c_uint32)))
+
+    def load(self, cells):
+        a = np.ascontiguousarray(cells, dtype=np.uint8)
+        self.L.hs_load(self.p, u8p(a))
+
+    def store(self):
+        out = np.zeros(self.w * self.h, dtype=np.uint8)
+        self.L.hs_store(self.p, u8p(out))
+        return out
+
+    def advance(self, n):
+        self.L.hs_advance(self.p, n)
+
+    def __del__(self):
+        self.L.hs_free(self.p)
+
+
+@pytest.mark.parametrize("rest", [True, False])
+def test_circuit_matches_the_collision_table(oracle, hs, rest):
+    stride, table = oracle.collision_table(rest)
+    out = np.zeros(table.size, dtype=np.uint8)
+    hs.hs_collide_table(int(rest), u8p(out))
+    assert np.array_equal(out, table.ravel())
+
+
+def test_charpoly_is_primitive_degree_64(hs):
+    """z^(2^64-1) == 1 mod P and z^((2^64-1)/q) != 1 for the prime factors q of
+    2^64-1 (xorshift64's full period), evaluated through jump_state: T^n s = s
+    exactly when n is a multiple of the period."""
+    s = 0x0123456789ABCDEF
+    period = (1 << 64) - 1
+    assert hs.hs_charpoly() != 0
+    assert hs.hs_jump_state(s, period) == s
+    for q in (3, 5, 17, 257, 641, 65537, 6700417):
+        assert hs.hs_jump_state(s, period // q) != s
+
+
+def test_jump_equals_sequential_steps(oracle, hs):
+    M = (1 << 64) - 1
+    s0 = oracle.rng_state(0xABC)
+    s, ns = s0, 0
+    for n in (1, 2, 3, 63, 64, 65, 1000, 4096, 20000):
+        while ns < n:
+            s ^= (s << 13) & M
+            s ^= s >> 7
+            s ^= (s << 17) & M
+            ns += 1
+        assert hs.hs_jump_state(s0, n) == s
+    # composition for large offsets: T^(a+b) = T^a T^b
+    for a, b in ((2**40 + 5, 2**33 + 7), (123456789123, 987654321)):
+        assert hs.hs_jump_state(hs.hs_jump_state(s0, a), b) == hs.hs_jump_state(s0, a + b)
+
+
+def compare(oracle, hs, w, h, rest, bands, inlet, seed, steps, disc=None, box=None,
+            density=0.22, inflow=(0.3, 0.0)):
+    lat = oracle.Lattice(w, h, density, inflow, inlet, rest, bands, seed)
+    if disc:
+        lat.add_solid_disc(*disc)
+    if box:
+        lat.add_solid_box(*box)
+    lat.init_equilibrium()
+    sim = HostSim(hs, w, h, rest, seed, bands, inlet, oracle.thresholds(density, *inflow, rest))
+    sim.load(lat.cells)
+    for s in range(steps):
+        lat.advance(1)
+        sim.advance(1)
+        got, ref = sim.store(), lat.cells
+        if not np.array_equal(got, ref):
+            bad = np.nonzero(got != ref)[0]
+            pytest.fail(f"step {s + 1}: {bad.size} cells differ, first at "
+                        f"{[(int(b) % w, int(b) // w) for b in bad[:4]]}")
+
+
+@pytest.mark.parametrize("idx", range(5))
+def test_golden_cases_every_step(oracle, hs, idx):
+    name, w, h, rest, inlet, plate, threads, steps = oracle.GOLDEN_CASES[idx]
+    box = oracle.golden_plate(w, h) if plate else None
+    compare(oracle, hs, w, h, rest, threads, inlet, oracle.GOLDEN_SEED, 60, box=box)
+
+
+@pytest.mark.parametrize("w,h,rest,bands,inlet", [
+    (65, 6, True, 1, 8), (100, 18, True, 3, 8), (129, 64, False, 4, 70), (200, 40, True, 7, 130),
+    (64, 32, True, 5, 0), (320, 10, True, 10, 8), (80, 48, True, 2, 80), (127, 8, False, 3, 127),
+])
+def test_ragged_shapes_every_step(oracle, hs, w, h, rest, bands, inlet):
+    yc = float(np.float32(np.float32(h) * oracle.SQRT3_2 / np.float32(2)))
+    compare(oracle, hs, w, h, rest, bands, inlet, 0xC0FFEE + w, 25, disc=(w / 3.0, yc, 3.5))
