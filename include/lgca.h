@@ -127,6 +127,45 @@ int lg_add_solid_disc(lg_t* g, float cx, float cy, float r);
  * stream seeded with seed ^ 0xA5A51234, bit-identical with the CPU. */
 int lg_init_equilibrium(lg_t* g, float density, float ux, float uy);
 
+/* transport::measure (src/transport.rs:183-319) on the engine: the viscosity
+ * from a decaying shear wave and the advection factor from a drifting wave in
+ * size x size periodic boxes, with the reference's protocol (transient, adaptive
+ * window, 4 realisations, 20 samples). `threads` is the CPU thread count whose
+ * random streams the runs reproduce; with the same value the results equal the
+ * CPU path's bit for bit. lg_measure_transport runs both concurrently. */
+int lg_measure_viscosity(float density, int rest, size_t threads, uint64_t seed, size_t size,
+                         float* nu);
+int lg_measure_advection(float density, int rest, size_t threads, uint64_t seed, size_t size,
+                         float* g);
+int lg_measure_transport(float density, int rest, size_t threads, uint64_t seed, size_t size,
+                         float* nu, float* g);
+
+/* Batches (no counterpart in the reference, which never batched lattices,
+ * docs/plan.md:482-485). A batch is a fixed set of distinct handles that are
+ * advanced together: one step kernel per step covers every member through a
+ * tile table, and the step loop is captured in a hipGraph and replayed. Each
+ * member ends exactly where its own lg_advance would have left it. While a
+ * batch exists its members may still be used on their own between batch calls
+ * (lg_batch_advance re-reads their state each call). */
+typedef struct lg_batch lg_batch_t;
+lg_batch_t* lg_batch_create(lg_t* const* members, size_t n);
+void lg_batch_destroy(lg_batch_t* b);
+int lg_batch_advance(lg_batch_t* b, uint64_t n);
+/* Launch shape: tile = words per workgroup (128 or 256), grid = step-kernel
+ * workgroups (0: one per tile; otherwise a persistent grid walking the
+ * tiles), chunk = steps per graph replay (even, at most 256). */
+int lg_batch_set_shape(lg_batch_t* b, uint32_t tile, uint32_t grid, uint32_t chunk);
+int lg_batch_get_shape(lg_batch_t* b, uint32_t* tile, uint32_t* grid, uint32_t* chunk);
+/* lg_sample_field for every member with a field, issued together. */
+int lg_batch_sample(lg_batch_t* b, float alpha);
+/* lg_sample_field without the wait (the sample is queued on the handle's
+ * stream; the next synchronous call on the handle sees it). */
+int lg_sample_field_async(lg_t* g, float alpha);
+/* Time each candidate shape on the batch itself (those steps are real steps
+ * of every member, reported in *used_steps) and keep the fastest.
+ * budget_steps = 0: no limit. */
+int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps);
+
 /* Diagnostics. The generated collision circuit as the device evaluates it,
  * for every state and every draw index k < stride, into out[state * stride +
  * k] (0xFF where the lanes of a word disagreed). Returns the stride (30 with

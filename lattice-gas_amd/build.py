@@ -1,6 +1,6 @@
 #!/usr/bin/env python3
 """Build the HIP engine for gfx950: generate the collision circuit, then
-compile lattice-gas_amd/csrc/lgca_engine.hip into liblgca_hip.so (in-tree)."""
+compile lattice-gas_amd/csrc/*.hip into liblgca_hip.so (in-tree)."""
 from __future__ import annotations
 
 import os
@@ -26,7 +26,8 @@ def build(verbose: bool = False) -> str:
                     "--out", os.path.join(CSRC, "lgca_collide.gen.h")], check=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB + ".tmp"
-    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp, os.path.join(CSRC, "lgca_engine.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("lgca_engine.hip", "lgca_batch.hip")]
+    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + srcs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

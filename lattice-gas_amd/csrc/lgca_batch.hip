@@ -1,0 +1,387 @@
+// Batched stepping: many independent lattices of uneven sizes advanced by one
+// step kernel per step, driven by a tile table, with the step loop captured in
+// a hipGraph and replayed.
+//
+// The reference never batched lattices ("would need replicas inside the step
+// kernel", docs/plan.md:482-485). Here each member keeps its own handle
+// (lg_t) and device buffers; a batch only adds
+//   - a descriptor per member (geometry, plane pointers, stream tables, seed,
+//     step counter and which plane set is current), in device memory;
+//   - a tile table: tile i = (member, first word of a run of TILE words);
+//   - per chunk of K steps: one tables launch for every member's streams, K
+//     step launches over all tiles (grid-stride, so a fixed persistent grid
+//     can walk any number of tiles), one launch that bumps the counters.
+// Every member computes exactly what its own lg_advance would: the per-word
+// work is lg_step_word with the member's own geometry and streams.
+#include <chrono>
+#include <cstring>
+
+#include "lgca_internal.h"
+
+using namespace lgca_internal;
+
+struct LgDesc {
+  LgGeom G;
+  uint64_t* p0;  // plane set 0 (7 planes)
+  uint64_t* p1;  // plane set 1
+  const uint64_t* solid;
+  const uint64_t* cjump;
+  const uint32_t* row_band;
+  const uint64_t* ijump;
+  uint64_t* tabs;  // >= K * (nbands + 1) * LG_WTAB
+  uint64_t seed;
+  uint64_t steps;  // step index of the current state
+  uint32_t cur;    // plane set holding the current state
+  uint32_t pad;
+};
+
+// Stream tables for members blockIdx.y, steps steps..steps+nsteps-1.
+__global__ void __launch_bounds__(256) lg_tables_batch_kernel(const LgDesc* __restrict D,
+                                                              uint32_t nsteps) {
+  const LgDesc& d = D[blockIdx.y];
+  uint32_t per = d.G.nbands + 1;
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nsteps * per * 16) return;
+  uint32_t grp = g & 15u, ks = g >> 4;
+  uint32_t k = ks / per, s = ks - k * per;
+  uint64_t t = d.steps + k;
+  uint64_t st = lg_rng_init(s < d.G.nbands ? lg_band_seed(d.seed, t, s) : lg_inlet_seed(d.seed, t + 1));
+  for (uint32_t i = 0; i < 4 * grp; ++i) st = lg_xs(st);
+  uint64_t b0 = st, b1 = lg_xs(b0), b2 = lg_xs(b1), b3 = lg_xs(b2);
+  uint64_t* out = d.tabs + (size_t)ks * LG_WTAB + grp * 16;
+  for (uint32_t v = 0; v < 16; ++v) {
+    uint64_t e = 0;
+    if (v & 1u) e ^= b0;
+    if (v & 2u) e ^= b1;
+    if (v & 4u) e ^= b2;
+    if (v & 8u) e ^= b3;
+    out[v] = e;
+  }
+}
+
+// Step k of the chunk for every tile. A tile is TILE consecutive thread
+// indices of one member, mapped to words exactly as lg_step_kernel maps them
+// (inflow words first).
+template <int TILE>
+__global__ void __launch_bounds__(TILE) lg_step_batch_kernel(const LgDesc* __restrict D,
+                                                             const uint2* __restrict tiles,
+                                                             uint32_t ntiles, uint32_t k) {
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint2 tl = tiles[t];
+    const LgDesc& d = D[tl.x];
+    const uint32_t g = tl.y + threadIdx.x;
+    if (g >= d.G.total) continue;
+    uint32_t y, j;
+    const uint32_t ninlet = d.G.h * d.G.iw;
+    if (g < ninlet) {
+      y = g / d.G.iw;
+      j = g - y * d.G.iw;
+    } else {
+      const uint32_t per = d.G.wpr - d.G.iw;
+      const uint32_t q = g - ninlet;
+      y = q / per;
+      j = d.G.iw + (q - y * per);
+    }
+    const uint32_t par = (d.cur + k) & 1u;
+    const uint64_t* src = par ? d.p1 : d.p0;
+    uint64_t* dst = par ? d.p0 : d.p1;
+    const uint64_t* tabs = d.tabs + (size_t)k * (d.G.nbands + 1) * LG_WTAB;
+    lg_step_word(d.G, src, dst, d.solid, d.cjump, d.row_band, d.ijump, tabs, y, j);
+  }
+}
+
+__global__ void lg_bump_kernel(LgDesc* __restrict D, uint32_t n, uint32_t count) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  D[i].steps += count;
+  D[i].cur ^= (count & 1u);
+}
+
+struct lg_batch {
+  std::vector<lg_t*> members;
+  hipStream_t stream = nullptr;
+  DevBuf<LgDesc> desc;
+  DevBuf<uint2> tiles128, tiles256;
+  uint32_t ntiles128 = 0, ntiles256 = 0;
+  uint32_t max_tab_blocks_per_step = 0;  // tables launch width per step of chunk
+  uint32_t kmax = 0;                     // largest chunk the members' tables hold
+  // tuned launch shape
+  uint32_t tile = 256;   // words per tile (128 or 256)
+  uint32_t grid = 0;     // step-kernel workgroups; 0 = one per tile
+  uint32_t chunk = 32;   // steps per graph replay (even)
+  // graph for `chunk` steps with the current shape
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t graph_chunk = 0, graph_tile = 0, graph_grid = 0;
+  int ncu = 256;
+  // host staging of the last uploads, alive until the stream has consumed them
+  std::vector<LgDesc> host_desc;
+  std::vector<uint2> host_tiles128, host_tiles256;
+};
+
+namespace {
+
+void drop_graph(lg_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_chunk = 0;
+}
+
+int upload_desc(lg_batch* b) {
+  std::vector<LgDesc>& h = b->host_desc;
+  h.assign(b->members.size(), LgDesc{});
+  for (size_t i = 0; i < h.size(); ++i) {
+    lg_t* g = b->members[i];
+    LgDesc& d = h[i];
+    std::memset(&d, 0, sizeof(d));
+    d.G = g->G;
+    d.p0 = g->a.p;
+    d.p1 = g->b.p;
+    d.solid = g->solid.p;
+    d.cjump = g->cjump.p;
+    d.row_band = g->row_band.p;
+    d.ijump = g->ijump.p;
+    d.tabs = g->basis.p;
+    d.seed = g->seed;
+    d.steps = g->steps;
+    d.cur = 0;
+  }
+  HIP_TRY(hipMemcpyAsync(b->desc.p, h.data(), h.size() * sizeof(LgDesc), hipMemcpyHostToDevice,
+                         b->stream));
+  return LG_OK;
+}
+
+template <int TILE>
+int build_tiles(lg_batch* b, DevBuf<uint2>& buf, uint32_t& n, std::vector<uint2>& t) {
+  t.clear();
+  for (size_t i = 0; i < b->members.size(); ++i) {
+    uint32_t total = b->members[i]->G.total;
+    for (uint32_t g0 = 0; g0 < total; g0 += TILE) t.push_back(make_uint2((uint32_t)i, g0));
+  }
+  n = (uint32_t)t.size();
+  HIP_TRY(buf.alloc(std::max<size_t>(t.size(), 1)));
+  if (!t.empty())
+    HIP_TRY(hipMemcpyAsync(buf.p, t.data(), t.size() * sizeof(uint2), hipMemcpyHostToDevice,
+                           b->stream));
+  return LG_OK;
+}
+
+// Enqueue `count` steps (count <= kmax) on the batch stream.
+int enqueue_chunk(lg_batch* b, uint32_t count) {
+  const uint32_t nm = (uint32_t)b->members.size();
+  dim3 tgrid(b->max_tab_blocks_per_step * count, nm);
+  hipLaunchKernelGGL(lg_tables_batch_kernel, tgrid, dim3(256), 0, b->stream, b->desc.p, count);
+  HIP_TRY(hipGetLastError());
+  const uint32_t ntiles = b->tile == 128 ? b->ntiles128 : b->ntiles256;
+  const uint2* tiles = b->tile == 128 ? b->tiles128.p : b->tiles256.p;
+  uint32_t grid = b->grid ? std::min(b->grid, ntiles) : ntiles;
+  for (uint32_t k = 0; k < count; ++k) {
+    if (b->tile == 128)
+      hipLaunchKernelGGL(lg_step_batch_kernel<128>, dim3(grid), dim3(128), 0, b->stream, b->desc.p,
+                         tiles, ntiles, k);
+    else
+      hipLaunchKernelGGL(lg_step_batch_kernel<256>, dim3(grid), dim3(256), 0, b->stream, b->desc.p,
+                         tiles, ntiles, k);
+  }
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(lg_bump_kernel, dim3(blocks_for(nm)), dim3(256), 0, b->stream, b->desc.p, nm,
+                     count);
+  HIP_TRY(hipGetLastError());
+  return LG_OK;
+}
+
+int ensure_graph(lg_batch* b) {
+  if (b->exec && b->graph_chunk == b->chunk && b->graph_tile == b->tile && b->graph_grid == b->grid)
+    return LG_OK;
+  drop_graph(b);
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_chunk(b, b->chunk);
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &gr);
+  if (rc) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(LG_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+  b->graph = gr;
+  HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->graph_chunk = b->chunk;
+  b->graph_tile = b->tile;
+  b->graph_grid = b->grid;
+  return LG_OK;
+}
+
+// Advance every member by n steps; the descriptors must be current.
+int run_steps(lg_batch* b, uint64_t n, bool use_graph) {
+  uint64_t left = n;
+  if (use_graph && left >= b->chunk) {
+    int rc = ensure_graph(b);
+    if (rc) return rc;
+    while (left >= b->chunk) {
+      HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+      left -= b->chunk;
+    }
+  }
+  while (left > 0) {
+    uint32_t c = (uint32_t)std::min<uint64_t>(left, b->kmax);
+    int rc = enqueue_chunk(b, c);
+    if (rc) return rc;
+    left -= c;
+  }
+  return LG_OK;
+}
+
+void commit_members(lg_batch* b, uint64_t n) {
+  for (lg_t* g : b->members) {
+    g->steps += n;
+    if (n & 1u) std::swap(g->a.p, g->b.p);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
+  if (!members || n == 0) {
+    fail(LG_EINVAL, "a batch needs at least one lattice");
+    return nullptr;
+  }
+  auto* b = new lg_batch();
+  auto bail = [&](const std::string& m) -> lg_batch_t* {
+    fail(LG_EDEVICE, m);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return nullptr;
+  };
+  b->members.assign(members, members + n);
+  for (size_t i = 0; i < n; ++i) {
+    if (!members[i]) return bail("null member");
+    for (size_t k = 0; k < i; ++k)
+      if (members[k] == members[i]) return bail("a lattice appears twice in the batch");
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail("hipStreamCreate failed");
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, members[0]->device) == hipSuccess) b->ncu = p.multiProcessorCount;
+  b->kmax = ~0u;
+  for (lg_t* g : b->members) {
+    b->kmax = std::min(b->kmax, g->basis_steps);
+    uint32_t per = g->G.nbands + 1;
+    b->max_tab_blocks_per_step = std::max(b->max_tab_blocks_per_step, blocks_for((size_t)per * 16));
+  }
+  b->chunk = std::min<uint32_t>(32, b->kmax & ~1u);
+  if (b->chunk == 0) b->chunk = 2;
+  if (b->desc.alloc(n)) return bail("hipMalloc failed for descriptors");
+  if (build_tiles<128>(b, b->tiles128, b->ntiles128, b->host_tiles128) ||
+      build_tiles<256>(b, b->tiles256, b->ntiles256, b->host_tiles256))
+    return bail(g_err);
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return bail("hipStreamSynchronize failed");
+  return b;
+}
+
+void lg_batch_destroy(lg_batch_t* b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  b->desc.release();
+  b->tiles128.release();
+  b->tiles256.release();
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int lg_batch_advance(lg_batch_t* b, uint64_t n) {
+  if (!b) return fail(LG_EINVAL, "null batch");
+  if (n == 0) return LG_OK;
+  for (lg_t* g : b->members) HIP_TRY(hipStreamSynchronize(g->stream));
+  int rc = upload_desc(b);
+  if (rc) return rc;
+  rc = run_steps(b, n, true);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  commit_members(b, n);
+  return LG_OK;
+}
+
+int lg_batch_set_shape(lg_batch_t* b, uint32_t tile, uint32_t grid, uint32_t chunk) {
+  if (!b) return fail(LG_EINVAL, "null batch");
+  if (tile != 128 && tile != 256) return fail(LG_EINVAL, "tile must be 128 or 256 words");
+  if (chunk == 0 || (chunk & 1u) || chunk > b->kmax)
+    return fail(LG_EINVAL, "chunk must be even and at most the members' table capacity");
+  b->tile = tile;
+  b->grid = grid;
+  b->chunk = chunk;
+  return LG_OK;
+}
+
+int lg_batch_get_shape(lg_batch_t* b, uint32_t* tile, uint32_t* grid, uint32_t* chunk) {
+  if (!b || !tile || !grid || !chunk) return fail(LG_EINVAL, "null argument");
+  *tile = b->tile;
+  *grid = b->grid;
+  *chunk = b->chunk;
+  return LG_OK;
+}
+
+int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps) {
+  if (!b || !used_steps) return fail(LG_EINVAL, "null argument");
+  *used_steps = 0;
+  for (lg_t* g : b->members) HIP_TRY(hipStreamSynchronize(g->stream));
+  int rc = upload_desc(b);
+  if (rc) return rc;
+  struct Cand {
+    uint32_t tile, grid, chunk;
+  };
+  std::vector<Cand> cands;
+  for (uint32_t tile : {256u, 128u})
+    for (uint32_t m : {0u, 4u, 8u, 16u})
+      for (uint32_t chunk : {8u, 32u}) {
+        uint32_t c = std::min(chunk, b->kmax & ~1u);
+        if (c == 0) continue;
+        cands.push_back({tile, m ? m * (uint32_t)b->ncu : 0u, c});
+      }
+  double best = 1e300;
+  Cand bestc = {b->tile, b->grid, b->chunk};
+  uint64_t used = 0;
+  for (const Cand& c : cands) {
+    uint64_t trial = 2ull * c.chunk;
+    if (budget_steps && used + 2 * trial > budget_steps) break;
+    b->tile = c.tile;
+    b->grid = c.grid;
+    b->chunk = c.chunk;
+    if ((rc = run_steps(b, trial, true))) return rc;  // warm (captures the graph)
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    auto t0 = std::chrono::steady_clock::now();
+    if ((rc = run_steps(b, trial, true))) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() /
+                (double)trial;
+    used += 2 * trial;
+    if (dt < best) {
+      best = dt;
+      bestc = c;
+    }
+  }
+  commit_members(b, used);
+  b->tile = bestc.tile;
+  b->grid = bestc.grid;
+  b->chunk = bestc.chunk;
+  *used_steps = used;
+  return LG_OK;
+}
+
+// Fold every member that has a field attached into it (src/render.rs:61-120),
+// on the members' own streams, and wait.
+int lg_batch_sample(lg_batch_t* b, float alpha) {
+  if (!b) return fail(LG_EINVAL, "null batch");
+  for (lg_t* g : b->members) {
+    int rc = lg_sample_field_async(g, alpha);
+    if (rc) return rc;
+  }
+  for (lg_t* g : b->members) HIP_TRY(hipStreamSynchronize(g->stream));
+  return LG_OK;
+}
+
+}  // extern "C"

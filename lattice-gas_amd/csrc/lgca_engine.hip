@@ -14,10 +14,10 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
-#include "../../include/lgca.h"
-#include "lgca_device.h"
+#include "lgca_internal.h"
 #include "lgca_jump.hpp"
 
 // ---------------------------------------------------------------------------
@@ -224,6 +224,48 @@ __global__ void __launch_bounds__(256) lg_init_kernel(LgGeom G, const uint64_t* 
   for (int d = 0; d < 7; ++d) planes[(size_t)d * G.total + g] = res[d];
 }
 
+// Per-row and per-column moment sums over every cell, obstacle sites
+// included (moments::WITH_WALLS, src/moments.rs:74): row y -> (mass, px2),
+// column x -> (mass, py2). Exact integers; transport::row_ux / column_uy
+// (src/transport.rs:51-91) derive their f32 velocities from these.
+__global__ void __launch_bounds__(256) lg_rows_kernel(LgGeom G, const uint64_t* __restrict planes,
+                                                      int64_t* __restrict out) {
+  uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= G.h) return;
+  const int CX2[6] = {2, 1, -1, -2, -1, 1};
+  int64_t mass = 0, px = 0;
+  for (uint32_t j = 0; j < G.wpr; ++j) {
+    size_t word = (size_t)y * G.wpr + j;
+    for (int d = 0; d < 6; ++d) {
+      int64_t c = __popcll(planes[(size_t)d * G.total + word]);
+      mass += c;
+      px += CX2[d] * c;
+    }
+    mass += __popcll(planes[(size_t)6 * G.total + word]);
+  }
+  out[2 * y] = mass;
+  out[2 * y + 1] = px;
+}
+
+__global__ void __launch_bounds__(256) lg_cols_kernel(LgGeom G, const uint64_t* __restrict planes,
+                                                      int64_t* __restrict out) {
+  uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= G.w) return;
+  const int CY2[6] = {0, 1, 1, 0, -1, -1};
+  uint32_t j = x >> 6, b = x & 63u;
+  int64_t mass = 0, py = 0;
+  for (uint32_t y = 0; y < G.h; ++y) {
+    size_t word = (size_t)y * G.wpr + j;
+    for (int d = 0; d < 7; ++d) {
+      int64_t c = (int64_t)((planes[(size_t)d * G.total + word] >> b) & 1ull);
+      mass += c;
+      if (d < 6) py += CY2[d] * c;
+    }
+  }
+  out[2 * x] = mass;
+  out[2 * x + 1] = py;
+}
+
 // Total particles, obstacle sites included (src/lattice.rs:299-312): one
 // thread per word, a 64-bit atomic per wave.
 __global__ void __launch_bounds__(256) lg_count_kernel(LgGeom G, const uint64_t* __restrict planes,
@@ -269,39 +311,9 @@ __global__ void __launch_bounds__(256) lg_verify_kernel(int rest, uint8_t* __res
 // Host side
 // ---------------------------------------------------------------------------
 
-namespace {
+namespace lgca_internal {
 
 thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HIP_TRY(expr)                                                                  \
-  do {                                                                                 \
-    hipError_t e_ = (expr);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail(LG_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));      \
-  } while (0)
-
-template <typename T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  hipError_t alloc(size_t count) {
-    release();
-    n = count;
-    if (count == 0) return hipSuccess;
-    return hipMalloc((void**)&p, count * sizeof(T));
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  ~DevBuf() { release(); }
-};
 
 // Equilibrium thresholds exactly as src/lattice.rs:59-82 computes them: each
 // f32 operation rounded on its own, in source order.
@@ -331,32 +343,14 @@ void equilibrium_thresholds(float density, float ux, float uy, bool rest, uint32
   out[6] = rest ? threshold_f32(density) : 0;
 }
 
-}  // namespace
 
-struct lg_lattice {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  LgGeom G{};
-  uint64_t seed = 0, steps = 0;
-  uint32_t bands = 1;
-  bool loaded = false;
-  // planes: a (current), b (next); solid; static tables
-  DevBuf<uint64_t> a, b, solid, cjump, ijump, basis;
-  DevBuf<uint32_t> row_band;
-  DevBuf<uint8_t> bytes;
-  uint32_t basis_steps = 0;  // capacity of `basis` in steps
-  // field
-  bool has_field = false;
-  uint32_t bx = 0, by = 0, bw = 0, bh = 0;
-  DevBuf<float> fux, fuy, frho, fsol;
-  DevBuf<int64_t> fsums;
-  DevBuf<unsigned long long> counter;
-};
+}  // namespace lgca_internal
+
+using namespace lgca_internal;
+
 
 namespace {
 
-uint32_t blocks_for(size_t n, uint32_t t = 256) { return (uint32_t)((n + t - 1) / t); }
 
 int rebuild_tables(lg_t* g, const std::vector<uint64_t>& solid_host) {
   std::vector<uint64_t> cj, ij;
@@ -658,6 +652,11 @@ int lg_advance_timed(lg_t* g, uint64_t n, float* ms) {
   return lg_advance_sampling_timed(g, n, 0, 0.0f, ms);
 }
 
+int lg_sample_field_async(lg_t* g, float alpha) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  return sample_now(g, alpha);
+}
+
 int lg_sample_field(lg_t* g, float alpha) {
   if (!g) return fail(LG_EINVAL, "null handle");
   int rc = sample_now(g, alpha);
@@ -775,6 +774,258 @@ int lg_init_equilibrium(lg_t* g, float density, float ux, float uy) {
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->loaded = true;
   return LG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// transport::measure on the engine (src/transport.rs:24-319)
+// ---------------------------------------------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+// Equilibrium::sample (src/lattice.rs:102-115) on the host, for the initial
+// states of the measurement's realisations.
+uint8_t eq_sample(const uint32_t t[7], bool rest, uint64_t& st) {
+  uint8_t s = 0;
+  for (int d = 0; d < 6; d += 2) {
+    st = lg_xs(st);
+    uint64_t r = st * LG_RNG_MUL;
+    s |= (uint8_t)(((uint32_t)(r >> 40) < t[d]) ? 1u : 0u) << d;
+    s |= (uint8_t)((((uint32_t)(r >> 16) & 0xFFFFFFu) < t[d + 1]) ? 1u : 0u) << (d + 1);
+  }
+  if (rest) {
+    st = lg_xs(st);
+    uint64_t r = st * LG_RNG_MUL;
+    if ((uint32_t)(r >> 40) < t[6]) s |= 0x40u;
+  }
+  return s;
+}
+
+struct Realisation {
+  std::vector<uint8_t> cells;
+  uint64_t seed;
+};
+
+const uint64_t TR_TRANSIENT = 200, TR_MIN_INTERVAL = 4, TR_REPS = 4, TR_MAX_WINDOW = 30000;
+const size_t TR_SAMPLES = 20;
+const float TR_SQRT3_2 = 0.8660254f;
+const float TR_PI_F = 3.14159265358979323846f;
+
+double fit_slope(const std::vector<std::pair<double, double>>& pts) {
+  double n = (double)pts.size(), sx = 0, sy = 0, sxx = 0, sxy = 0;
+  for (auto& p : pts) sx += p.first;
+  for (auto& p : pts) sy += p.second;
+  for (auto& p : pts) sxx += p.first * p.first;
+  for (auto& p : pts) sxy += p.first * p.second;
+  return (n * sxy - sx * sy) / (n * sxx - sx * sx);
+}
+
+// The measurement's projection: exact row or column sums from the device,
+// then the reference's f32 velocity and f32 cos/sin projection in f64.
+struct Projector {
+  lg_t* g;
+  bool rows;
+  float k;
+  DevBuf<int64_t> buf;
+  std::vector<int64_t> host;
+  int init() {
+    size_t n = rows ? g->G.h : g->G.w;
+    host.resize(2 * n);
+    HIP_TRY(buf.alloc(2 * n));
+    return LG_OK;
+  }
+  int project(std::pair<double, double>* out) {
+    size_t n = rows ? g->G.h : g->G.w;
+    if (rows)
+      hipLaunchKernelGGL(lg_rows_kernel, dim3(blocks_for(n)), dim3(256), 0, g->stream, g->G, g->a.p,
+                         buf.p);
+    else
+      hipLaunchKernelGGL(lg_cols_kernel, dim3(blocks_for(n)), dim3(256), 0, g->stream, g->G, g->a.p,
+                         buf.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(host.data(), buf.p, host.size() * 8, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    double a = 0.0, b = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+      int64_t m = host[2 * i], p = host[2 * i + 1];
+      float u, ph;
+      if (rows) {  // Moments::velocity().0 (src/moments.rs:91-97)
+        u = m == 0 ? 0.0f : (float)p * 0.5f / (float)m;
+        ph = k * (float)i * TR_SQRT3_2;
+      } else {  // column_uy (src/transport.rs:75-83)
+        u = m > 0 ? (float)p * TR_SQRT3_2 / (float)m : 0.0f;
+        ph = k * (float)i;
+      }
+      a += (double)(u * std::cos(ph));
+      b += (double)(u * std::sin(ph));
+    }
+    out->first = 2.0 * a / (double)n;
+    out->second = 2.0 * b / (double)n;
+    return LG_OK;
+  }
+};
+
+// averaged() of src/transport.rs:131-171 over the engine.
+template <class StartFn>
+int measure_mode(lg_t* g, Projector& P, const StartFn& start, std::vector<double>& times,
+                 std::vector<std::pair<double, double>>& acc) {
+  int rc;
+  Realisation r0 = start(0);
+  if ((rc = lg_restart(g, r0.cells.data(), r0.cells.size(), r0.seed))) return rc;
+  if ((rc = lg_advance(g, TR_TRANSIENT))) return rc;
+  std::pair<double, double> ab;
+  if ((rc = P.project(&ab))) return rc;
+  double a0 = std::max(std::sqrt(ab.first * ab.first + ab.second * ab.second), 1e-12);
+  uint64_t t = 0;
+  while (t < TR_MAX_WINDOW) {
+    if ((rc = lg_advance(g, 25))) return rc;
+    t += 25;
+    if ((rc = P.project(&ab))) return rc;
+    if (std::sqrt(ab.first * ab.first + ab.second * ab.second) < a0 / 3.0) break;
+  }
+  uint64_t window = std::min(std::max(t, (uint64_t)TR_SAMPLES * TR_MIN_INTERVAL), TR_MAX_WINDOW);
+  uint64_t interval = std::max(window / (uint64_t)TR_SAMPLES, TR_MIN_INTERVAL);
+  acc.assign(TR_SAMPLES, {0.0, 0.0});
+  for (uint64_t rep = 0; rep < TR_REPS; ++rep) {
+    Realisation r = start(rep);
+    if ((rc = lg_restart(g, r.cells.data(), r.cells.size(), r.seed))) return rc;
+    if ((rc = lg_advance(g, TR_TRANSIENT))) return rc;
+    for (size_t i = 0; i < TR_SAMPLES; ++i) {
+      if ((rc = P.project(&ab))) return rc;
+      acc[i].first += ab.first / (double)TR_REPS;
+      acc[i].second += ab.second / (double)TR_REPS;
+      if ((rc = lg_advance(g, interval))) return rc;
+    }
+  }
+  times.resize(TR_SAMPLES);
+  for (size_t i = 0; i < TR_SAMPLES; ++i) times[i] = (double)(i * interval);
+  return LG_OK;
+}
+
+struct Handle {
+  lg_t* g = nullptr;
+  ~Handle() { lg_destroy(g); }
+};
+
+}  // namespace
+
+extern "C" {
+
+// transport::viscosity (src/transport.rs:183-221)
+int lg_measure_viscosity(float density, int rest, size_t threads, uint64_t seed, size_t size,
+                         float* nu) {
+  if (!nu) return fail(LG_EINVAL, "null argument");
+  const size_t w = size, h = size;
+  const float k = 2.0f * TR_PI_F / ((float)h * TR_SQRT3_2);
+  const float amp = 0.05f;
+  auto start = [&](uint64_t rep) {
+    uint64_t s = seed ^ ((rep * 0x9E3779B9ull) << 20);
+    uint64_t st = lg_rng_init(s ^ 0xBEEF);
+    Realisation r{std::vector<uint8_t>(w * h), s};
+    uint32_t t[7];
+    for (size_t y = 0; y < h; ++y) {
+      float ux = amp * std::sin(k * (float)y * TR_SQRT3_2);
+      equilibrium_thresholds(density, ux, 0.0f, rest != 0, t);
+      for (size_t x = 0; x < w; ++x) r.cells[y * w + x] = eq_sample(t, rest != 0, st);
+    }
+    return r;
+  };
+  Handle H;
+  H.g = lg_create(size, size, rest, 1, threads);  // box_of, src/transport.rs:176-179
+  if (!H.g) return LG_EDEVICE;
+  Projector P{H.g, true, k, {}, {}};
+  int rc = P.init();
+  if (rc) return rc;
+  std::vector<double> times;
+  std::vector<std::pair<double, double>> mode;
+  if ((rc = measure_mode(H.g, P, start, times, mode))) return rc;
+  std::vector<std::pair<double, double>> pts;
+  for (size_t i = 0; i < times.size(); ++i) {
+    double a = mode[i].first, b = mode[i].second;
+    pts.push_back({times[i], std::log(std::max(std::sqrt(a * a + b * b), 1e-12))});
+  }
+  *nu = (float)(-fit_slope(pts) / (double)(k * k));
+  return LG_OK;
+}
+
+// transport::advection_factor (src/transport.rs:225-289)
+int lg_measure_advection(float density, int rest, size_t threads, uint64_t seed, size_t size,
+                         float* g_out) {
+  if (!g_out) return fail(LG_EINVAL, "null argument");
+  const size_t w = size, h = size;
+  const float u0 = 0.25f;
+  const float k = 2.0f * TR_PI_F / (float)w;
+  const float amp = 0.05f;
+  std::vector<uint32_t> cols((size_t)w * 7);
+  for (size_t x = 0; x < w; ++x)
+    equilibrium_thresholds(density, u0, amp * std::sin(k * (float)x), rest != 0, &cols[x * 7]);
+  auto start = [&](uint64_t rep) {
+    uint64_t s = seed ^ ((rep * 0x9E3779B9ull) << 20) ^ 0x1357;
+    uint64_t st = lg_rng_init(s ^ 0xF00D);
+    Realisation r{std::vector<uint8_t>(w * h), s};
+    for (size_t y = 0; y < h; ++y)
+      for (size_t x = 0; x < w; ++x) r.cells[y * w + x] = eq_sample(&cols[x * 7], rest != 0, st);
+    return r;
+  };
+  // The realised mean speed (src/transport.rs:264-265): WITH_WALLS sums of
+  // the first realisation, Moments::velocity().0.
+  float u_actual;
+  {
+    Realisation r0 = start(0);
+    int64_t mass = 0, px = 0;
+    const int CX2[6] = {2, 1, -1, -2, -1, 1};
+    for (uint8_t c : r0.cells) {
+      for (int d = 0; d < 6; ++d)
+        if (c & (1u << d)) {
+          mass += 1;
+          px += CX2[d];
+        }
+      if (c & 0x40u) mass += 1;
+    }
+    u_actual = mass == 0 ? 0.0f : (float)px * 0.5f / (float)mass;
+  }
+  Handle H;
+  H.g = lg_create(size, size, rest, 1, threads);
+  if (!H.g) return LG_EDEVICE;
+  Projector P{H.g, false, k, {}, {}};
+  int rc = P.init();
+  if (rc) return rc;
+  std::vector<double> times;
+  std::vector<std::pair<double, double>> mode;
+  if ((rc = measure_mode(H.g, P, start, times, mode))) return rc;
+  const double PI = 3.14159265358979323846;
+  double unwrapped = 0.0, prev = std::atan2(mode[0].first, mode[0].second);
+  std::vector<std::pair<double, double>> pts;
+  for (size_t i = 0; i < mode.size(); ++i) {
+    double ph = std::atan2(mode[i].first, mode[i].second);
+    double delta = ph - prev;
+    while (delta > PI) delta -= 2.0 * PI;
+    while (delta < -PI) delta += 2.0 * PI;
+    unwrapped += delta;
+    prev = ph;
+    pts.push_back({times[i], unwrapped});
+  }
+  *g_out = (float)(-fit_slope(pts) / (double)(k * u_actual));
+  return LG_OK;
+}
+
+// transport::measure (src/transport.rs:291-319): both measurements, each on
+// its own handle and stream, run concurrently from two host threads as the
+// reference does on its GPU path.
+int lg_measure_transport(float density, int rest, size_t threads, uint64_t seed, size_t size,
+                         float* nu, float* g_out) {
+  if (!nu || !g_out) return fail(LG_EINVAL, "null argument");
+  int rc_nu = 0;
+  std::string err_nu;
+  std::thread t([&] {
+    rc_nu = lg_measure_viscosity(density, rest, threads, seed, size, nu);
+    if (rc_nu) err_nu = g_err;
+  });
+  int rc_g = lg_measure_advection(density, rest, threads, seed, size, g_out);
+  t.join();
+  if (rc_nu) return fail(rc_nu, err_nu);
+  return rc_g;
 }
 
 // Test hook: the generated circuit evaluated on the device for every state

@@ -1,0 +1,77 @@
+// Internal host-side definitions shared by the engine's translation units:
+// error plumbing, device buffers, and the handle behind lg_t.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/lgca.h"
+#include "lgca_device.h"
+
+namespace lgca_internal {
+
+extern thread_local std::string g_err;
+
+inline int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return lgca_internal::fail(LG_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)&p, count * sizeof(T));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+inline uint32_t blocks_for(size_t n, uint32_t t = 256) { return (uint32_t)((n + t - 1) / t); }
+
+// Equilibrium thresholds exactly as src/lattice.rs:59-82 computes them.
+void equilibrium_thresholds(float density, float ux, float uy, bool rest, uint32_t out[7]);
+
+}  // namespace lgca_internal
+
+// The handle behind lg_t: owns every device allocation of one lattice.
+struct lg_lattice {
+  template <typename T>
+  using DevBuf = lgca_internal::DevBuf<T>;
+
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  LgGeom G{};
+  uint64_t seed = 0, steps = 0;
+  uint32_t bands = 1;
+  bool loaded = false;
+  // planes: a (current), b (next); solid; static tables
+  DevBuf<uint64_t> a, b, solid, cjump, ijump, basis;
+  DevBuf<uint32_t> row_band;
+  DevBuf<uint8_t> bytes;
+  uint32_t basis_steps = 0;  // capacity of `basis` in steps
+  // field
+  bool has_field = false;
+  uint32_t bx = 0, by = 0, bw = 0, bh = 0;
+  DevBuf<float> fux, fuy, frho, fsol;
+  DevBuf<int64_t> fsums;
+  DevBuf<unsigned long long> counter;
+};

@@ -62,6 +62,17 @@ def load_library(path: str = LIB_PATH):
         "lg_add_solid_box": (i32, [vp, f32, f32, f32, f32]),
         "lg_add_solid_disc": (i32, [vp, f32, f32, f32]),
         "lg_init_equilibrium": (i32, [vp, f32, f32, f32]),
+        "lg_batch_create": (vp, [P(vp), sz]),
+        "lg_batch_destroy": (None, [vp]),
+        "lg_batch_advance": (i32, [vp, u64]),
+        "lg_batch_set_shape": (i32, [vp, C.c_uint32, C.c_uint32, C.c_uint32]),
+        "lg_batch_get_shape": (i32, [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
+        "lg_batch_autotune": (i32, [vp, u64, P(u64)]),
+        "lg_batch_sample": (i32, [vp, f32]),
+        "lg_sample_field_async": (i32, [vp, f32]),
+        "lg_measure_viscosity": (i32, [f32, i32, sz, u64, sz, P(f32)]),
+        "lg_measure_advection": (i32, [f32, i32, sz, u64, sz, P(f32)]),
+        "lg_measure_transport": (i32, [f32, i32, sz, u64, sz, P(f32), P(f32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -211,6 +222,71 @@ class GpuLattice:
     @steps.setter
     def steps(self, v: int):
         _check(load_library().lg_set_steps(self._p, int(v)))
+
+
+class Batch:
+    """Advance several GpuLattice members together: one tile-table step kernel
+    per step, the step loop replayed from a hipGraph. Members end exactly
+    where their own advance() would leave them."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m._p for m in self.members])
+        self._p = load_library().lg_batch_create(arr, len(self.members))
+        if not self._p:
+            raise LgcaError(load_library().lg_last_error().decode())
+
+    def close(self):
+        p = getattr(self, "_p", None)
+        if p:
+            load_library().lg_batch_destroy(p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+    def advance(self, n: int):
+        _check(load_library().lg_batch_advance(self._p, int(n)))
+
+    def set_shape(self, tile: int, grid: int, chunk: int):
+        _check(load_library().lg_batch_set_shape(self._p, int(tile), int(grid), int(chunk)))
+
+    def shape(self):
+        t, g, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(load_library().lg_batch_get_shape(self._p, C.byref(t), C.byref(g), C.byref(c)))
+        return t.value, g.value, c.value
+
+    def sample(self, alpha: float):
+        _check(load_library().lg_batch_sample(self._p, float(alpha)))
+
+    def autotune(self, budget_steps: int = 0) -> int:
+        used = C.c_uint64()
+        _check(load_library().lg_batch_autotune(self._p, int(budget_steps), C.byref(used)))
+        return int(used.value)
+
+
+def measure_transport(density: float, rest: bool, threads: int, seed: int, size: int = 256):
+    """transport::measure (src/transport.rs:291-319) on the engine: (nu, g),
+    equal to the CPU path's at the same thread count."""
+    nu, g = C.c_float(), C.c_float()
+    _check(load_library().lg_measure_transport(float(density), int(rest), int(threads),
+                                               int(seed) & (2**64 - 1), int(size),
+                                               C.byref(nu), C.byref(g)))
+    return nu.value, g.value
+
+
+def measure_viscosity(density, rest, threads, seed, size=256) -> float:
+    nu = C.c_float()
+    _check(load_library().lg_measure_viscosity(float(density), int(rest), int(threads),
+                                               int(seed) & (2**64 - 1), int(size), C.byref(nu)))
+    return nu.value
+
+
+def measure_advection(density, rest, threads, seed, size=256) -> float:
+    g = C.c_float()
+    _check(load_library().lg_measure_advection(float(density), int(rest), int(threads),
+                                               int(seed) & (2**64 - 1), int(size), C.byref(g)))
+    return g.value
 
 
 def collide_table_on_device(rest: bool) -> np.ndarray:

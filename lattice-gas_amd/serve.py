@@ -1,0 +1,233 @@
+"""Serving FHP runs on one MI355X with continuous batching.
+
+A request is one lattice-gas run: size, step count, plate geometry, inflow
+drift, seed and CPU band count (the band count fixes its random streams, so
+each served run stays bit-exact with its own CPU replay). The server
+
+  * admits requests as they arrive, while their HBM footprint fits the budget;
+  * keeps every admitted lattice resident on the device;
+  * advances all resident lattices together, one tile-table step kernel per
+    step with the step loop replayed from a hipGraph (lgca_amd.Batch), in
+    quanta that end when the next member is due a snapshot or is done;
+  * streams back each run's coarse-grained velocity field every `every` of its
+    own steps (blocks of `block` x `block` cells, the reference's Field with
+    alpha = 1) and at its end;
+  * retires finished runs, freeing their memory for the queue.
+
+The plain loop runs the same trace one lattice at a time, first come first
+served, with the same snapshots. Both report p50/p99 latency (completion minus
+arrival) and aggregate throughput (cell updates over the makespan).
+"""
+from __future__ import annotations
+
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from lgca_amd import Batch, GpuLattice
+import workloads as W
+
+GIB = 1 << 30
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float      # seconds after the trace starts
+    w: int
+    h: int
+    steps: int
+    seed: int
+    bands: int
+    drift: float
+    plate_x: float
+    plate_rows: int
+    density: float = 0.25
+    inlet: int = 8
+
+    @property
+    def cells(self) -> int:
+        return self.w * self.h
+
+
+@dataclass
+class Result:
+    rid: int
+    arrival: float
+    start: float = 0.0
+    done: float = 0.0
+    snapshots: list = field(default_factory=list)   # (step, ux, uy, rho, solid)
+    final_cells: np.ndarray | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+def make_trace(n: int, rate: float = 50.0, seed: int = 1, min_cells: int = 256 * 256,
+               max_cells: int = 4096 * 2304, min_steps: int = 500, max_steps: int = 20000,
+               bands: int = 8) -> list[Request]:
+    """Poisson arrivals at `rate` per second; cell counts log-uniform between
+    the bounds (aspect from 1:1 up to 16:9, sides multiples of 64, even
+    height); steps uniform; plate 10-25% of H tall at x in [W/8, W/3];
+    inflow drift uniform in [0.1, 0.3]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        cells = math.exp(rng.uniform(math.log(min_cells), math.log(max_cells)))
+        aspect = rng.uniform(1.0, 16.0 / 9.0)
+        w = max(64, int(round(math.sqrt(cells * aspect) / 64.0)) * 64)
+        h = max(64, int(round(cells / w / 64.0)) * 64)
+        steps = int(rng.integers(min_steps, max_steps + 1))
+        frac = rng.uniform(0.10, 0.25)
+        out.append(Request(rid=i, arrival=t, w=w, h=h, steps=steps,
+                           seed=int(rng.integers(1, 2**63)), bands=bands,
+                           drift=float(rng.uniform(0.1, 0.3)),
+                           plate_x=float(rng.uniform(w / 8.0, w / 3.0)),
+                           plate_rows=max(2, int(round(frac * h)))))
+    return out
+
+
+def build(r: Request, block: int) -> GpuLattice:
+    g = GpuLattice(r.w, r.h, True, r.seed, r.bands)
+    g.set_inlet(r.inlet, r.density, (r.drift, 0.0))
+    g.add_solid_box(*W.plate_box(r.w, r.h, r.plate_x, r.plate_rows))
+    g.init_equilibrium(r.density, 0.0, 0.0)
+    g.attach_field(block, block)
+    return g
+
+
+def footprint(r: Request, block: int) -> int:
+    """Device bytes of one resident run: two plane sets, solid and jump
+    tables, the stream tables, the field."""
+    wpr = (r.w + 63) // 64
+    words = wpr * r.h
+    nb = -(-r.h // (-(-r.h // min(r.bands, r.h))))
+    tab_steps = max(1, min(256, (32 << 20) // ((nb + 1) * 256 * 8)))
+    blocks = (r.w // block) * (r.h // block)
+    return 8 * (16 * words + r.h) + tab_steps * (nb + 1) * 2048 + 48 * blocks + r.cells
+
+
+def _snapshot(g: GpuLattice, step: int):
+    ux, uy, rho, sol = g.read_field()
+    return (step, ux, uy, rho, sol)
+
+
+def _percentile(xs, q):
+    return float(np.percentile(np.asarray(xs), q)) if xs else float("nan")
+
+
+def _summary(trace, results, t_first, t_last, mode):
+    lat = [results[r.rid].latency for r in trace]
+    work = sum(r.cells * r.steps for r in trace)
+    span = t_last - t_first
+    return {"mode": mode, "requests": len(trace), "cell_updates": work,
+            "makespan_s": round(span, 4), "gcups": round(work / span / 1e9, 3),
+            "p50_latency_s": round(_percentile(lat, 50), 4),
+            "p99_latency_s": round(_percentile(lat, 99), 4),
+            "mean_latency_s": round(float(np.mean(lat)), 4)}
+
+
+def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
+                  hbm_budget: int = 200 * GIB, keep_cells: bool = False, shape=None):
+    """Continuous batching over the trace; returns (summary, results)."""
+    pending = deque(sorted(trace, key=lambda r: r.arrival))
+    results = {r.rid: Result(r.rid, r.arrival) for r in trace}
+    active = []             # dicts: req, g, done, until_snap, bytes
+    used = 0
+    batch = None
+    members_changed = True
+    t0 = time.perf_counter()
+    clock = lambda: time.perf_counter() - t0
+    quanta = 0
+    peak = 0
+    while pending or active:
+        now = clock()
+        while pending and pending[0].arrival <= now:
+            r = pending[0]
+            fp = footprint(r, block)
+            if used + fp > hbm_budget and active:
+                break                                # wait for memory to free up
+            pending.popleft()
+            g = build(r, block)
+            used += fp
+            results[r.rid].start = clock()
+            active.append({"req": r, "g": g, "done": 0, "until": min(every, r.steps), "bytes": fp})
+            members_changed = True
+        if not active:
+            if pending:
+                time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
+            continue
+        peak = max(peak, len(active))
+        if members_changed:
+            if batch is not None:
+                batch.close()
+            batch = Batch([a["g"] for a in active])
+            if shape:
+                batch.set_shape(*shape)
+            members_changed = False
+        q = min(a["until"] for a in active)
+        batch.advance(q)
+        quanta += 1
+        due = []
+        for a in active:
+            a["done"] += q
+            a["until"] -= q
+            if a["until"] == 0:
+                due.append(a)
+        for a in due:
+            a["g"].sample_field(1.0)
+            results[a["req"].rid].snapshots.append(_snapshot(a["g"], a["done"]))
+            a["until"] = min(every, a["req"].steps - a["done"])
+        finished = [a for a in active if a["done"] >= a["req"].steps]
+        if finished:
+            for a in finished:
+                res = results[a["req"].rid]
+                if keep_cells:
+                    res.final_cells = a["g"].store()
+                res.done = clock()
+                used -= a["bytes"]
+            active = [a for a in active if a["done"] < a["req"].steps]
+            if batch is not None:
+                batch.close()
+                batch = None
+            for a in finished:
+                a["g"].close()
+            members_changed = True
+    t_last = max(res.done for res in results.values())
+    s = _summary(trace, results, 0.0, t_last, "batched")
+    s["quanta"] = quanta
+    s["peak_resident"] = peak
+    return s, results
+
+
+def serve_plain(trace: list[Request], every: int = 100, block: int = 32, keep_cells: bool = False):
+    """The plain loop: one lattice at a time, first come first served."""
+    results = {r.rid: Result(r.rid, r.arrival) for r in trace}
+    t0 = time.perf_counter()
+    clock = lambda: time.perf_counter() - t0
+    for r in sorted(trace, key=lambda r: r.arrival):
+        wait = r.arrival - clock()
+        if wait > 0:
+            time.sleep(wait)
+        res = results[r.rid]
+        res.start = clock()
+        g = build(r, block)
+        done = 0
+        while done < r.steps:
+            q = min(every, r.steps - done)
+            g.advance(q)
+            done += q
+            g.sample_field(1.0)
+            res.snapshots.append(_snapshot(g, done))
+        if keep_cells:
+            res.final_cells = g.store()
+        res.done = clock()
+        g.close()
+    t_last = max(res.done for res in results.values())
+    return _summary(trace, results, 0.0, t_last, "plain"), results

@@ -4,6 +4,7 @@
 // Each entry point forwards to the restatement in lgca_oracle.hpp, which cites
 // the reference file:line it follows.
 #include "lgca_oracle.hpp"
+#include "lgca_transport.hpp"
 
 using namespace oracle;
 
@@ -147,6 +148,24 @@ void orc_moments_sum(int walls, const uint8_t* cells, size_t n, int64_t* out4, f
   out4[2] = m.py2;
   out4[3] = m.solid;
   m.velocity(&v2[0], &v2[1]);
+}
+
+// transport::viscosity / advection_factor (src/transport.rs:183-289)
+float orc_viscosity(float density, int rest, size_t threads, uint64_t seed, size_t size) {
+  return viscosity(density, rest != 0, threads, seed, size);
+}
+float orc_advection_factor(float density, int rest, size_t threads, uint64_t seed, size_t size) {
+  return advection_factor(density, rest != 0, threads, seed, size);
+}
+// Per-row x velocity and per-column y velocity of a byte lattice
+// (src/transport.rs:51-91), for checking the engine's projections.
+void orc_row_ux(const uint8_t* cells, size_t w, size_t h, float* out) {
+  auto u = row_ux(cells, w, h);
+  std::memcpy(out, u.data(), h * 4);
+}
+void orc_column_uy(const uint8_t* cells, size_t w, size_t h, float* out) {
+  auto u = column_uy(cells, w, h);
+  std::memcpy(out, u.data(), w * 4);
 }
 
 }  // extern "C"

@@ -70,6 +70,10 @@ def lib():
             "orc_classes": (sz, [i32, P(C.c_uint8), P(C.c_uint32)]),
             "orc_moment_tables": (None, [P(u64), P(u64)]),
             "orc_moments_sum": (None, [i32, P(C.c_uint8), sz, P(C.c_int64), P(f32)]),
+            "orc_viscosity": (f32, [f32, i32, sz, u64, sz]),
+            "orc_advection_factor": (f32, [f32, i32, sz, u64, sz]),
+            "orc_row_ux": (None, [P(C.c_uint8), sz, sz, P(f32)]),
+            "orc_column_uy": (None, [P(C.c_uint8), sz, sz, P(f32)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -145,6 +149,30 @@ def moments_sum(cells: np.ndarray, walls: bool):
     v = np.zeros(2, dtype=np.float32)
     lib().orc_moments_sum(int(walls), _ptr(a, C.c_uint8), a.size, _ptr(out, C.c_int64), _ptr(v, C.c_float))
     return tuple(int(x) for x in out), (float(v[0]), float(v[1]))
+
+
+def viscosity(density, rest, threads, seed, size=256) -> float:
+    """transport::viscosity (src/transport.rs:183-221)."""
+    return float(lib().orc_viscosity(density, int(rest), threads, seed, size))
+
+
+def advection_factor(density, rest, threads, seed, size=256) -> float:
+    """transport::advection_factor (src/transport.rs:225-289)."""
+    return float(lib().orc_advection_factor(density, int(rest), threads, seed, size))
+
+
+def row_ux(cells, w, h):
+    a = np.ascontiguousarray(cells, dtype=np.uint8)
+    out = np.zeros(h, dtype=np.float32)
+    lib().orc_row_ux(_ptr(a, C.c_uint8), w, h, _ptr(out, C.c_float))
+    return out
+
+
+def column_uy(cells, w, h):
+    a = np.ascontiguousarray(cells, dtype=np.uint8)
+    out = np.zeros(w, dtype=np.float32)
+    lib().orc_column_uy(_ptr(a, C.c_uint8), w, h, _ptr(out, C.c_float))
+    return out
 
 
 # ---- Lattice / Field -------------------------------------------------------

@@ -1,0 +1,82 @@
+"""Batched stepping (tile table + hipGraph replay) equals every member's own
+run, bit for bit (gpu)."""
+import numpy as np
+import pytest
+
+from lgca_amd import Batch, GpuLattice
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def member(w, h, rest, seed, bands, inlet, plate):
+    g = GpuLattice(w, h, rest, seed, bands)
+    g.set_inlet(inlet, 0.25, (0.2, 0.0))
+    if plate:
+        g.add_solid_box(*W.plate_box(w, h, w / 5.0, max(4, h // 6)))
+    g.init_equilibrium(0.25, 0.0, 0.0)
+    return g
+
+
+SPECS = [  # uneven sizes, ragged tails, bands, inflow on/off, plates
+    (256, 128, True, 1, 3, 8, True),
+    (192, 64, True, 2, 1, 0, False),
+    (100, 36, False, 3, 5, 8, True),
+    (1024, 320, True, 4, 16, 8, True),
+    (65, 10, True, 5, 2, 70, False),
+    (640, 512, True, 6, 7, 8, True),
+]
+
+
+def build_all():
+    return [member(*s) for s in SPECS]
+
+
+@pytest.mark.parametrize("shape", [None, (128, 0, 8), (256, 1024, 2), (128, 4096, 16)])
+def test_batch_equals_solo_runs(shape):
+    print(f"batch shape={shape}", flush=True)
+    solo = build_all()
+    for g in solo:
+        g.advance(45)
+    batched = build_all()
+    b = Batch(batched)
+    if shape:
+        b.set_shape(*shape)
+    b.advance(20)          # graph replays + remainder
+    b.advance(25)
+    for s, g in zip(solo, batched):
+        assert g.steps == s.steps == 45
+        assert np.array_equal(g.store(), s.store())
+
+
+def test_graph_replays_with_eager_work_between():
+    """Each replay checked against eager stepping, with the members' own
+    eager calls (store, solo advance) between replays."""
+    print("graph replays", flush=True)
+    ref = build_all()
+    bat = build_all()
+    b = Batch(bat)
+    b.set_shape(256, 0, 4)
+    for rep in range(4):
+        b.advance(4)                 # exactly one replay
+        for r in ref:
+            r.advance(4)
+        for r, g in zip(ref, bat):
+            assert np.array_equal(g.store(), r.store())
+        bat[rep].advance(3)          # a member steps on its own in between
+        ref[rep].advance(3)
+
+
+def test_autotune_keeps_results_exact():
+    print("autotune", flush=True)
+    ref = build_all()
+    bat = build_all()
+    b = Batch(bat)
+    used = b.autotune(budget_steps=400)
+    assert used > 0 and b.shape()[0] in (128, 256)
+    b.advance(30)
+    for r in ref:
+        r.advance(used + 30)
+    for r, g in zip(ref, bat):
+        assert g.steps == r.steps
+        assert np.array_equal(g.store(), r.store())
