@@ -127,6 +127,12 @@ int lg_add_solid_disc(lg_t* g, float cx, float cy, float r);
  * stream seeded with seed ^ 0xA5A51234, bit-identical with the CPU. */
 int lg_init_equilibrium(lg_t* g, float density, float ux, float uy);
 
+/* Lattice::init_equilibrium with a profile: site (x, y) is drawn from
+ * Equilibrium(density, ux[i], uy[i]) with i = x (axis 0, n = w) or i = y
+ * (axis 1, n = h), in the same stream and order as lg_init_equilibrium; the
+ * column form is how src/transport.rs:238-251 builds its start states. */
+int lg_init_profile(lg_t* g, int axis, float density, const float* ux, const float* uy, size_t n);
+
 /* transport::measure (src/transport.rs:183-319) on the engine: the viscosity
  * from a decaying shear wave and the advection factor from a drifting wave in
  * size x size periodic boxes, with the reference's protocol (transient, adaptive

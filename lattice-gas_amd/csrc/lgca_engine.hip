@@ -224,6 +224,43 @@ __global__ void __launch_bounds__(256) lg_init_kernel(LgGeom G, const uint64_t* 
   for (int d = 0; d < 7; ++d) planes[(size_t)d * G.total + g] = res[d];
 }
 
+// Lattice::init_equilibrium with a profile: cell (x, y) is drawn from the
+// equilibrium of column x (axis 0) or of row y (axis 1), thresholds thr[i * 7 +
+// d]; same stream and order as lg_init_kernel (the column form is how
+// src/transport.rs:238-251 builds its start states).
+__global__ void __launch_bounds__(256) lg_init_profile_kernel(LgGeom G, const uint64_t* __restrict cinit,
+                                                              const uint64_t* __restrict basis,
+                                                              const uint32_t* __restrict thr, int axis,
+                                                              uint64_t* __restrict planes,
+                                                              const uint64_t* __restrict solid) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G.total) return;
+  uint32_t y = g / G.wpr, j = g - y * G.wpr;
+  uint64_t tail = (j == G.lastword) ? G.tailmask : ~0ull;
+  uint64_t fm = ~solid[g] & tail;
+  uint64_t bits[7] = {0, 0, 0, 0, 0, 0, 0};
+  uint64_t st = fm ? lg_jump(cinit[g], basis) : 0;
+  for (uint32_t x = 0; x < 64; ++x) {
+    if (!((fm >> x) & 1ull)) continue;
+    const uint32_t* t = thr + (size_t)(axis == 0 ? j * 64 + x : y) * 7;
+    uint64_t s = 0;
+    for (int d = 0; d < 6; d += 2) {
+      st = lg_xs(st);
+      uint64_t r = st * LG_RNG_MUL;
+      s |= (uint64_t)((uint32_t)(r >> 40) < t[d]) << d;
+      s |= (uint64_t)(((uint32_t)(r >> 16) & 0xFFFFFFu) < t[d + 1]) << (d + 1);
+    }
+    if (G.rest) {
+      st = lg_xs(st);
+      uint64_t r = st * LG_RNG_MUL;
+      s |= (uint64_t)((uint32_t)(r >> 40) < t[6]) << 6;
+    }
+    for (int d = 0; d < 7; ++d) bits[d] |= ((s >> d) & 1ull) << x;
+  }
+#pragma unroll
+  for (int d = 0; d < 7; ++d) planes[(size_t)d * G.total + g] = bits[d];
+}
+
 // Per-row and per-column moment sums over every cell, obstacle sites
 // included (moments::WITH_WALLS, src/moments.rs:74): row y -> (mass, px2),
 // column x -> (mass, py2). Exact integers; transport::row_ux / column_uy
@@ -737,16 +774,20 @@ int lg_add_solid_disc(lg_t* g, float cx, float cy, float r) {
   return add_solid(g, 1, cx, cy, r, 0.0f);
 }
 
-int lg_init_equilibrium(lg_t* g, float density, float ux, float uy) {
-  if (!g) return fail(LG_EINVAL, "null handle");
+}  // extern "C"
+
+namespace {
+
+// Start of every word in the init stream (seed ^ 0xA5A51234, 4 draws per
+// fluid cell, 3 without rest, src/lattice.rs:197-208) and the stream basis.
+int init_stream(lg_t* g, std::vector<uint64_t>& cinit, uint64_t basis[64]) {
   std::vector<uint64_t> s;
   int rc = download_solid(g, s);
   if (rc) return rc;
   const LgGeom& G0 = g->G;
-  // per-word start in the init stream: 4 (3) draws per fluid cell before it
   const uint64_t p = lgca::charpoly_low();
   const uint32_t per = G0.rest ? 4 : 3;
-  std::vector<uint64_t> cinit((size_t)G0.total);
+  cinit.assign((size_t)G0.total, 0);
   uint64_t c = 1;
   for (uint32_t y = 0; y < G0.h; ++y)
     for (uint32_t j = 0; j < G0.wpr; ++j) {
@@ -755,13 +796,25 @@ int lg_init_equilibrium(lg_t* g, float density, float ux, float uy) {
       uint64_t tail = (j == G0.lastword) ? G0.tailmask : ~0ull;
       c = lgca::poly_mulz(c, (uint32_t)__builtin_popcountll(~s[word] & tail) * per, p);
     }
-  uint64_t basis[64];
   uint64_t st = lg_rng_init(g->seed ^ 0xA5A51234ull);
   for (int i = 0; i < 64; ++i) {
     basis[i] = st;
     st = lg_xs(st);
   }
-  LgGeom G = G0;
+  return LG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lg_init_equilibrium(lg_t* g, float density, float ux, float uy) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  std::vector<uint64_t> cinit;
+  uint64_t basis[64];
+  int rc = init_stream(g, cinit, basis);
+  if (rc) return rc;
+  LgGeom G = g->G;
   equilibrium_thresholds(density, ux, uy, G.rest != 0, G.thresh);
   DevBuf<uint64_t> dc, db;
   HIP_TRY(dc.alloc(cinit.size()));
@@ -770,6 +823,33 @@ int lg_init_equilibrium(lg_t* g, float density, float ux, float uy) {
   HIP_TRY(hipMemcpyAsync(db.p, basis, 64 * 8, hipMemcpyHostToDevice, g->stream));
   hipLaunchKernelGGL(lg_init_kernel, dim3(blocks_for(G.total)), dim3(256), 0, g->stream, G, dc.p,
                      db.p, g->a.p, g->solid.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->loaded = true;
+  return LG_OK;
+}
+
+int lg_init_profile(lg_t* g, int axis, float density, const float* ux, const float* uy, size_t n) {
+  if (!g || !ux || !uy) return fail(LG_EINVAL, "null argument");
+  if (axis != 0 && axis != 1) return fail(LG_EINVAL, "axis must be 0 (columns) or 1 (rows)");
+  if (n != (axis == 0 ? g->G.w : g->G.h)) return fail(LG_EINVAL, "profile length differs from the lattice");
+  std::vector<uint32_t> thr(n * 7);
+  for (size_t i = 0; i < n; ++i)
+    equilibrium_thresholds(density, ux[i], uy[i], g->G.rest != 0, &thr[i * 7]);
+  std::vector<uint64_t> cinit;
+  uint64_t basis[64];
+  int rc = init_stream(g, cinit, basis);
+  if (rc) return rc;
+  DevBuf<uint64_t> dc, db;
+  DevBuf<uint32_t> dt;
+  HIP_TRY(dc.alloc(cinit.size()));
+  HIP_TRY(db.alloc(64));
+  HIP_TRY(dt.alloc(thr.size()));
+  HIP_TRY(hipMemcpyAsync(dc.p, cinit.data(), cinit.size() * 8, hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipMemcpyAsync(db.p, basis, 64 * 8, hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipMemcpyAsync(dt.p, thr.data(), thr.size() * 4, hipMemcpyHostToDevice, g->stream));
+  hipLaunchKernelGGL(lg_init_profile_kernel, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream,
+                     g->G, dc.p, db.p, dt.p, axis, g->a.p, g->solid.p);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->loaded = true;

@@ -62,6 +62,7 @@ def load_library(path: str = LIB_PATH):
         "lg_add_solid_box": (i32, [vp, f32, f32, f32, f32]),
         "lg_add_solid_disc": (i32, [vp, f32, f32, f32]),
         "lg_init_equilibrium": (i32, [vp, f32, f32, f32]),
+        "lg_init_profile": (i32, [vp, i32, f32, P(f32), P(f32), sz]),
         "lg_batch_create": (vp, [P(vp), sz]),
         "lg_batch_destroy": (None, [vp]),
         "lg_batch_advance": (i32, [vp, u64]),
@@ -153,6 +154,15 @@ class GpuLattice:
     # src/lattice.rs:197-208, on the device
     def init_equilibrium(self, density, ux, uy):
         _check(load_library().lg_init_equilibrium(self._p, float(density), float(ux), float(uy)))
+
+    def init_profile(self, axis: int, density: float, ux, uy):
+        """init_equilibrium with a per-column (axis 0) or per-row (axis 1)
+        velocity profile."""
+        a = np.ascontiguousarray(ux, dtype=np.float32)
+        b = np.ascontiguousarray(uy, dtype=np.float32)
+        _check(load_library().lg_init_profile(self._p, int(axis), float(density),
+                                              a.ctypes.data_as(C.POINTER(C.c_float)),
+                                              b.ctypes.data_as(C.POINTER(C.c_float)), a.size))
 
     # src/gpu.rs:572-610
     def load(self, cells: np.ndarray):

@@ -74,12 +74,35 @@ def config2(bands: int = 1) -> dict:
                 sample_every=100)
 
 
+def shear_profile(w: int, amp: float = 0.05):
+    """u_y(x) = amp * sin(2 pi x / W) in f32, as src/transport.rs:234-242
+    evaluates its column profile (k = 2 pi / W in f32, then k * x)."""
+    k = np.float32(np.float32(2.0) * np.float32(np.pi) / np.float32(w))
+    x = np.arange(w, dtype=np.float32)
+    return (np.float32(amp) * np.sin((k * x).astype(np.float32))).astype(np.float32)
+
+
+def config3(bands: int = 1, w: int = 32768, h: int = 16384) -> dict:
+    """Transport-measure workload: periodic w x h lattice (default 32768 x
+    16384, ~537M cells), no obstacle, no inflow, density 0.3 per link with
+    rest particles, transverse shear wave u_y = 0.05 sin(2 pi x / W) imposed by
+    biased link occupancy from seed 7 (init stream of src/lattice.rs:197-208,
+    one equilibrium per column), 4000 steps."""
+    return dict(name=f"fhp-{w}x{h}-shear", w=w, h=h, rest=True, density=0.3, init_u=(0.0, 0.0),
+                inflow=(0.0, 0.0), inlet=0, seed=7, bands=bands, plate=None, steps=4000,
+                profile="shear")
+
+
 def config_gpu(cfg: dict) -> GpuLattice:
     g = GpuLattice(cfg["w"], cfg["h"], cfg["rest"], cfg["seed"], cfg["bands"])
     g.set_inlet(cfg["inlet"], cfg["density"], cfg["inflow"])
     if cfg.get("plate"):
         g.add_solid_box(*cfg["plate"])
-    g.init_equilibrium(cfg["density"], *cfg["init_u"])
+    if cfg.get("profile") == "shear":
+        uy = shear_profile(cfg["w"])
+        g.init_profile(0, cfg["density"], np.zeros_like(uy), uy)
+    else:
+        g.init_equilibrium(cfg["density"], *cfg["init_u"])
     return g
 
 

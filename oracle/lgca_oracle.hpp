@@ -310,6 +310,20 @@ struct Lattice {
       cells[i] = (cells[i] & SOLID_BIT) ? SOLID_BIT : eq.sample(rng);
   }
 
+  // init_equilibrium (src/lattice.rs:197-208) with one equilibrium per column
+  // (axis 0) or per row (axis 1), the column form being how
+  // src/transport.rs:238-251 builds its start states.
+  void init_profile(int axis, const float* ux, const float* uy) {
+    Rng rng(seed ^ 0xA5A51234ull);
+    size_t n = axis == 0 ? w : h;
+    std::vector<Equilibrium> eq;
+    for (size_t i = 0; i < n; ++i) eq.emplace_back(density, ux[i], uy[i], table.rest_particles);
+    for (size_t i = 0; i < cells.size(); ++i) {
+      size_t k = axis == 0 ? i % w : i / w;
+      cells[i] = (cells[i] & SOLID_BIT) ? SOLID_BIT : eq[k].sample(rng);
+    }
+  }
+
   // src/lattice.rs:210-220 with the plate predicate of tests/golden.rs:78-83
   // and src/main.rs:584-588: x0 <= px <= x1 && y0 <= py <= y1.
   void add_solid_box(float x0, float x1, float y0, float y1) {

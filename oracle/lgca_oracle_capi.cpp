@@ -32,6 +32,9 @@ void orc_set_inflow(void* p, float ux, float uy) {
   static_cast<Lattice*>(p)->inflow_y = uy;
 }
 void orc_init_equilibrium(void* p) { static_cast<Lattice*>(p)->init_equilibrium(); }
+void orc_init_profile(void* p, int axis, const float* ux, const float* uy) {
+  static_cast<Lattice*>(p)->init_profile(axis, ux, uy);
+}
 int orc_restart(void* p, const uint8_t* cells, size_t n, uint64_t seed) {
   auto* L = static_cast<Lattice*>(p);
   if (n != L->cells.size()) return -1;

@@ -45,6 +45,7 @@ def lib():
             "orc_add_solid_disc": (None, [vp, f32, f32, f32]),
             "orc_init_equilibrium": (None, [vp]),
             "orc_set_inflow": (None, [vp, f32, f32]),
+            "orc_init_profile": (None, [vp, i32, P(f32), P(f32)]),
             "orc_restart": (i32, [vp, P(C.c_uint8), sz, u64]),
             "orc_advance": (None, [vp, u64]),
             "orc_steps": (u64, [vp]),
@@ -210,6 +211,11 @@ class Lattice:
 
     def init_equilibrium(self):
         lib().orc_init_equilibrium(self._p)
+
+    def init_profile(self, axis, ux, uy):
+        a = np.ascontiguousarray(ux, dtype=np.float32)
+        b = np.ascontiguousarray(uy, dtype=np.float32)
+        lib().orc_init_profile(self._p, int(axis), _ptr(a, C.c_float), _ptr(b, C.c_float))
 
     def set_inflow(self, ux, uy):
         lib().orc_set_inflow(self._p, ux, uy)

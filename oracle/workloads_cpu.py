@@ -10,6 +10,13 @@ def config_oracle(O, cfg: dict):
                     cfg["bands"], cfg["seed"])
     if cfg.get("plate"):
         lat.add_solid_box(*cfg["plate"])
-    lat.init_equilibrium()
+    if cfg.get("profile") == "shear":
+        import numpy as np
+
+        from workloads import shear_profile  # the same f32 profile the engine gets
+        uy = shear_profile(cfg["w"])
+        lat.init_profile(0, np.zeros_like(uy), uy)
+    else:
+        lat.init_equilibrium()
     lat.set_inflow(*cfg["inflow"])
     return lat

@@ -43,9 +43,16 @@ def test_served_runs_equal_their_cpu_replay(oracle):
                    inflow=(r.drift, 0.0), inlet=r.inlet, seed=r.seed, bands=r.bands,
                    plate=W.plate_box(r.w, r.h, r.plate_x, r.plate_rows))
         lat = config_oracle(oracle, cfg)
-        lat.advance(r.steps)
-        assert np.array_equal(res[r.rid].final_cells, lat.cells)
+        # the same sampling history: Field::sample with alpha 1 every 50 steps
+        # and at the end, into one running field (src/render.rs:107-115)
         f = oracle.Field(r.w, r.h, 16, 16)
-        f.sample(lat, 1.0)
-        for x, y in zip(res[r.rid].snapshots[-1][1:], f.read()):
-            assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+        done = 0
+        snaps = res[r.rid].snapshots
+        for i, snap in enumerate(snaps):
+            lat.advance(snap[0] - done)
+            done = snap[0]
+            f.sample(lat, 1.0)
+            for x, y in zip(snap[1:], f.read()):
+                assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), (r.rid, i)
+        assert done == r.steps
+        assert np.array_equal(res[r.rid].final_cells, lat.cells)
