@@ -162,6 +162,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--bands", type=int, default=0, help="CPU bands (default: host threads, <= 16)")
+    ap.add_argument("--cells-per-lane", type=int, default=64, choices=(64, 32),
+                    help="step-kernel work unit: a whole u64 word or half of it")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--ensemble", type=int, default=256, metavar="N",
@@ -203,6 +205,7 @@ def main():
     every = cfg["sample_every"]
 
     g = W.config_gpu(cfg)
+    g.set_work_unit(args.cells_per_lane)
     g.attach_field(cfg["block"], cfg["block"])
     if args.warmup:
         g.advance_sampling(args.warmup, every, 1.0)
@@ -232,7 +235,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": cfg["name"], "w": cfg["w"], "h": cfg["h"], "cells": cells,
                    "rest": True, "bands": bands, "inlet_cols": cfg["inlet"],
-                   "sample_every": every, "block": cfg["block"], "seed": cfg["seed"]},
+                   "sample_every": every, "block": cfg["block"], "seed": cfg["seed"],
+                   "cells_per_lane": args.cells_per_lane},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "lg_step_kernel", "algorithmic_bytes_per_launch": int(traffic_alg),

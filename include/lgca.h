@@ -107,6 +107,11 @@ uint64_t lg_steps(lg_t* g);
  * run exactly (the reference can only restart at step 0). */
 int lg_set_steps(lg_t* g, uint64_t steps);
 
+/* Launch shape of the step: each lane steps 64 cells (a whole u64 word) or
+ * 32 (half a word, twice the lanes: more parallelism for lattices too small
+ * to fill the GPU). The results are identical either way. */
+int lg_set_work_unit(lg_t* g, uint32_t cells_per_lane);
+
 /* New: advance n steps and report the device time of the step work, measured
  * with HIP events on the handle's stream around the step launches. */
 int lg_advance_timed(lg_t* g, uint64_t n, float* ms);
@@ -157,17 +162,29 @@ typedef struct lg_batch lg_batch_t;
 lg_batch_t* lg_batch_create(lg_t* const* members, size_t n);
 void lg_batch_destroy(lg_batch_t* b);
 int lg_batch_advance(lg_batch_t* b, uint64_t n);
+/* Advance member i by min(n, limits[i]) steps in the same launches (members
+ * with fewer steps left stop early); limits has one entry per member. */
+int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits);
+/* Sample every member's attached field (lg_sample_field with alpha) in one
+ * launch and copy all of them out: for each member with a field, in member
+ * order, its ux, uy, rho, solid arrays back to back. nfloats must equal
+ * lg_batch_snapshot_size(b). */
+int lg_batch_snapshot(lg_batch_t* b, float alpha, float* out, size_t nfloats);
+size_t lg_batch_snapshot_size(lg_batch_t* b);
 /* Launch shape: tile = words per workgroup (128 or 256), grid = step-kernel
  * workgroups (0: one per tile; otherwise a persistent grid walking the
  * tiles), chunk = steps per graph replay (even, at most 256). */
 int lg_batch_set_shape(lg_batch_t* b, uint32_t tile, uint32_t grid, uint32_t chunk);
 int lg_batch_get_shape(lg_batch_t* b, uint32_t* tile, uint32_t* grid, uint32_t* chunk);
+/* Cells per lane (64 or 32) for the batch's step kernel, as lg_set_work_unit. */
+int lg_batch_set_units(lg_batch_t* b, uint32_t cells_per_lane);
+int lg_batch_get_units(lg_batch_t* b, uint32_t* cells_per_lane);
 /* lg_sample_field for every member with a field, issued together. */
 int lg_batch_sample(lg_batch_t* b, float alpha);
 /* lg_sample_field without the wait (the sample is queued on the handle's
  * stream; the next synchronous call on the handle sees it). */
 int lg_sample_field_async(lg_t* g, float alpha);
-/* Time each candidate shape on the batch itself (those steps are real steps
+/* Time each candidate shape (tile, grid, chunk, cells per lane) on the batch itself (those steps are real steps
  * of every member, reported in *used_steps) and keep the fastest.
  * budget_steps = 0: no limit. */
 int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps);

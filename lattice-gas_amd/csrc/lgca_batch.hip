@@ -28,11 +28,18 @@ struct LgDesc {
   const uint64_t* cjump;
   const uint32_t* row_band;
   const uint64_t* ijump;
+  const uint32_t* order;
   uint64_t* tabs;  // >= K * (nbands + 1) * LG_WTAB
   uint64_t seed;
-  uint64_t steps;  // step index of the current state
-  uint32_t cur;    // plane set holding the current state
+  uint64_t steps;      // step index of the current state
+  uint64_t remaining;  // steps this member still takes in the current call
+  uint32_t cur;        // plane set holding the current state
   uint32_t pad;
+  // coarse field (src/render.rs:21-48); fux == nullptr when none is attached
+  float *fux, *fuy, *frho, *fsol;
+  int64_t* fsums;
+  uint32_t bx, by, bw, bh;
+  uint64_t snap_off;   // offset of this member's snapshot in the staging buffer
 };
 
 // Stream tables for members blockIdx.y, steps steps..steps+nsteps-1.
@@ -44,6 +51,7 @@ __global__ void __launch_bounds__(256) lg_tables_batch_kernel(const LgDesc* __re
   if (g >= nsteps * per * 16) return;
   uint32_t grp = g & 15u, ks = g >> 4;
   uint32_t k = ks / per, s = ks - k * per;
+  if (k >= d.remaining) return;
   uint64_t t = d.steps + k;
   uint64_t st = lg_rng_init(s < d.G.nbands ? lg_band_seed(d.seed, t, s) : lg_inlet_seed(d.seed, t + 1));
   for (uint32_t i = 0; i < 4 * grp; ++i) st = lg_xs(st);
@@ -60,41 +68,54 @@ __global__ void __launch_bounds__(256) lg_tables_batch_kernel(const LgDesc* __re
 }
 
 // Step k of the chunk for every tile. A tile is TILE consecutive thread
-// indices of one member, mapped to words exactly as lg_step_kernel maps them
-// (inflow words first).
-template <int TILE>
-__global__ void __launch_bounds__(TILE) lg_step_batch_kernel(const LgDesc* __restrict D,
+// indices of one member, mapped to work units exactly as lg_step_kernel maps
+// them (the member's static word order, NH units per word).
+template <int TILE, int NH>
+__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) lg_step_batch_kernel(const LgDesc* __restrict D,
                                                              const uint2* __restrict tiles,
                                                              uint32_t ntiles, uint32_t k) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const uint2 tl = tiles[t];
     const LgDesc& d = D[tl.x];
     const uint32_t g = tl.y + threadIdx.x;
-    if (g >= d.G.total) continue;
-    uint32_t y, j;
-    const uint32_t ninlet = d.G.h * d.G.iw;
-    if (g < ninlet) {
-      y = g / d.G.iw;
-      j = g - y * d.G.iw;
-    } else {
-      const uint32_t per = d.G.wpr - d.G.iw;
-      const uint32_t q = g - ninlet;
-      y = q / per;
-      j = d.G.iw + (q - y * per);
-    }
+    if (k >= d.remaining || g >= d.G.total * NH) continue;
+    const uint32_t e = d.order[NH == 1 ? g : g >> 1];
     const uint32_t par = (d.cur + k) & 1u;
     const uint64_t* src = par ? d.p1 : d.p0;
     uint64_t* dst = par ? d.p0 : d.p1;
     const uint64_t* tabs = d.tabs + (size_t)k * (d.G.nbands + 1) * LG_WTAB;
-    lg_step_word(d.G, src, dst, d.solid, d.cjump, d.row_band, d.ijump, tabs, y, j);
+    lg_step_unit<NH>(d.G, src, dst, d.solid, d.cjump, d.row_band, d.ijump, tabs, e >> 16,
+                     e & 0xFFFFu, NH == 1 ? 0u : (g & 1u));
   }
 }
 
 __global__ void lg_bump_kernel(LgDesc* __restrict D, uint32_t n, uint32_t count) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  D[i].steps += count;
-  D[i].cur ^= (count & 1u);
+  uint64_t adv = D[i].remaining < count ? D[i].remaining : count;
+  D[i].steps += adv;
+  D[i].cur ^= (uint32_t)(adv & 1u);
+  D[i].remaining -= adv;
+}
+
+// Coarse-grain every member with a field (blockIdx.y = member), exactly as
+// lg_sample_kernel does, and copy the result (ux, uy, rho, solid arrays of
+// the member, back to back) into the staging buffer for one read-back.
+__global__ void __launch_bounds__(256) lg_snapshot_kernel(const LgDesc* __restrict D, float alpha,
+                                                          float* __restrict staging) {
+  const LgDesc& d = D[blockIdx.y];
+  if (!d.fux) return;
+  const uint32_t nb = d.bw * d.bh;
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nb) return;
+  const uint64_t* planes = d.cur ? d.p1 : d.p0;
+  lg_sample_block(d.G, planes, d.solid, d.bx, d.by, d.bw, alpha, d.fux, d.fuy, d.frho, d.fsol,
+                  d.fsums, g);
+  float* out = staging + d.snap_off;
+  out[g] = d.fux[g];
+  out[nb + g] = d.fuy[g];
+  out[2 * nb + g] = d.frho[g];
+  out[3 * nb + g] = d.fsol[g];
 }
 
 struct lg_batch {
@@ -103,19 +124,26 @@ struct lg_batch {
   DevBuf<LgDesc> desc;
   DevBuf<uint2> tiles128, tiles256;
   uint32_t ntiles128 = 0, ntiles256 = 0;
+  uint32_t tiles_units = 0;  // NH the tile tables were built for
   uint32_t max_tab_blocks_per_step = 0;  // tables launch width per step of chunk
   uint32_t kmax = 0;                     // largest chunk the members' tables hold
   // tuned launch shape
   uint32_t tile = 256;   // words per tile (128 or 256)
   uint32_t grid = 0;     // step-kernel workgroups; 0 = one per tile
   uint32_t chunk = 32;   // steps per graph replay (even)
+  uint32_t units = 1;    // 1: a lane steps 64 cells, 2: 32 cells
   // graph for `chunk` steps with the current shape
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  uint32_t graph_chunk = 0, graph_tile = 0, graph_grid = 0;
+  uint32_t graph_chunk = 0, graph_tile = 0, graph_grid = 0, graph_units = 0;
   int ncu = 256;
   // host staging of the last uploads, alive until the stream has consumed them
   std::vector<LgDesc> host_desc;
+  std::vector<uint64_t> advanced;  // per member, steps taken by the last call
+  DevBuf<float> staging;           // snapshots of every member's field
+  size_t staging_floats = 0;
+  uint32_t max_field_blocks = 0;
+  uint32_t calls = 0;              // advance calls on this membership
   std::vector<uint2> host_tiles128, host_tiles256;
 };
 
@@ -129,9 +157,10 @@ void drop_graph(lg_batch* b) {
   b->graph_chunk = 0;
 }
 
-int upload_desc(lg_batch* b) {
+int upload_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
   std::vector<LgDesc>& h = b->host_desc;
   h.assign(b->members.size(), LgDesc{});
+  size_t off = 0;
   for (size_t i = 0; i < h.size(); ++i) {
     lg_t* g = b->members[i];
     LgDesc& d = h[i];
@@ -143,11 +172,27 @@ int upload_desc(lg_batch* b) {
     d.cjump = g->cjump.p;
     d.row_band = g->row_band.p;
     d.ijump = g->ijump.p;
+    d.order = g->order.p;
     d.tabs = g->basis.p;
     d.seed = g->seed;
     d.steps = g->steps;
+    d.remaining = limits ? std::min<uint64_t>(limits[i], n) : n;
     d.cur = 0;
+    if (g->has_field) {
+      d.fux = g->fux.p;
+      d.fuy = g->fuy.p;
+      d.frho = g->frho.p;
+      d.fsol = g->fsol.p;
+      d.fsums = g->fsums.p;
+      d.bx = g->bx;
+      d.by = g->by;
+      d.bw = g->bw;
+      d.bh = g->bh;
+      d.snap_off = off;
+      off += (size_t)4 * g->bw * g->bh;
+    }
   }
+  b->staging_floats = off;
   HIP_TRY(hipMemcpyAsync(b->desc.p, h.data(), h.size() * sizeof(LgDesc), hipMemcpyHostToDevice,
                          b->stream));
   return LG_OK;
@@ -157,7 +202,7 @@ template <int TILE>
 int build_tiles(lg_batch* b, DevBuf<uint2>& buf, uint32_t& n, std::vector<uint2>& t) {
   t.clear();
   for (size_t i = 0; i < b->members.size(); ++i) {
-    uint32_t total = b->members[i]->G.total;
+    uint32_t total = b->members[i]->G.total * b->units;
     for (uint32_t g0 = 0; g0 < total; g0 += TILE) t.push_back(make_uint2((uint32_t)i, g0));
   }
   n = (uint32_t)t.size();
@@ -178,12 +223,18 @@ int enqueue_chunk(lg_batch* b, uint32_t count) {
   const uint2* tiles = b->tile == 128 ? b->tiles128.p : b->tiles256.p;
   uint32_t grid = b->grid ? std::min(b->grid, ntiles) : ntiles;
   for (uint32_t k = 0; k < count; ++k) {
-    if (b->tile == 128)
-      hipLaunchKernelGGL(lg_step_batch_kernel<128>, dim3(grid), dim3(128), 0, b->stream, b->desc.p,
-                         tiles, ntiles, k);
+    if (b->tile == 128 && b->units == 1)
+      hipLaunchKernelGGL((lg_step_batch_kernel<128, 1>), dim3(grid), dim3(128), 0, b->stream,
+                         b->desc.p, tiles, ntiles, k);
+    else if (b->tile == 128)
+      hipLaunchKernelGGL((lg_step_batch_kernel<128, 2>), dim3(grid), dim3(128), 0, b->stream,
+                         b->desc.p, tiles, ntiles, k);
+    else if (b->units == 1)
+      hipLaunchKernelGGL((lg_step_batch_kernel<256, 1>), dim3(grid), dim3(256), 0, b->stream,
+                         b->desc.p, tiles, ntiles, k);
     else
-      hipLaunchKernelGGL(lg_step_batch_kernel<256>, dim3(grid), dim3(256), 0, b->stream, b->desc.p,
-                         tiles, ntiles, k);
+      hipLaunchKernelGGL((lg_step_batch_kernel<256, 2>), dim3(grid), dim3(256), 0, b->stream,
+                         b->desc.p, tiles, ntiles, k);
   }
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(lg_bump_kernel, dim3(blocks_for(nm)), dim3(256), 0, b->stream, b->desc.p, nm,
@@ -192,8 +243,19 @@ int enqueue_chunk(lg_batch* b, uint32_t count) {
   return LG_OK;
 }
 
+int ensure_tiles(lg_batch* b) {
+  if (b->tiles_units == b->units) return LG_OK;
+  int rc = build_tiles<128>(b, b->tiles128, b->ntiles128, b->host_tiles128);
+  if (rc) return rc;
+  rc = build_tiles<256>(b, b->tiles256, b->ntiles256, b->host_tiles256);
+  if (rc) return rc;
+  b->tiles_units = b->units;
+  return LG_OK;
+}
+
 int ensure_graph(lg_batch* b) {
-  if (b->exec && b->graph_chunk == b->chunk && b->graph_tile == b->tile && b->graph_grid == b->grid)
+  if (b->exec && b->graph_chunk == b->chunk && b->graph_tile == b->tile &&
+      b->graph_grid == b->grid && b->graph_units == b->units)
     return LG_OK;
   drop_graph(b);
   HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -210,11 +272,14 @@ int ensure_graph(lg_batch* b) {
   b->graph_chunk = b->chunk;
   b->graph_tile = b->tile;
   b->graph_grid = b->grid;
+  b->graph_units = b->units;
   return LG_OK;
 }
 
 // Advance every member by n steps; the descriptors must be current.
 int run_steps(lg_batch* b, uint64_t n, bool use_graph) {
+  int trc = ensure_tiles(b);
+  if (trc) return trc;
   uint64_t left = n;
   if (use_graph && left >= b->chunk) {
     int rc = ensure_graph(b);
@@ -233,10 +298,14 @@ int run_steps(lg_batch* b, uint64_t n, bool use_graph) {
   return LG_OK;
 }
 
-void commit_members(lg_batch* b, uint64_t n) {
-  for (lg_t* g : b->members) {
-    g->steps += n;
-    if (n & 1u) std::swap(g->a.p, g->b.p);
+void commit_members(lg_batch* b, const uint64_t* limits, uint64_t n) {
+  b->advanced.assign(b->members.size(), 0);
+  for (size_t i = 0; i < b->members.size(); ++i) {
+    lg_t* g = b->members[i];
+    uint64_t adv = limits ? std::min<uint64_t>(limits[i], n) : n;
+    g->steps += adv;
+    if (adv & 1u) std::swap(g->a.p, g->b.p);
+    b->advanced[i] = adv;
   }
 }
 
@@ -275,9 +344,7 @@ lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
   b->chunk = std::min<uint32_t>(32, b->kmax & ~1u);
   if (b->chunk == 0) b->chunk = 2;
   if (b->desc.alloc(n)) return bail("hipMalloc failed for descriptors");
-  if (build_tiles<128>(b, b->tiles128, b->ntiles128, b->host_tiles128) ||
-      build_tiles<256>(b, b->tiles256, b->ntiles256, b->host_tiles256))
-    return bail(g_err);
+  if (ensure_tiles(b)) return bail(g_err);
   if (hipStreamSynchronize(b->stream) != hipSuccess) return bail("hipStreamSynchronize failed");
   return b;
 }
@@ -293,16 +360,58 @@ void lg_batch_destroy(lg_batch_t* b) {
   delete b;
 }
 
-int lg_batch_advance(lg_batch_t* b, uint64_t n) {
+int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits) {
   if (!b) return fail(LG_EINVAL, "null batch");
   if (n == 0) return LG_OK;
   for (lg_t* g : b->members) HIP_TRY(hipStreamSynchronize(g->stream));
-  int rc = upload_desc(b);
+  int rc = upload_desc(b, limits, n);
   if (rc) return rc;
-  rc = run_steps(b, n, true);
+  // a fresh membership runs its first call eagerly; from the second call on
+  // the chunk is captured once and replayed
+  rc = run_steps(b, n, b->calls > 0);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
-  commit_members(b, n);
+  commit_members(b, limits, n);
+  b->calls += 1;
+  return LG_OK;
+}
+
+int lg_batch_advance(lg_batch_t* b, uint64_t n) { return lg_batch_advance_limited(b, n, nullptr); }
+
+int lg_batch_snapshot(lg_batch_t* b, float alpha, float* out, size_t nfloats) {
+  if (!b || !out) return fail(LG_EINVAL, "null argument");
+  for (lg_t* g : b->members) HIP_TRY(hipStreamSynchronize(g->stream));
+  int rc = upload_desc(b, nullptr, 0);
+  if (rc) return rc;
+  if (nfloats != b->staging_floats)
+    return fail(LG_EINVAL, "output size differs from the members' fields (4 x blocks each)");
+  if (nfloats == 0) return LG_OK;
+  if (b->staging.n < nfloats) HIP_TRY(b->staging.alloc(nfloats));
+  uint32_t maxnb = 0;
+  for (lg_t* g : b->members)
+    if (g->has_field) maxnb = std::max(maxnb, g->bw * g->bh);
+  dim3 grid(blocks_for(maxnb), (uint32_t)b->members.size());
+  hipLaunchKernelGGL(lg_snapshot_kernel, grid, dim3(256), 0, b->stream, b->desc.p, alpha,
+                     b->staging.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, b->staging.p, nfloats * 4, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return LG_OK;
+}
+
+size_t lg_batch_snapshot_size(lg_batch_t* b) {
+  if (!b) return 0;
+  size_t n = 0;
+  for (lg_t* g : b->members)
+    if (g->has_field) n += (size_t)4 * g->bw * g->bh;
+  return n;
+}
+
+int lg_batch_set_units(lg_batch_t* b, uint32_t cells_per_lane) {
+  if (!b) return fail(LG_EINVAL, "null batch");
+  if (cells_per_lane != 64 && cells_per_lane != 32)
+    return fail(LG_EINVAL, "cells per lane must be 64 or 32");
+  b->units = cells_per_lane == 64 ? 1u : 2u;
   return LG_OK;
 }
 
@@ -314,6 +423,12 @@ int lg_batch_set_shape(lg_batch_t* b, uint32_t tile, uint32_t grid, uint32_t chu
   b->tile = tile;
   b->grid = grid;
   b->chunk = chunk;
+  return LG_OK;
+}
+
+int lg_batch_get_units(lg_batch_t* b, uint32_t* cells_per_lane) {
+  if (!b || !cells_per_lane) return fail(LG_EINVAL, "null argument");
+  *cells_per_lane = b->units == 1 ? 64u : 32u;
   return LG_OK;
 }
 
@@ -329,21 +444,22 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
   if (!b || !used_steps) return fail(LG_EINVAL, "null argument");
   *used_steps = 0;
   for (lg_t* g : b->members) HIP_TRY(hipStreamSynchronize(g->stream));
-  int rc = upload_desc(b);
+  int rc = upload_desc(b, nullptr, ~0ull);
   if (rc) return rc;
   struct Cand {
-    uint32_t tile, grid, chunk;
+    uint32_t tile, grid, chunk, units;
   };
   std::vector<Cand> cands;
-  for (uint32_t tile : {256u, 128u})
-    for (uint32_t m : {0u, 4u, 8u, 16u})
-      for (uint32_t chunk : {8u, 32u}) {
-        uint32_t c = std::min(chunk, b->kmax & ~1u);
-        if (c == 0) continue;
-        cands.push_back({tile, m ? m * (uint32_t)b->ncu : 0u, c});
-      }
+  for (uint32_t units : {1u, 2u})
+    for (uint32_t tile : {256u, 128u})
+      for (uint32_t m : {0u, 4u, 16u})
+        for (uint32_t chunk : {8u, 32u}) {
+          uint32_t c = std::min(chunk, b->kmax & ~1u);
+          if (c == 0) continue;
+          cands.push_back({tile, m ? m * (uint32_t)b->ncu : 0u, c, units});
+        }
   double best = 1e300;
-  Cand bestc = {b->tile, b->grid, b->chunk};
+  Cand bestc = {b->tile, b->grid, b->chunk, b->units};
   uint64_t used = 0;
   for (const Cand& c : cands) {
     uint64_t trial = 2ull * c.chunk;
@@ -351,6 +467,7 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
     b->tile = c.tile;
     b->grid = c.grid;
     b->chunk = c.chunk;
+    b->units = c.units;
     if ((rc = run_steps(b, trial, true))) return rc;  // warm (captures the graph)
     HIP_TRY(hipStreamSynchronize(b->stream));
     auto t0 = std::chrono::steady_clock::now();
@@ -364,10 +481,11 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
       bestc = c;
     }
   }
-  commit_members(b, used);
+  commit_members(b, nullptr, used);
   b->tile = bestc.tile;
   b->grid = bestc.grid;
   b->chunk = bestc.chunk;
+  b->units = bestc.units;
   *used_steps = used;
   return LG_OK;
 }

@@ -136,6 +136,60 @@ struct LgGeom {
 };
 
 // ---------------------------------------------------------------------------
+// Coarse-graining of block g of a bw-wide grid of bx x by blocks: exact
+// integer sums over fluid cells (src/moments.rs:37-65 FLUID,
+// src/render.rs:93-106), then the f32 running average of src/render.rs:107-115
+// with every operation rounded on its own (no fused multiply-add), as the CPU
+// computes it. sums[4 g ..] receives (mass, px2, py2, nsolid).
+#if defined(__HIPCC__)
+__device__ inline void lg_sample_block(const LgGeom& G, const uint64_t* __restrict planes,
+                                       const uint64_t* __restrict solid, uint32_t bx, uint32_t by,
+                                       uint32_t bw, float alpha, float* __restrict ux,
+                                       float* __restrict uy, float* __restrict rho,
+                                       float* __restrict sol, int64_t* __restrict sums, uint32_t g) {
+  uint32_t jb = g / bw, ib = g - jb * bw;
+  const int CX2[6] = {2, 1, -1, -2, -1, 1};
+  const int CY2[6] = {0, 1, 1, 0, -1, -1};
+  uint32_t x0 = ib * bx, x1 = x0 + bx;
+  uint32_t w0 = x0 >> 6, w1 = (x1 - 1) >> 6;
+  int64_t mass = 0, px = 0, py = 0, ns = 0;
+  for (uint32_t y = jb * by; y < (jb + 1) * by; ++y) {
+    size_t base = (size_t)y * G.wpr;
+    for (uint32_t wi = w0; wi <= w1; ++wi) {
+      uint32_t lo = max(x0, wi << 6) - (wi << 6);
+      uint32_t hi = min(x1, (wi + 1) << 6) - (wi << 6);
+      uint64_t m = (hi - lo >= 64) ? ~0ull : (((1ull << (hi - lo)) - 1ull) << lo);
+      uint64_t sm = solid[base + wi] & m;
+      ns += __popcll(sm);
+      uint64_t fm = m & ~sm;
+      for (int d = 0; d < 6; ++d) {
+        int64_t c = __popcll(planes[(size_t)d * G.total + base + wi] & fm);
+        mass += c;
+        px += CX2[d] * c;
+        py += CY2[d] * c;
+      }
+      mass += __popcll(planes[(size_t)6 * G.total + base + wi] & fm);
+    }
+  }
+  sums[(size_t)g * 4 + 0] = mass;
+  sums[(size_t)g * 4 + 1] = px;
+  sums[(size_t)g * 4 + 2] = py;
+  sums[(size_t)g * 4 + 3] = ns;
+  const float SQRT3_2 = 0.8660254f;
+  float n = (float)(bx * by);
+  float vx = 0.0f, vy = 0.0f;
+  if (mass != 0) {
+    float mf = (float)mass;
+    vx = __fdiv_rn(__fmul_rn((float)px, 0.5f), mf);
+    vy = __fdiv_rn(__fmul_rn((float)py, SQRT3_2), mf);
+  }
+  ux[g] = __fadd_rn(ux[g], __fmul_rn(alpha, __fsub_rn(vx, ux[g])));
+  uy[g] = __fadd_rn(uy[g], __fmul_rn(alpha, __fsub_rn(vy, uy[g])));
+  rho[g] = __fadd_rn(rho[g], __fmul_rn(alpha, __fsub_rn(__fdiv_rn((float)mass, n), rho[g])));
+  sol[g] = __fdiv_rn((float)ns, n);
+}
+#endif
+
 // Collision draws for one word: the binary digits of k for the 64 cells.
 // ---------------------------------------------------------------------------
 // Fast path (every cell draws). Cell x's digits are OR-ed into accumulator
@@ -152,10 +206,19 @@ LG_HD inline uint64_t lg_draw_half(uint64_t st, uint32_t stride, uint32_t P[NK])
   for (uint32_t x = 0; x < 32; ++x) {
     st = lg_xs(st);
     uint32_t k = lg_kcol(lg_out32(st), stride);
-    if (x + NK <= 32)
+    if (x + NK <= 32) {
+#if defined(__HIP_DEVICE_COMPILE__)
+      // one shift-or per cell (the compiler otherwise emits a funnel shift
+      // and a masked or, not knowing k < 2^NK)
+      uint32_t t;
+      asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(t) : "v"(k), "i"(x), "v"(R[x & 7]));
+      R[x & 7] = t;
+#else
       R[x & 7] |= k << x;
-    else
+#endif
+    } else {
       R[x & 7] |= (k << x) | (k >> (32 - x));
+    }
   }
   LG_UNROLL
   for (int b = 0; b < NK; ++b) {
@@ -168,6 +231,52 @@ LG_HD inline uint64_t lg_draw_half(uint64_t st, uint32_t stride, uint32_t P[NK])
     P[b] = acc;
   }
   return st;
+}
+
+// Both halves of a word at once, as two independent chains (the upper one
+// started by its own jump): the same digits as two lg_draw_half calls, with
+// twice the independent work per lane to hide each chain's latency.
+template <int NK>
+LG_HD inline void lg_draw_pair(uint64_t s0, uint64_t s1, uint32_t stride, uint32_t P0[NK],
+                               uint32_t P1[NK]) {
+  uint32_t R0[8], R1[8];
+  LG_UNROLL
+  for (int m = 0; m < 8; ++m) R0[m] = R1[m] = 0;
+  LG_UNROLL
+  for (uint32_t x = 0; x < 32; ++x) {
+    s0 = lg_xs(s0);
+    s1 = lg_xs(s1);
+    uint32_t k0 = lg_kcol(lg_out32(s0), stride);
+    uint32_t k1 = lg_kcol(lg_out32(s1), stride);
+    if (x + NK <= 32) {
+#if defined(__HIP_DEVICE_COMPILE__)
+      uint32_t t0, t1;
+      asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(t0) : "v"(k0), "i"(x), "v"(R0[x & 7]));
+      asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(t1) : "v"(k1), "i"(x), "v"(R1[x & 7]));
+      R0[x & 7] = t0;
+      R1[x & 7] = t1;
+#else
+      R0[x & 7] |= k0 << x;
+      R1[x & 7] |= k1 << x;
+#endif
+    } else {
+      R0[x & 7] |= (k0 << x) | (k0 >> (32 - x));
+      R1[x & 7] |= (k1 << x) | (k1 >> (32 - x));
+    }
+  }
+  LG_UNROLL
+  for (int b = 0; b < NK; ++b) {
+    uint32_t a0 = 0, a1 = 0;
+    LG_UNROLL
+    for (int m = 0; m < 8; ++m) {
+      uint32_t r0 = b ? ((R0[m] >> b) | (R0[m] << (32 - b))) : R0[m];
+      uint32_t r1 = b ? ((R1[m] >> b) | (R1[m] << (32 - b))) : R1[m];
+      a0 |= r0 & (0x01010101u << m);
+      a1 |= r1 & (0x01010101u << m);
+    }
+    P0[b] = a0;
+    P1[b] = a1;
+  }
 }
 
 // General path: only the cells in `fm` draw, in order (solid sites and the
@@ -191,6 +300,17 @@ LG_HD inline void lg_draw_masked(uint64_t st, uint64_t fm, uint32_t stride, uint
 }
 
 template <int NK>
+LG_HD inline void lg_draw_masked32(uint64_t st, uint32_t fm, uint32_t stride, uint32_t K[NK]) {
+  for (int b = 0; b < NK; ++b) K[b] = 0;
+  for (uint32_t x = 0; x < 32; ++x) {
+    if (!((fm >> x) & 1u)) continue;
+    st = lg_xs(st);
+    uint32_t k = lg_kcol(lg_out32(st), stride);
+    for (int b = 0; b < NK; ++b) K[b] |= ((k >> b) & 1u) << x;
+  }
+}
+
+template <int NK>
 LG_HD inline void lg_draw_word(uint64_t st, uint64_t fm, uint32_t stride, uint32_t lo[NK],
                                uint32_t hi[NK]) {
   if (fm == ~0ull) {
@@ -203,9 +323,9 @@ LG_HD inline void lg_draw_word(uint64_t st, uint64_t fm, uint32_t stride, uint32
 
 // Inflow re-seed / equilibrium sampling for the cells `apply` of one word
 // (src/lattice.rs:102-115, 197-208, 270-285): each applied cell takes 4 draws
-// (3 without rest) in order, and its seven bits replace res[] at that cell.
-LG_HD inline void lg_inlet_word(const LgGeom& G, uint64_t st, uint64_t apply, uint64_t res[7]) {
-  uint64_t bits[7] = {0, 0, 0, 0, 0, 0, 0};
+// (3 without rest) in order; bits[d] receives the seven sampled planes.
+LG_HD inline void lg_inlet_bits(const LgGeom& G, uint64_t st, uint64_t apply, uint64_t bits[7]) {
+  for (int d = 0; d < 7; ++d) bits[d] = 0;
   for (uint32_t x = 0; x < 64; ++x) {
     if (!((apply >> x) & 1ull)) continue;
     uint64_t s = 0;
@@ -222,6 +342,10 @@ LG_HD inline void lg_inlet_word(const LgGeom& G, uint64_t st, uint64_t apply, ui
     }
     for (int d = 0; d < 7; ++d) bits[d] |= ((s >> d) & 1ull) << x;
   }
+}
+LG_HD inline void lg_inlet_word(const LgGeom& G, uint64_t st, uint64_t apply, uint64_t res[7]) {
+  uint64_t bits[7];
+  lg_inlet_bits(G, st, apply, bits);
   for (int d = 0; d < 7; ++d) res[d] = (res[d] & ~apply) | bits[d];
 }
 
@@ -244,20 +368,23 @@ LG_HD inline uint64_t lg_shr_in(uint64_t w0, uint64_t wr, uint32_t csh) {
   return (w0 >> 1) | ((wr & 1ull) << csh);
 }
 
-// One word of the fused step: gather, exact-stream collision, bounce-back on
+// One work unit of the fused step: a whole word (NH = 1) or one 32-bit half
+// of it (NH = 2, half h): gather, exact-stream collision, bounce-back on
 // solid sites, inflow re-seed, tail mask (src/lattice.rs:228-285).
 //   src/dst:  7 planes each (ping-pong)
 //   solid:    the solid plane
-//   cjump:    per-word collision jump polynomial
+//   cjump:    2 per word: jump polynomial to the word's first draw and to
+//             the first draw of its upper half
 //   row_band: collision stream of each row
-//   ijump:    per inlet word (y * iw + j) inlet jump polynomial
+//   ijump:    2 per inlet word (y * iw + j), the same for the inflow stream
 //   tabs:     (nbands + 1) window tables for this step; the last is the
 //             inflow stream of the step's end
-LG_HD inline void lg_step_word(const LgGeom& G, const uint64_t* __restrict src,
+template <int NH>
+LG_HD inline void lg_step_unit(const LgGeom& G, const uint64_t* __restrict src,
                                uint64_t* __restrict dst, const uint64_t* __restrict solid,
                                const uint64_t* __restrict cjump, const uint32_t* __restrict row_band,
                                const uint64_t* __restrict ijump, const uint64_t* __restrict tabs,
-                               uint32_t y, uint32_t j) {
+                               uint32_t y, uint32_t j, uint32_t h) {
   const size_t word = (size_t)y * G.wpr + j;
   const uint32_t odd = y & 1u;
   const size_t T = G.total;
@@ -269,10 +396,70 @@ LG_HD inline void lg_step_word(const LgGeom& G, const uint64_t* __restrict src,
   const uint32_t cr = (j == G.lastword) ? G.lastbit : 63u;  // where the right carry lands
   const size_t r0 = (size_t)y * G.wpr, rm = (size_t)ym * G.wpr, rp = (size_t)yp * G.wpr;
 
-  // issue every load first
-  const uint64_t c = cjump[word];
+  const uint64_t c = cjump[2 * word + h];
   const uint32_t band = row_band[y];
   const uint64_t sm = solid[word];
+  const uint64_t tail = (j == G.lastword) ? G.tailmask : ~0ull;
+  const uint64_t fm = ~sm & tail;
+  const uint64_t st = lg_jump_tab(c, tabs + (size_t)band * LG_WTAB);
+
+  // Draw first and gather afterwards, so the draw loop does not hold the
+  // gathered words in registers (more waves fit a SIMD). A whole-word unit
+  // draws both halves (the upper half's chain continues the lower one's when
+  // every cell of the lower half draws); a half-word unit draws its own.
+  uint32_t Kh[LG_COLLIDE_REST_KBITS], Kl[LG_COLLIDE_REST_KBITS];
+  if (NH == 2) {
+    const uint32_t fh = (uint32_t)(fm >> (32 * h));
+    if (G.rest) {
+      if (fh == ~0u)
+        lg_draw_half<LG_COLLIDE_REST_KBITS>(st, LG_COLLIDE_REST_STRIDE, Kh);
+      else
+        lg_draw_masked32<LG_COLLIDE_REST_KBITS>(st, fh, LG_COLLIDE_REST_STRIDE, Kh);
+    } else {
+      if (fh == ~0u)
+        lg_draw_half<LG_COLLIDE_NOREST_KBITS>(st, LG_COLLIDE_NOREST_STRIDE, Kh);
+      else
+        lg_draw_masked32<LG_COLLIDE_NOREST_KBITS>(st, fh, LG_COLLIDE_NOREST_STRIDE, Kh);
+    }
+  } else if (fm == ~0ull) {
+    // every cell draws: the two halves as two interleaved chains
+    const uint64_t st1 = lg_jump_tab(cjump[2 * word + 1], tabs + (size_t)band * LG_WTAB);
+    if (G.rest)
+      lg_draw_pair<LG_COLLIDE_REST_KBITS>(st, st1, LG_COLLIDE_REST_STRIDE, Kl, Kh);
+    else
+      lg_draw_pair<LG_COLLIDE_NOREST_KBITS>(st, st1, LG_COLLIDE_NOREST_STRIDE, Kl, Kh);
+  } else {
+    const uint32_t f0 = (uint32_t)fm, f1 = (uint32_t)(fm >> 32);
+    uint64_t st1;
+    if (G.rest) {
+      if (f0 == ~0u) {
+        st1 = lg_draw_half<LG_COLLIDE_REST_KBITS>(st, LG_COLLIDE_REST_STRIDE, Kl);
+      } else {
+        lg_draw_masked32<LG_COLLIDE_REST_KBITS>(st, f0, LG_COLLIDE_REST_STRIDE, Kl);
+        st1 = lg_jump_tab(cjump[2 * word + 1], tabs + (size_t)band * LG_WTAB);
+      }
+      if (f1 == ~0u)
+        lg_draw_half<LG_COLLIDE_REST_KBITS>(st1, LG_COLLIDE_REST_STRIDE, Kh);
+      else
+        lg_draw_masked32<LG_COLLIDE_REST_KBITS>(st1, f1, LG_COLLIDE_REST_STRIDE, Kh);
+    } else {
+      if (f0 == ~0u) {
+        st1 = lg_draw_half<LG_COLLIDE_NOREST_KBITS>(st, LG_COLLIDE_NOREST_STRIDE, Kl);
+      } else {
+        lg_draw_masked32<LG_COLLIDE_NOREST_KBITS>(st, f0, LG_COLLIDE_NOREST_STRIDE, Kl);
+        st1 = lg_jump_tab(cjump[2 * word + 1], tabs + (size_t)band * LG_WTAB);
+      }
+      if (f1 == ~0u)
+        lg_draw_half<LG_COLLIDE_NOREST_KBITS>(st1, LG_COLLIDE_NOREST_STRIDE, Kh);
+      else
+        lg_draw_masked32<LG_COLLIDE_NOREST_KBITS>(st1, f1, LG_COLLIDE_NOREST_STRIDE, Kh);
+    }
+  }
+#if defined(__HIP_DEVICE_COMPILE__)
+  // keep the plane loads below the draw loop
+  asm volatile("" : "+v"(Kh[0]) :: "memory");
+#endif
+
   const uint64_t a0 = src[0 * T + r0 + j], a0l = src[0 * T + r0 + jl];
   const uint64_t a1 = src[1 * T + rm + j], a1l = src[1 * T + rm + jl];
   const uint64_t a2 = src[2 * T + rm + j], a2r = src[2 * T + rm + jr];
@@ -280,10 +467,6 @@ LG_HD inline void lg_step_word(const LgGeom& G, const uint64_t* __restrict src,
   const uint64_t a4 = src[4 * T + rp + j], a4r = src[4 * T + rp + jr];
   const uint64_t a5 = src[5 * T + rp + j], a5l = src[5 * T + rp + jl];
   const uint64_t a6 = src[6 * T + word];
-
-  const uint64_t tail = (j == G.lastword) ? G.tailmask : ~0ull;
-  const uint64_t fm = ~sm & tail;
-  const uint64_t st = lg_jump_tab(c, tabs + (size_t)band * LG_WTAB);
 
   uint64_t n[7];
   n[0] = lg_shl_in(a0, a0l, cl);
@@ -295,34 +478,28 @@ LG_HD inline void lg_step_word(const LgGeom& G, const uint64_t* __restrict src,
   n[6] = a6;
 
   uint64_t o[7];
-  if (G.rest) {
-    uint32_t klo[LG_COLLIDE_REST_KBITS], khi[LG_COLLIDE_REST_KBITS];
-    lg_draw_word<LG_COLLIDE_REST_KBITS>(st, fm, LG_COLLIDE_REST_STRIDE, klo, khi);
+  {
     uint32_t nh[7], oh[7];
+    if (NH == 1) {
+      LG_UNROLL
+      for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)n[d];
+      if (G.rest)
+        lg_collide_rest<uint32_t>(nh, Kl, oh);
+      else
+        lg_collide_norest<uint32_t>(nh, Kl, oh);
+      LG_UNROLL
+      for (int d = 0; d < 7; ++d) o[d] = oh[d];
+    }
+    // the upper half (NH = 1) or this lane's half (NH = 2)
+    const uint32_t sh = NH == 1 ? 32u : 32u * h;
     LG_UNROLL
-    for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)n[d];
-    lg_collide_rest<uint32_t>(nh, klo, oh);
+    for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)(n[d] >> sh);
+    if (G.rest)
+      lg_collide_rest<uint32_t>(nh, Kh, oh);
+    else
+      lg_collide_norest<uint32_t>(nh, Kh, oh);
     LG_UNROLL
-    for (int d = 0; d < 7; ++d) o[d] = oh[d];
-    LG_UNROLL
-    for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)(n[d] >> 32);
-    lg_collide_rest<uint32_t>(nh, khi, oh);
-    LG_UNROLL
-    for (int d = 0; d < 7; ++d) o[d] |= (uint64_t)oh[d] << 32;
-  } else {
-    uint32_t klo[LG_COLLIDE_NOREST_KBITS], khi[LG_COLLIDE_NOREST_KBITS];
-    lg_draw_word<LG_COLLIDE_NOREST_KBITS>(st, fm, LG_COLLIDE_NOREST_STRIDE, klo, khi);
-    uint32_t nh[7], oh[7];
-    LG_UNROLL
-    for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)n[d];
-    lg_collide_norest<uint32_t>(nh, klo, oh);
-    LG_UNROLL
-    for (int d = 0; d < 7; ++d) o[d] = oh[d];
-    LG_UNROLL
-    for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)(n[d] >> 32);
-    lg_collide_norest<uint32_t>(nh, khi, oh);
-    LG_UNROLL
-    for (int d = 0; d < 7; ++d) o[d] |= (uint64_t)oh[d] << 32;
+    for (int d = 0; d < 7; ++d) o[d] = (NH == 1 ? o[d] : 0ull) | ((uint64_t)oh[d] << sh);
   }
 
   // A wall sends every particle back the way it came and never collides;
@@ -332,15 +509,24 @@ LG_HD inline void lg_step_word(const LgGeom& G, const uint64_t* __restrict src,
   for (int d = 0; d < 6; ++d) res[d] = (sm & n[(d + 3) % 6]) | (~sm & o[d]);
   res[6] = (sm & n[6]) | (~sm & o[6]);
 
+  const uint64_t mine = NH == 1 ? ~0ull : (0xFFFFFFFFull << (32 * h));
   if (j < G.iw) {
     uint32_t span = G.inlet_cols - j * 64u;  // > 0 since j < iw
     uint64_t imask = span >= 64 ? ~0ull : ((1ull << span) - 1ull);
-    uint64_t apply = imask & fm;
+    uint64_t apply = imask & fm & mine;
     if (apply) {
-      uint64_t ist = lg_jump_tab(ijump[(size_t)y * G.iw + j], tabs + (size_t)G.nbands * LG_WTAB);
+      uint64_t ist = lg_jump_tab(ijump[2 * ((size_t)y * G.iw + j) + (NH == 1 ? 0 : h)],
+                                 tabs + (size_t)G.nbands * LG_WTAB);
       lg_inlet_word(G, ist, apply, res);
     }
   }
-  LG_UNROLL
-  for (int d = 0; d < 7; ++d) dst[(size_t)d * T + word] = res[d] & tail;
+  if (NH == 1) {
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) dst[(size_t)d * T + word] = res[d] & tail;
+  } else {
+    uint32_t* dst32 = reinterpret_cast<uint32_t*>(dst);
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d)
+      dst32[2 * ((size_t)d * T + word) + h] = (uint32_t)((res[d] & tail) >> (32 * h));
+  }
 }

@@ -52,31 +52,40 @@ __global__ void __launch_bounds__(256) lg_tables_kernel(uint64_t seed, uint64_t 
   }
 }
 
-// One step. Threads are mapped with the inflow words first (as the reference
-// does, src/gpu.rs:212-228) so the waves that run the serial re-seed hold
-// nothing else: g < h * iw -> word (g / iw, g % iw); otherwise row-major over
-// the remaining wpr - iw words of each row.
-__global__ void __launch_bounds__(256) lg_step_kernel(LgGeom G, const uint64_t* __restrict src,
-                                                      uint64_t* __restrict dst,
-                                                      const uint64_t* __restrict solid,
-                                                      const uint64_t* __restrict cjump,
-                                                      const uint32_t* __restrict row_band,
-                                                      const uint64_t* __restrict ijump,
-                                                      const uint64_t* __restrict tabs) {
+// One step. Thread g steps unit g % NH of word order[g / NH]: the static
+// order puts the inflow words first (as the reference does,
+// src/gpu.rs:212-228), then the words holding a solid site or a ragged row
+// end, then the rest, so waves of ordinary words never diverge.
+template <int NH>
+__device__ __forceinline__ void lg_step_body(const LgGeom& G, const uint64_t* __restrict src,
+                                             uint64_t* __restrict dst,
+                                             const uint64_t* __restrict solid,
+                                             const uint64_t* __restrict cjump,
+                                             const uint32_t* __restrict row_band,
+                                             const uint64_t* __restrict ijump,
+                                             const uint32_t* __restrict order,
+                                             const uint64_t* __restrict tabs) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= G.total) return;
-  uint32_t y, j;
-  uint32_t ninlet = G.h * G.iw;
-  if (g < ninlet) {
-    y = g / G.iw;
-    j = g - y * G.iw;
-  } else {
-    uint32_t per = G.wpr - G.iw;
-    uint32_t q = g - ninlet;
-    y = q / per;
-    j = G.iw + (q - y * per);
-  }
-  lg_step_word(G, src, dst, solid, cjump, row_band, ijump, tabs, y, j);
+  if (g >= G.total * NH) return;
+  const uint32_t e = order[NH == 1 ? g : g >> 1];
+  lg_step_unit<NH>(G, src, dst, solid, cjump, row_band, ijump, tabs, e >> 16, e & 0xFFFFu,
+                   NH == 1 ? 0u : (g & 1u));
+}
+
+// Whole-word units: 4 waves per SIMD; half-word units: 5.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+lg_step_kernel_w(LgGeom G, const uint64_t* __restrict src, uint64_t* __restrict dst,
+                 const uint64_t* __restrict solid, const uint64_t* __restrict cjump,
+                 const uint32_t* __restrict row_band, const uint64_t* __restrict ijump,
+                 const uint32_t* __restrict order, const uint64_t* __restrict tabs) {
+  lg_step_body<1>(G, src, dst, solid, cjump, row_band, ijump, order, tabs);
+}
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+lg_step_kernel_h(LgGeom G, const uint64_t* __restrict src, uint64_t* __restrict dst,
+                 const uint64_t* __restrict solid, const uint64_t* __restrict cjump,
+                 const uint32_t* __restrict row_band, const uint64_t* __restrict ijump,
+                 const uint32_t* __restrict order, const uint64_t* __restrict tabs) {
+  lg_step_body<2>(G, src, dst, solid, cjump, row_band, ijump, order, tabs);
 }
 
 // Bytes -> planes (GpuLattice::load, src/gpu.rs:572-610), one thread per word.
@@ -130,47 +139,7 @@ __global__ void __launch_bounds__(256) lg_sample_kernel(
     float* __restrict rho, float* __restrict sol, int64_t* __restrict sums) {
   uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= bw * bh) return;
-  uint32_t jb = g / bw, ib = g - jb * bw;
-  const int CX2[6] = {2, 1, -1, -2, -1, 1};
-  const int CY2[6] = {0, 1, 1, 0, -1, -1};
-  uint32_t x0 = ib * bx, x1 = x0 + bx;
-  uint32_t w0 = x0 >> 6, w1 = (x1 - 1) >> 6;
-  int64_t mass = 0, px = 0, py = 0, ns = 0;
-  for (uint32_t y = jb * by; y < (jb + 1) * by; ++y) {
-    size_t base = (size_t)y * G.wpr;
-    for (uint32_t wi = w0; wi <= w1; ++wi) {
-      uint32_t lo = max(x0, wi << 6) - (wi << 6);
-      uint32_t hi = min(x1, (wi + 1) << 6) - (wi << 6);
-      uint64_t m = (hi - lo >= 64) ? ~0ull : (((1ull << (hi - lo)) - 1ull) << lo);
-      uint64_t sm = solid[base + wi] & m;
-      ns += __popcll(sm);
-      uint64_t fm = m & ~sm;
-      for (int d = 0; d < 6; ++d) {
-        int64_t c = __popcll(planes[(size_t)d * G.total + base + wi] & fm);
-        mass += c;
-        px += CX2[d] * c;
-        py += CY2[d] * c;
-      }
-      mass += __popcll(planes[(size_t)6 * G.total + base + wi] & fm);
-    }
-  }
-  sums[(size_t)g * 4 + 0] = mass;
-  sums[(size_t)g * 4 + 1] = px;
-  sums[(size_t)g * 4 + 2] = py;
-  sums[(size_t)g * 4 + 3] = ns;
-  // src/moments.rs:91-97 and src/render.rs:107-115
-  const float SQRT3_2 = 0.8660254f;
-  float n = (float)(bx * by);
-  float vx = 0.0f, vy = 0.0f;
-  if (mass != 0) {
-    float mf = (float)mass;
-    vx = __fdiv_rn(__fmul_rn((float)px, 0.5f), mf);
-    vy = __fdiv_rn(__fmul_rn((float)py, SQRT3_2), mf);
-  }
-  ux[g] = __fadd_rn(ux[g], __fmul_rn(alpha, __fsub_rn(vx, ux[g])));
-  uy[g] = __fadd_rn(uy[g], __fmul_rn(alpha, __fsub_rn(vy, uy[g])));
-  rho[g] = __fadd_rn(rho[g], __fmul_rn(alpha, __fsub_rn(__fdiv_rn((float)mass, n), rho[g])));
-  sol[g] = __fdiv_rn((float)ns, n);
+  lg_sample_block(G, planes, solid, bx, by, bw, alpha, ux, uy, rho, sol, sums, g);
 }
 
 // Obstacles (src/lattice.rs:210-220): every site whose f32 position
@@ -391,15 +360,17 @@ namespace {
 
 int rebuild_tables(lg_t* g, const std::vector<uint64_t>& solid_host) {
   std::vector<uint64_t> cj, ij;
-  std::vector<uint32_t> rb;
+  std::vector<uint32_t> rb, ord;
   lgca::build_tables(g->G.w, g->G.h, g->bands, g->G.inlet_cols, g->G.rest != 0, solid_host, cj, rb,
-                     ij);
+                     ij, ord);
   if (lgca::charpoly_low() == 0) return fail(LG_EDEVICE, "jump polynomial not found");
   HIP_TRY(g->cjump.alloc(cj.size()));
   HIP_TRY(g->row_band.alloc(rb.size()));
   HIP_TRY(g->ijump.alloc(std::max<size_t>(ij.size(), 1)));
   HIP_TRY(hipMemcpyAsync(g->cjump.p, cj.data(), cj.size() * 8, hipMemcpyHostToDevice, g->stream));
   HIP_TRY(hipMemcpyAsync(g->row_band.p, rb.data(), rb.size() * 4, hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(g->order.alloc(ord.size()));
+  HIP_TRY(hipMemcpyAsync(g->order.p, ord.data(), ord.size() * 4, hipMemcpyHostToDevice, g->stream));
   if (!ij.empty())
     HIP_TRY(hipMemcpyAsync(g->ijump.p, ij.data(), ij.size() * 8, hipMemcpyHostToDevice, g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
@@ -458,9 +429,14 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
     if (timed && first) HIP_TRY(hipEventRecord(g->ev0, g->stream));
     first = false;
     for (uint64_t k = 0; k < chunk; ++k) {
-      hipLaunchKernelGGL(lg_step_kernel, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream,
-                         g->G, g->a.p, g->b.p, g->solid.p, g->cjump.p, g->row_band.p, g->ijump.p,
-                         g->basis.p + (size_t)k * per * LG_WTAB);
+      if (g->units == 2)
+        hipLaunchKernelGGL(lg_step_kernel_h, dim3(blocks_for((size_t)g->G.total * 2)), dim3(256), 0,
+                           g->stream, g->G, g->a.p, g->b.p, g->solid.p, g->cjump.p, g->row_band.p,
+                           g->ijump.p, g->order.p, g->basis.p + (size_t)k * per * LG_WTAB);
+      else
+        hipLaunchKernelGGL(lg_step_kernel_w, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream,
+                           g->G, g->a.p, g->b.p, g->solid.p, g->cjump.p, g->row_band.p, g->ijump.p,
+                           g->order.p, g->basis.p + (size_t)k * per * LG_WTAB);
       std::swap(g->a.p, g->b.p);
       g->steps += 1;
     }
@@ -492,8 +468,8 @@ lg_t* lg_create(size_t w, size_t h, int rest, uint64_t seed, size_t bands) {
     fail(LG_EINVAL, "lattice is too small");
     return nullptr;
   }
-  if (w > (1u << 30) || h > (1u << 30) || ((w + 63) / 64) * h >= (1ull << 32)) {
-    fail(LG_EINVAL, "lattice is too large for 32-bit word indices");
+  if (h >= 65536 || (w + 63) / 64 >= 65536 || ((w + 63) / 64) * h * 2 >= (1ull << 32)) {
+    fail(LG_EINVAL, "lattice is too large (h and words per row must stay below 65536)");
     return nullptr;
   }
   int ndev = 0;
@@ -554,6 +530,7 @@ void lg_destroy(lg_t* g) {
   g->ijump.release();
   g->basis.release();
   g->row_band.release();
+  g->order.release();
   g->bytes.release();
   g->fux.release();
   g->fuy.release();
@@ -757,6 +734,14 @@ int lg_dims(lg_t* g, size_t* w, size_t* h) {
 }
 
 uint64_t lg_steps(lg_t* g) { return g ? g->steps : 0; }
+
+int lg_set_work_unit(lg_t* g, uint32_t cells_per_lane) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  if (cells_per_lane != 64 && cells_per_lane != 32)
+    return fail(LG_EINVAL, "cells per lane must be 64 or 32");
+  g->units = cells_per_lane == 64 ? 1u : 2u;
+  return LG_OK;
+}
 
 int lg_set_steps(lg_t* g, uint64_t steps) {
   if (!g) return fail(LG_EINVAL, "null handle");

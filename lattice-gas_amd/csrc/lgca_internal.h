@@ -65,7 +65,8 @@ struct lg_lattice {
   bool loaded = false;
   // planes: a (current), b (next); solid; static tables
   DevBuf<uint64_t> a, b, solid, cjump, ijump, basis;
-  DevBuf<uint32_t> row_band;
+  DevBuf<uint32_t> row_band, order;
+  uint32_t units = 1;  // 1: a lane steps a 64-cell word; 2: a 32-cell half
   DevBuf<uint8_t> bytes;
   uint32_t basis_steps = 0;  // capacity of `basis` in steps
   // field

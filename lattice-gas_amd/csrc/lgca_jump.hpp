@@ -127,19 +127,28 @@ inline uint32_t n_bands(uint32_t h, uint32_t threads) {
 }
 
 // Static per-load tables.
-//   cjump[y * wpr + j]: z^n mod P, n = fluid cells of the row band before
-//                       word (y, j) in row-major order.
+//   cjump[2 * (y * wpr + j) + h]: z^n mod P, n = draws of the row band's
+//                       stream before the first cell of word (y, j) (h = 0)
+//                       or before its cell 32 (h = 1); a draw is one fluid
+//                       cell in row-major order.
 //   row_band[y]:        the band (collision stream) of row y.
-//   ijump[y * iw + j]:  z^n mod P, n = draws of the inflow stream before the
-//                       first inlet cell of word (y, j).
+//   ijump[2 * (y * iw + j) + h]: the same for the inflow stream (4 draws per
+//                       fluid inflow cell, 3 without rest).
+//   order[k]:           the k-th word a step visits, packed (y << 16) | j:
+//                       first the inflow words, then the words holding a
+//                       solid site or the row's ragged end, then the rest in
+//                       row-major order, so that waves of ordinary words never
+//                       diverge into the slower paths.
 // `solid` is the solid plane, wpr words per row; tail bits must be zero.
 inline void build_tables(uint32_t w, uint32_t h, uint32_t bands, uint32_t inlet_cols, bool rest,
                          const std::vector<uint64_t>& solid, std::vector<uint64_t>& cjump,
-                         std::vector<uint32_t>& row_band, std::vector<uint64_t>& ijump) {
+                         std::vector<uint32_t>& row_band, std::vector<uint64_t>& ijump,
+                         std::vector<uint32_t>& order) {
   const uint64_t p = charpoly_low();
   const uint32_t wpr = (w + 63) / 64;
   const uint32_t band = band_rows(h, bands);
-  cjump.assign((size_t)wpr * h, 0);
+  const uint64_t LO = 0xFFFFFFFFull;
+  cjump.assign((size_t)2 * wpr * h, 0);
   row_band.assign(h, 0);
   uint64_t c = 1;
   for (uint32_t y = 0; y < h; ++y) {
@@ -147,26 +156,41 @@ inline void build_tables(uint32_t w, uint32_t h, uint32_t bands, uint32_t inlet_
     row_band[y] = y / band;
     for (uint32_t j = 0; j < wpr; ++j) {
       size_t word = (size_t)y * wpr + j;
-      cjump[word] = c;
       uint32_t ncell = (j + 1) * 64 <= w ? 64 : w - j * 64;
       uint64_t cells = ncell == 64 ? ~0ull : ((1ull << ncell) - 1);
-      uint32_t fluid = (uint32_t)__builtin_popcountll(cells & ~solid[word]);
-      c = poly_mulz(c, fluid, p);
+      uint64_t fl = cells & ~solid[word];
+      cjump[2 * word] = c;
+      c = poly_mulz(c, (uint32_t)__builtin_popcountll(fl & LO), p);
+      cjump[2 * word + 1] = c;
+      c = poly_mulz(c, (uint32_t)__builtin_popcountll(fl & ~LO), p);
     }
   }
   const uint32_t cols = inlet_cols < w ? inlet_cols : w;
   const uint32_t iw = (cols + 63) / 64;
   const uint32_t per = rest ? 4 : 3;
-  ijump.assign((size_t)iw * h, 0);
+  ijump.assign((size_t)2 * iw * h, 0);
   c = 1;
   for (uint32_t y = 0; y < h; ++y)
     for (uint32_t j = 0; j < iw; ++j) {
-      ijump[(size_t)y * iw + j] = c;
       uint32_t span = cols - j * 64;
       uint64_t m = span >= 64 ? ~0ull : ((1ull << span) - 1);
-      uint32_t fluid = (uint32_t)__builtin_popcountll(m & ~solid[(size_t)y * wpr + j]);
-      c = poly_mulz(c, fluid * per, p);
+      uint64_t fl = m & ~solid[(size_t)y * wpr + j];
+      ijump[2 * ((size_t)y * iw + j)] = c;
+      c = poly_mulz(c, (uint32_t)__builtin_popcountll(fl & LO) * per, p);
+      ijump[2 * ((size_t)y * iw + j) + 1] = c;
+      c = poly_mulz(c, (uint32_t)__builtin_popcountll(fl & ~LO) * per, p);
     }
+  const bool ragged = (w % 64) != 0;
+  order.clear();
+  order.reserve((size_t)wpr * h);
+  for (uint32_t y = 0; y < h; ++y)
+    for (uint32_t j = 0; j < iw; ++j) order.push_back((y << 16) | j);
+  for (uint32_t y = 0; y < h; ++y)
+    for (uint32_t j = iw; j < wpr; ++j)
+      if (solid[(size_t)y * wpr + j] || (ragged && j == wpr - 1)) order.push_back((y << 16) | j);
+  for (uint32_t y = 0; y < h; ++y)
+    for (uint32_t j = iw; j < wpr; ++j)
+      if (!(solid[(size_t)y * wpr + j] || (ragged && j == wpr - 1))) order.push_back((y << 16) | j);
 }
 
 }  // namespace lgca

@@ -26,11 +26,13 @@ class LgcaError(RuntimeError):
     pass
 
 
-def load_library(path: str = LIB_PATH):
-    """Load liblgca_hip.so and declare its C signatures (no device calls)."""
+def load_library(path: str | None = None):
+    """Load liblgca_hip.so (or the library named by $LGCA_LIB) and declare its
+    C signatures (no device calls)."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("LGCA_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise LgcaError(f"{path} is missing: build it with lattice-gas_amd/build.py")
     L = C.CDLL(path)
@@ -56,6 +58,9 @@ def load_library(path: str = LIB_PATH):
         "lg_dims": (i32, [vp, P(sz), P(sz)]),
         "lg_steps": (u64, [vp]),
         "lg_set_steps": (i32, [vp, u64]),
+        "lg_set_work_unit": (i32, [vp, C.c_uint32]),
+        "lg_batch_set_units": (i32, [vp, C.c_uint32]),
+        "lg_batch_get_units": (i32, [vp, P(C.c_uint32)]),
         "lg_advance_timed": (i32, [vp, u64, P(f32)]),
         "lg_advance_sampling_timed": (i32, [vp, u64, u64, f32, P(f32)]),
         "lg_debug_collide_table": (i32, [i32, P(C.c_uint8), sz]),
@@ -70,6 +75,9 @@ def load_library(path: str = LIB_PATH):
         "lg_batch_get_shape": (i32, [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "lg_batch_autotune": (i32, [vp, u64, P(u64)]),
         "lg_batch_sample": (i32, [vp, f32]),
+        "lg_batch_advance_limited": (i32, [vp, u64, P(u64)]),
+        "lg_batch_snapshot": (i32, [vp, f32, P(f32), sz]),
+        "lg_batch_snapshot_size": (sz, [vp]),
         "lg_sample_field_async": (i32, [vp, f32]),
         "lg_measure_viscosity": (i32, [f32, i32, sz, u64, sz, P(f32)]),
         "lg_measure_advection": (i32, [f32, i32, sz, u64, sz, P(f32)]),
@@ -189,6 +197,10 @@ class GpuLattice:
     def advance_sampling(self, n: int, sample_every: int, alpha: float):
         _check(load_library().lg_advance_sampling(self._p, int(n), int(sample_every), float(alpha)))
 
+    def set_work_unit(self, cells_per_lane: int):
+        """64 (a lane steps a u64 word) or 32 (half a word, twice the lanes)."""
+        _check(load_library().lg_set_work_unit(self._p, int(cells_per_lane)))
+
     def advance_timed(self, n: int, sample_every: int = 0, alpha: float = 0.0) -> float:
         """Advance and return the device milliseconds between HIP events
         recorded on the engine's stream around the step launches."""
@@ -260,6 +272,39 @@ class Batch:
 
     def set_shape(self, tile: int, grid: int, chunk: int):
         _check(load_library().lg_batch_set_shape(self._p, int(tile), int(grid), int(chunk)))
+
+    def advance_limited(self, n: int, limits):
+        """Member i advances min(n, limits[i]) steps."""
+        lim = np.ascontiguousarray(limits, dtype=np.uint64)
+        if lim.size != len(self.members):
+            raise LgcaError("one limit per member")
+        _check(load_library().lg_batch_advance_limited(self._p, int(n),
+                                                       lim.ctypes.data_as(C.POINTER(C.c_uint64))))
+
+    def snapshot(self, alpha: float):
+        """Sample every member's attached field in one launch; returns, per
+        member, (ux, uy, rho, solid) or None for a member without a field."""
+        L = load_library()
+        n = int(L.lg_batch_snapshot_size(self._p))
+        buf = np.empty(n, dtype=np.float32)
+        _check(L.lg_batch_snapshot(self._p, float(alpha), buf.ctypes.data_as(C.POINTER(C.c_float)), n))
+        out, off = [], 0
+        for m in self.members:
+            if m._field is None:
+                out.append(None)
+                continue
+            nb = m._field[2] * m._field[3]
+            out.append(tuple(buf[off + i * nb: off + (i + 1) * nb] for i in range(4)))
+            off += 4 * nb
+        return out
+
+    def set_units(self, cells_per_lane: int):
+        _check(load_library().lg_batch_set_units(self._p, int(cells_per_lane)))
+
+    def units(self) -> int:
+        u = C.c_uint32()
+        _check(load_library().lg_batch_get_units(self._p, C.byref(u)))
+        return u.value
 
     def shape(self):
         t, g, c = C.c_uint32(), C.c_uint32(), C.c_uint32()

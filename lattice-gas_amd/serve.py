@@ -8,7 +8,8 @@ each served run stays bit-exact with its own CPU replay). The server
   * keeps every admitted lattice resident on the device;
   * advances all resident lattices together, one tile-table step kernel per
     step with the step loop replayed from a hipGraph (lgca_amd.Batch), in
-    quanta that end when the next member is due a snapshot or is done;
+    quanta of `every` steps (a run with fewer steps left stops early within
+    the quantum);
   * streams back each run's coarse-grained velocity field every `every` of its
     own steps (blocks of `block` x `block` cells, the reference's Field with
     alpha = 1) and at its end;
@@ -135,19 +136,26 @@ def _summary(trace, results, t_first, t_last, mode):
 
 def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
                   hbm_budget: int = 200 * GIB, keep_cells: bool = False, shape=None):
-    """Continuous batching over the trace; returns (summary, results)."""
+    """Continuous batching over the trace; returns (summary, results).
+
+    Time is cut into quanta of `every` steps. At each quantum boundary the
+    server admits what has arrived (while the HBM budget allows), advances
+    every resident run by min(every, steps left) in the same launches, takes
+    one batched snapshot of all of them, and retires the runs that are done.
+    A run joins at a boundary, so its snapshots fall every `every` of its own
+    steps and at its end, exactly as in a run of its own."""
     pending = deque(sorted(trace, key=lambda r: r.arrival))
     results = {r.rid: Result(r.rid, r.arrival) for r in trace}
-    active = []             # dicts: req, g, done, until_snap, bytes
+    active = []             # dicts: req, g, done, bytes
     used = 0
     batch = None
-    members_changed = True
     t0 = time.perf_counter()
     clock = lambda: time.perf_counter() - t0
     quanta = 0
     peak = 0
     while pending or active:
         now = clock()
+        changed = False
         while pending and pending[0].arrival <= now:
             r = pending[0]
             fp = footprint(r, block)
@@ -157,33 +165,26 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
             g = build(r, block)
             used += fp
             results[r.rid].start = clock()
-            active.append({"req": r, "g": g, "done": 0, "until": min(every, r.steps), "bytes": fp})
-            members_changed = True
+            active.append({"req": r, "g": g, "done": 0, "bytes": fp})
+            changed = True
         if not active:
             if pending:
                 time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
             continue
         peak = max(peak, len(active))
-        if members_changed:
+        if changed or batch is None:
             if batch is not None:
                 batch.close()
             batch = Batch([a["g"] for a in active])
             if shape:
                 batch.set_shape(*shape)
-            members_changed = False
-        q = min(a["until"] for a in active)
-        batch.advance(q)
+        limits = [a["req"].steps - a["done"] for a in active]
+        batch.advance_limited(every, limits)
         quanta += 1
-        due = []
-        for a in active:
-            a["done"] += q
-            a["until"] -= q
-            if a["until"] == 0:
-                due.append(a)
-        for a in due:
-            a["g"].sample_field(1.0)
-            results[a["req"].rid].snapshots.append(_snapshot(a["g"], a["done"]))
-            a["until"] = min(every, a["req"].steps - a["done"])
+        snaps = batch.snapshot(1.0)
+        for a, lim, snap in zip(active, limits, snaps):
+            a["done"] += min(every, lim)
+            results[a["req"].rid].snapshots.append((a["done"],) + tuple(np.array(x) for x in snap))
         finished = [a for a in active if a["done"] >= a["req"].steps]
         if finished:
             for a in finished:
@@ -193,12 +194,10 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
                 res.done = clock()
                 used -= a["bytes"]
             active = [a for a in active if a["done"] < a["req"].steps]
-            if batch is not None:
-                batch.close()
-                batch = None
+            batch.close()
+            batch = None
             for a in finished:
                 a["g"].close()
-            members_changed = True
     t_last = max(res.done for res in results.values())
     s = _summary(trace, results, 0.0, t_last, "batched")
     s["quanta"] = quanta

@@ -14,7 +14,8 @@ struct Sim {
   uint64_t seed, steps = 0;
   uint32_t bands;
   std::vector<uint64_t> a, b, solid, cjump, ijump, basis;
-  std::vector<uint32_t> row_band;
+  std::vector<uint32_t> row_band, order;
+  uint32_t units = 1;
 };
 }  // namespace
 
@@ -47,6 +48,7 @@ void* hs_new(uint32_t w, uint32_t h, int rest, uint64_t seed, uint32_t bands, ui
 }
 
 void hs_free(void* p) { delete static_cast<Sim*>(p); }
+void hs_set_units(void* p, uint32_t units) { static_cast<Sim*>(p)->units = units; }
 
 void hs_load(void* p, const uint8_t* cells) {
   auto* S = static_cast<Sim*>(p);
@@ -63,7 +65,7 @@ void hs_load(void* p, const uint8_t* cells) {
       if (c & 0x80u) S->solid[word] |= bit;
     }
   lgca::build_tables(G.w, G.h, S->bands, G.inlet_cols, G.rest != 0, S->solid, S->cjump,
-                     S->row_band, S->ijump);
+                     S->row_band, S->ijump, S->order);
 }
 
 void hs_store(void* p, uint8_t* cells) {
@@ -90,10 +92,16 @@ void hs_advance(void* p, uint64_t n) {
                                              : lg_inlet_seed(S->seed, t + 1));
       lg_build_tab(st, S->basis.data() + (size_t)s * LG_WTAB);
     }
-    for (uint32_t y = 0; y < G.h; ++y)
-      for (uint32_t j = 0; j < G.wpr; ++j)
-        lg_step_word(G, S->a.data(), S->b.data(), S->solid.data(), S->cjump.data(),
-                     S->row_band.data(), S->ijump.data(), S->basis.data(), y, j);
+    // visit the words in the device's order, each as S->units units
+    for (uint32_t e : S->order)
+      for (uint32_t hh = 0; hh < S->units; ++hh) {
+        if (S->units == 1)
+          lg_step_unit<1>(G, S->a.data(), S->b.data(), S->solid.data(), S->cjump.data(),
+                          S->row_band.data(), S->ijump.data(), S->basis.data(), e >> 16, e & 0xFFFFu, 0);
+        else
+          lg_step_unit<2>(G, S->a.data(), S->b.data(), S->solid.data(), S->cjump.data(),
+                          S->row_band.data(), S->ijump.data(), S->basis.data(), e >> 16, e & 0xFFFFu, hh);
+      }
     std::swap(S->a, S->b);
     S->steps += 1;
   }

@@ -32,14 +32,16 @@ def build_all():
     return [member(*s) for s in SPECS]
 
 
+@pytest.mark.parametrize("units", [64, 32])
 @pytest.mark.parametrize("shape", [None, (128, 0, 8), (256, 1024, 2), (128, 4096, 16)])
-def test_batch_equals_solo_runs(shape):
-    print(f"batch shape={shape}", flush=True)
+def test_batch_equals_solo_runs(shape, units):
+    print(f"batch shape={shape} cells/lane={units}", flush=True)
     solo = build_all()
     for g in solo:
         g.advance(45)
     batched = build_all()
     b = Batch(batched)
+    b.set_units(units)
     if shape:
         b.set_shape(*shape)
     b.advance(20)          # graph replays + remainder
@@ -80,3 +82,25 @@ def test_autotune_keeps_results_exact():
     for r, g in zip(ref, bat):
         assert g.steps == r.steps
         assert np.array_equal(g.store(), r.store())
+
+
+def test_limited_advance_and_batched_snapshot():
+    """Members with fewer steps left stop early inside the same launches; the
+    batched snapshot equals each member's own sample + read_field."""
+    print("limited advance + snapshot", flush=True)
+    ref = build_all()
+    bat = build_all()
+    for g in ref + bat:
+        g.attach_field(16, 16)
+    b = Batch(bat)
+    limits = [40, 7, 40, 33, 1, 26]
+    for rep in range(3):                  # eager first call, then graph replays
+        b.advance_limited(40, limits)
+        snaps = b.snapshot(1.0)
+        for r, g, lim, snap in zip(ref, bat, limits, snaps):
+            r.advance(min(40, lim))
+            r.sample_field(1.0)
+            assert g.steps == r.steps
+            assert np.array_equal(g.store(), r.store())
+            for x, y in zip(snap, r.read_field()):
+                assert np.array_equal(x.view(np.uint32), y.view(np.uint32))

@@ -18,6 +18,7 @@ def hs():
     L.hs_new.restype = C.c_void_p
     L.hs_new.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, P(C.c_uint32)]
     L.hs_free.argtypes = [C.c_void_p]
+    L.hs_set_units.argtypes = [C.c_void_p, C.c_uint32]
     L.hs_load.argtypes = [C.c_void_p, P(C.c_uint8)]
     L.hs_store.argtypes = [C.c_void_p, P(C.c_uint8)]
     L.hs_advance.argtypes = [C.c_void_p, C.c_uint64]
@@ -37,6 +38,9 @@ class HostSim:
         self.L, self.w, self.h = L, w, h
         t = np.ascontiguousarray(thresh, dtype=np.uint32)
         self.p = L.hs_new(w, h, int(rest), seed, bands, inlet, t.ctypes.data_as(C.POINTER(C.c_uint32)))
+
+    def set_units(self, units):
+        self.L.hs_set_units(self.p, units)
 
     def load(self, cells):
         a = np.ascontiguousarray(cells, dtype=np.uint8)
@@ -91,7 +95,7 @@ def test_jump_equals_sequential_steps(oracle, hs):
 
 
 def compare(oracle, hs, w, h, rest, bands, inlet, seed, steps, disc=None, box=None,
-            density=0.22, inflow=(0.3, 0.0)):
+            density=0.22, inflow=(0.3, 0.0), units=1):
     lat = oracle.Lattice(w, h, density, inflow, inlet, rest, bands, seed)
     if disc:
         lat.add_solid_disc(*disc)
@@ -99,6 +103,7 @@ def compare(oracle, hs, w, h, rest, bands, inlet, seed, steps, disc=None, box=No
         lat.add_solid_box(*box)
     lat.init_equilibrium()
     sim = HostSim(hs, w, h, rest, seed, bands, inlet, oracle.thresholds(density, *inflow, rest))
+    sim.set_units(units)
     sim.load(lat.cells)
     for s in range(steps):
         lat.advance(1)
@@ -110,17 +115,20 @@ def compare(oracle, hs, w, h, rest, bands, inlet, seed, steps, disc=None, box=No
                         f"{[(int(b) % w, int(b) // w) for b in bad[:4]]}")
 
 
+@pytest.mark.parametrize("units", [1, 2])
 @pytest.mark.parametrize("idx", range(5))
-def test_golden_cases_every_step(oracle, hs, idx):
+def test_golden_cases_every_step(oracle, hs, idx, units):
     name, w, h, rest, inlet, plate, threads, steps = oracle.GOLDEN_CASES[idx]
     box = oracle.golden_plate(w, h) if plate else None
-    compare(oracle, hs, w, h, rest, threads, inlet, oracle.GOLDEN_SEED, 60, box=box)
+    compare(oracle, hs, w, h, rest, threads, inlet, oracle.GOLDEN_SEED, 60, box=box, units=units)
 
 
 @pytest.mark.parametrize("w,h,rest,bands,inlet", [
     (65, 6, True, 1, 8), (100, 18, True, 3, 8), (129, 64, False, 4, 70), (200, 40, True, 7, 130),
     (64, 32, True, 5, 0), (320, 10, True, 10, 8), (80, 48, True, 2, 80), (127, 8, False, 3, 127),
 ])
-def test_ragged_shapes_every_step(oracle, hs, w, h, rest, bands, inlet):
+@pytest.mark.parametrize("units", [1, 2])
+def test_ragged_shapes_every_step(oracle, hs, w, h, rest, bands, inlet, units):
     yc = float(np.float32(np.float32(h) * oracle.SQRT3_2 / np.float32(2)))
-    compare(oracle, hs, w, h, rest, bands, inlet, 0xC0FFEE + w, 25, disc=(w / 3.0, yc, 3.5))
+    compare(oracle, hs, w, h, rest, bands, inlet, 0xC0FFEE + w, 25, disc=(w / 3.0, yc, 3.5),
+            units=units)
