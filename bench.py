@@ -86,6 +86,25 @@ def cpu_baseline(cfg, total_steps, gpu_cells, budget_s):
     }
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload`, as recorded in a
+    profile summary under profiles/ by a line
+        traffic <kernel> <workload> bytes_per_launch=<N>
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
+    same counter's reading for a plain copy of known size); None if absent."""
+    import glob
+    import re
+
+    pat = re.compile(r"^traffic (\S+) (\S+) bytes_per_launch=(\d+)\s*$")
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                m = pat.match(line.strip())
+                if m and m.group(1) == kernel and m.group(2) == workload:
+                    return int(m.group(3))
+    return None
+
+
 def ensemble_leg(n, steps, bands, seed=4):
     """Batched ensemble: n independent lattices, widths and heights uniform on
     256..2048 in steps of 64, per-lattice seed, plate 10-25% of H tall at x in
@@ -121,6 +140,7 @@ def ensemble_leg(n, steps, bands, seed=4):
     b.advance(steps - used)
     t_batch = time.perf_counter() - t0
     shape = b.shape()
+    units = b.units()
     b.close()
     # the plain loop over the same lattices (rebuilt), one at a time
     t_plain = 0.0
@@ -138,7 +158,8 @@ def ensemble_leg(n, steps, bands, seed=4):
     return {"lattices": n, "steps": steps, "cells": cells, "bands": bands,
             "batched_gcups": round(gc_b, 3), "plain_loop_gcups": round(gc_p, 3),
             "speedup": round(gc_b / gc_p, 3), "batched_s": round(t_batch, 4),
-            "autotuned_shape": {"tile": shape[0], "grid": shape[1], "chunk": shape[2]},
+            "autotuned_shape": {"tile": shape[0], "grid": shape[1], "chunk": shape[2],
+                                "cells_per_lane": units},
             "plain_loop_sample": f"first {probe} lattices, {steps} steps each"}
 
 
@@ -162,7 +183,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--bands", type=int, default=0, help="CPU bands (default: host threads, <= 16)")
-    ap.add_argument("--cells-per-lane", type=int, default=64, choices=(64, 32),
+    ap.add_argument("--cells-per-lane", type=int, default=32, choices=(64, 32),
                     help="step-kernel work unit: a whole u64 word or half of it")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
@@ -219,6 +240,7 @@ def main():
     step_ms = dev_ms / args.steps
     traffic_alg = cells * BYTES_PER_CELL_UPDATE
     achieved = traffic_alg / (step_ms * 1e-3) / 1e9
+    kernel = "lg_step_kernel_h" if args.cells_per_lane == 32 else "lg_step_kernel_w"
 
     out = {
         "metric": METRIC,
@@ -238,8 +260,9 @@ def main():
                    "sample_every": every, "block": cfg["block"], "seed": cfg["seed"],
                    "cells_per_lane": args.cells_per_lane},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "lg_step_kernel", "algorithmic_bytes_per_launch": int(traffic_alg),
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(kernel, cfg["name"]),
+                     "kernel": kernel, "algorithmic_bytes_per_launch": int(traffic_alg),
                      "mean_launch_ms": round(step_ms, 6)},
     }
     if rank == 0 and not args.no_cpu:

@@ -108,12 +108,14 @@ uint64_t lg_steps(lg_t* g);
 int lg_set_steps(lg_t* g, uint64_t steps);
 
 /* Launch shape of the step: each lane steps 64 cells (a whole u64 word) or
- * 32 (half a word, twice the lanes: more parallelism for lattices too small
- * to fill the GPU). The results are identical either way. */
+ * 32 (half a word, twice the lanes; the default: its shorter serial random
+ * chain per lane measured faster). The results are identical either way. */
 int lg_set_work_unit(lg_t* g, uint32_t cells_per_lane);
 
-/* New: advance n steps and report the device time of the step work, measured
- * with HIP events on the handle's stream around the step launches. */
+/* New: advance n steps and report the device time of the step kernels in ms:
+ * the sum over HIP event pairs recorded on the handle's stream around each
+ * run of back-to-back step launches (the random-table and sampling kernels
+ * between those runs are not counted). */
 int lg_advance_timed(lg_t* g, uint64_t n, float* ms);
 
 /* New: lg_advance_sampling with the same device-time report. */

@@ -233,52 +233,6 @@ LG_HD inline uint64_t lg_draw_half(uint64_t st, uint32_t stride, uint32_t P[NK])
   return st;
 }
 
-// Both halves of a word at once, as two independent chains (the upper one
-// started by its own jump): the same digits as two lg_draw_half calls, with
-// twice the independent work per lane to hide each chain's latency.
-template <int NK>
-LG_HD inline void lg_draw_pair(uint64_t s0, uint64_t s1, uint32_t stride, uint32_t P0[NK],
-                               uint32_t P1[NK]) {
-  uint32_t R0[8], R1[8];
-  LG_UNROLL
-  for (int m = 0; m < 8; ++m) R0[m] = R1[m] = 0;
-  LG_UNROLL
-  for (uint32_t x = 0; x < 32; ++x) {
-    s0 = lg_xs(s0);
-    s1 = lg_xs(s1);
-    uint32_t k0 = lg_kcol(lg_out32(s0), stride);
-    uint32_t k1 = lg_kcol(lg_out32(s1), stride);
-    if (x + NK <= 32) {
-#if defined(__HIP_DEVICE_COMPILE__)
-      uint32_t t0, t1;
-      asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(t0) : "v"(k0), "i"(x), "v"(R0[x & 7]));
-      asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(t1) : "v"(k1), "i"(x), "v"(R1[x & 7]));
-      R0[x & 7] = t0;
-      R1[x & 7] = t1;
-#else
-      R0[x & 7] |= k0 << x;
-      R1[x & 7] |= k1 << x;
-#endif
-    } else {
-      R0[x & 7] |= (k0 << x) | (k0 >> (32 - x));
-      R1[x & 7] |= (k1 << x) | (k1 >> (32 - x));
-    }
-  }
-  LG_UNROLL
-  for (int b = 0; b < NK; ++b) {
-    uint32_t a0 = 0, a1 = 0;
-    LG_UNROLL
-    for (int m = 0; m < 8; ++m) {
-      uint32_t r0 = b ? ((R0[m] >> b) | (R0[m] << (32 - b))) : R0[m];
-      uint32_t r1 = b ? ((R1[m] >> b) | (R1[m] << (32 - b))) : R1[m];
-      a0 |= r0 & (0x01010101u << m);
-      a1 |= r1 & (0x01010101u << m);
-    }
-    P0[b] = a0;
-    P1[b] = a1;
-  }
-}
-
 // General path: only the cells in `fm` draw, in order (solid sites and the
 // cells past the row's end take no draw, src/lattice.rs:358-362).
 template <int NK>
@@ -405,8 +359,9 @@ LG_HD inline void lg_step_unit(const LgGeom& G, const uint64_t* __restrict src,
 
   // Draw first and gather afterwards, so the draw loop does not hold the
   // gathered words in registers (more waves fit a SIMD). A whole-word unit
-  // draws both halves (the upper half's chain continues the lower one's when
-  // every cell of the lower half draws); a half-word unit draws its own.
+  // draws both halves, the upper half's chain continuing the lower one's when
+  // every cell of the lower half draws (a second, interleaved chain started
+  // by its own jump measured slower); a half-word unit draws its own.
   uint32_t Kh[LG_COLLIDE_REST_KBITS], Kl[LG_COLLIDE_REST_KBITS];
   if (NH == 2) {
     const uint32_t fh = (uint32_t)(fm >> (32 * h));
@@ -421,13 +376,6 @@ LG_HD inline void lg_step_unit(const LgGeom& G, const uint64_t* __restrict src,
       else
         lg_draw_masked32<LG_COLLIDE_NOREST_KBITS>(st, fh, LG_COLLIDE_NOREST_STRIDE, Kh);
     }
-  } else if (fm == ~0ull) {
-    // every cell draws: the two halves as two interleaved chains
-    const uint64_t st1 = lg_jump_tab(cjump[2 * word + 1], tabs + (size_t)band * LG_WTAB);
-    if (G.rest)
-      lg_draw_pair<LG_COLLIDE_REST_KBITS>(st, st1, LG_COLLIDE_REST_STRIDE, Kl, Kh);
-    else
-      lg_draw_pair<LG_COLLIDE_NOREST_KBITS>(st, st1, LG_COLLIDE_NOREST_STRIDE, Kl, Kh);
   } else {
     const uint32_t f0 = (uint32_t)fm, f1 = (uint32_t)(fm >> 32);
     uint64_t st1;

@@ -72,7 +72,8 @@ __device__ __forceinline__ void lg_step_body(const LgGeom& G, const uint64_t* __
                    NH == 1 ? 0u : (g & 1u));
 }
 
-// Whole-word units: 4 waves per SIMD; half-word units: 5.
+// Both unit modes run at 4 waves per SIMD (the register budget that fits);
+// 5 for half-word units measured slower on the 4096 x 2304 plate workload.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 lg_step_kernel_w(LgGeom G, const uint64_t* __restrict src, uint64_t* __restrict dst,
                  const uint64_t* __restrict solid, const uint64_t* __restrict cjump,
@@ -80,7 +81,7 @@ lg_step_kernel_w(LgGeom G, const uint64_t* __restrict src, uint64_t* __restrict 
                  const uint32_t* __restrict order, const uint64_t* __restrict tabs) {
   lg_step_body<1>(G, src, dst, solid, cjump, row_band, ijump, order, tabs);
 }
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 lg_step_kernel_h(LgGeom G, const uint64_t* __restrict src, uint64_t* __restrict dst,
                  const uint64_t* __restrict solid, const uint64_t* __restrict cjump,
                  const uint32_t* __restrict row_band, const uint64_t* __restrict ijump,
@@ -407,7 +408,8 @@ int add_solid(lg_t* g, int kind, float a, float b, float c, float d) {
 }
 
 int sample_now(lg_t* g, float alpha) {
-  if (!g->has_field) return LG_OK;
+  // a lattice smaller than one block has an empty field (src/render.rs:73-75)
+  if (!g->has_field || g->bw == 0 || g->bh == 0) return LG_OK;
   hipLaunchKernelGGL(lg_sample_kernel, dim3(blocks_for((size_t)g->bw * g->bh)), dim3(256), 0,
                      g->stream, g->G, g->a.p, g->solid.p, g->bx, g->by, g->bw, g->bh, alpha,
                      g->fux.p, g->fuy.p, g->frho.p, g->fsol.p, g->fsums.p);
@@ -419,15 +421,21 @@ int sample_now(lg_t* g, float alpha) {
 int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) {
   uint64_t left = n, since = 0;
   const uint32_t per = g->G.nbands + 1;
-  bool first = true;
+  if (timed) g->tev_used = 0;
   while (left > 0) {
     uint64_t chunk = std::min<uint64_t>(left, g->basis_steps);
     if (every > 0) chunk = std::min<uint64_t>(chunk, every - since);
     hipLaunchKernelGGL(lg_tables_kernel, dim3(blocks_for((size_t)chunk * per * 16)), dim3(256), 0,
                        g->stream, g->seed, g->steps, (uint32_t)chunk, g->G.nbands, g->basis.p);
     HIP_TRY(hipGetLastError());
-    if (timed && first) HIP_TRY(hipEventRecord(g->ev0, g->stream));
-    first = false;
+    if (timed) {
+      while (g->tev.size() < 2 * (g->tev_used + 1)) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        g->tev.push_back(e);
+      }
+      HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used], g->stream));
+    }
     for (uint64_t k = 0; k < chunk; ++k) {
       if (g->units == 2)
         hipLaunchKernelGGL(lg_step_kernel_h, dim3(blocks_for((size_t)g->G.total * 2)), dim3(256), 0,
@@ -441,6 +449,7 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
       g->steps += 1;
     }
     HIP_TRY(hipGetLastError());
+    if (timed) HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used++ + 1], g->stream));
     left -= chunk;
     since += chunk;
     if (every > 0 && (since >= every || left == 0)) {
@@ -449,7 +458,6 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
       if (rc) return rc;
     }
   }
-  if (timed) HIP_TRY(hipEventRecord(g->ev1, g->stream));
   return LG_OK;
 }
 
@@ -487,8 +495,6 @@ lg_t* lg_create(size_t w, size_t h, int rest, uint64_t seed, size_t bands) {
   if (hipGetDevice(&g->device) != hipSuccess) return bail("hipGetDevice failed");
   if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
     return bail("hipStreamCreate failed");
-  if (hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess)
-    return bail("hipEventCreate failed");
   LgGeom& G = g->G;
   G.w = (uint32_t)w;
   G.h = (uint32_t)h;
@@ -538,8 +544,7 @@ void lg_destroy(lg_t* g) {
   g->fsol.release();
   g->fsums.release();
   g->counter.release();
-  if (g->ev0) (void)hipEventDestroy(g->ev0);
-  if (g->ev1) (void)hipEventDestroy(g->ev1);
+  for (hipEvent_t e : g->tev) (void)hipEventDestroy(e);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
 }
@@ -657,8 +662,14 @@ int lg_advance_sampling_timed(lg_t* g, uint64_t n, uint64_t every, float alpha, 
   }
   int rc = enqueue_steps(g, n, every, alpha, true);
   if (rc) return rc;
-  HIP_TRY(hipEventSynchronize(g->ev1));
-  HIP_TRY(hipEventElapsedTime(ms, g->ev0, g->ev1));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  float total = 0.0f;
+  for (size_t i = 0; i < g->tev_used; ++i) {
+    float t = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&t, g->tev[2 * i], g->tev[2 * i + 1]));
+    total += t;
+  }
+  *ms = total;
   return LG_OK;
 }
 

@@ -58,7 +58,10 @@ struct lg_lattice {
 
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Timed advance: one (start, end) event pair around each chunk's step
+  // launches, so table and sampling kernels stay out of the step time.
+  std::vector<hipEvent_t> tev;
+  size_t tev_used = 0;
   LgGeom G{};
   uint64_t seed = 0, steps = 0;
   uint32_t bands = 1;
@@ -66,7 +69,7 @@ struct lg_lattice {
   // planes: a (current), b (next); solid; static tables
   DevBuf<uint64_t> a, b, solid, cjump, ijump, basis;
   DevBuf<uint32_t> row_band, order;
-  uint32_t units = 1;  // 1: a lane steps a 64-cell word; 2: a 32-cell half
+  uint32_t units = 2;  // 1: a lane steps a 64-cell word; 2: a 32-cell half
   DevBuf<uint8_t> bytes;
   uint32_t basis_steps = 0;  // capacity of `basis` in steps
   // field
