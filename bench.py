@@ -12,7 +12,9 @@ before the timed region starts.
 Prints one JSON line. `value` is cell updates per second (GCUPS) over the
 timed steps; `roofline.achieved` is the algorithmic traffic of one step
 (15 bits per cell: 7 planes read, 7 written, the solid plane read) divided by
-the mean duration of one step launch, from HIP events on the engine's stream.
+the mean device time of one step's kernels (the collide launch plus its share
+of the draw launch that made its random columns, or the one fused launch),
+from HIP event pairs on the engine's stream around each run of step launches.
 `cpu_baseline` is the CPU oracle (a restatement of the reference's CPU path,
 one thread per row band, spawned per step as the reference does) replaying
 the same run on this host; its final cells are compared with the GPU's.
@@ -183,8 +185,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--bands", type=int, default=0, help="CPU bands (default: host threads, <= 16)")
-    ap.add_argument("--cells-per-lane", type=int, default=32, choices=(64, 32),
+    ap.add_argument("--cells-per-lane", type=int, default=64, choices=(64, 32),
                     help="step-kernel work unit: a whole u64 word or half of it")
+    ap.add_argument("--passes", type=int, default=2, choices=(1, 2),
+                    help="kernels per step: draw pass + collide pass (2) or one fused kernel (1)")
+    ap.add_argument("--draw-cells-per-lane", type=int, default=64, choices=(64, 32),
+                    help="work unit of the two-pass step's draw work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--ensemble", type=int, default=256, metavar="N",
@@ -227,6 +233,8 @@ def main():
 
     g = W.config_gpu(cfg)
     g.set_work_unit(args.cells_per_lane)
+    g.set_passes(args.passes)
+    g.set_draw_unit(args.draw_cells_per_lane)
     g.attach_field(cfg["block"], cfg["block"])
     if args.warmup:
         g.advance_sampling(args.warmup, every, 1.0)
@@ -240,7 +248,9 @@ def main():
     step_ms = dev_ms / args.steps
     traffic_alg = cells * BYTES_PER_CELL_UPDATE
     achieved = traffic_alg / (step_ms * 1e-3) / 1e9
-    kernel = "lg_step_kernel_h" if args.cells_per_lane == 32 else "lg_step_kernel_w"
+    unit = "h" if args.cells_per_lane == 32 else "w"
+    kernel = (f"lg_step_kernel_{unit}" if args.passes == 1 else
+              f"lg_step2_kernel<{3 - args.cells_per_lane // 32},{3 - args.draw_cells_per_lane // 32}>")
 
     out = {
         "metric": METRIC,
@@ -258,7 +268,8 @@ def main():
         "config": {"workload": cfg["name"], "w": cfg["w"], "h": cfg["h"], "cells": cells,
                    "rest": True, "bands": bands, "inlet_cols": cfg["inlet"],
                    "sample_every": every, "block": cfg["block"], "seed": cfg["seed"],
-                   "cells_per_lane": args.cells_per_lane},
+                   "cells_per_lane": args.cells_per_lane, "passes": args.passes,
+                   "draw_cells_per_lane": args.draw_cells_per_lane if args.passes == 2 else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(kernel, cfg["name"]),

@@ -107,10 +107,22 @@ uint64_t lg_steps(lg_t* g);
  * run exactly (the reference can only restart at step 0). */
 int lg_set_steps(lg_t* g, uint64_t steps);
 
-/* Launch shape of the step: each lane steps 64 cells (a whole u64 word) or
- * 32 (half a word, twice the lanes; the default: its shorter serial random
- * chain per lane measured faster). The results are identical either way. */
+/* Launch shape of the step: each lane steps 64 cells (a whole u64 word, the
+ * default) or 32 (half a word, twice the lanes). The results are identical
+ * either way. */
 int lg_set_work_unit(lg_t* g, uint32_t cells_per_lane);
+
+/* Work per step launch: 2 (the default) splits the step into a draw pass
+ * (the random collision columns and inflow samples, which depend only on the
+ * seed, the step and the obstacle layout) and a gather-collide-store pass;
+ * each launch collides step t while it draws step t + 1, so memory waits
+ * of the one overlap the arithmetic of the other. 1 fuses both passes of
+ * one step into each lane. The results are identical either way. */
+int lg_set_passes(lg_t* g, uint32_t passes);
+
+/* Cells per lane (64, the default, or 32) of the two-pass step's draw work,
+ * as lg_set_work_unit sets them for its collide work. Results are identical. */
+int lg_set_draw_unit(lg_t* g, uint32_t cells_per_lane);
 
 /* New: advance n steps and report the device time of the step kernels in ms:
  * the sum over HIP event pairs recorded on the handle's stream around each

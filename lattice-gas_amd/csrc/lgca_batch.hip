@@ -158,6 +158,10 @@ void drop_graph(lg_batch* b) {
 }
 
 int upload_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
+  for (lg_t* g : b->members) {
+    int rc = ensure_tables(g);
+    if (rc) return rc;
+  }
   std::vector<LgDesc>& h = b->host_desc;
   h.assign(b->members.size(), LgDesc{});
   size_t off = 0;

@@ -96,13 +96,14 @@ LG_HD inline uint64_t lg_jump(uint64_t c, const uint64_t* B) {
 // T^(4g + b) s over the set bits b of v: 16 lookups instead of 64 terms.
 #define LG_WTAB 256  // entries per stream per step
 LG_HD inline uint64_t lg_jump_tab(uint64_t c, const uint64_t* __restrict tab) {
-  uint64_t a = 0, b = 0;
+  uint64_t e[16];
   LG_UNROLL
-  for (int g = 0; g < 16; g += 2) {
-    a ^= tab[g * 16 + (uint32_t)((c >> (4 * g)) & 15u)];
-    b ^= tab[(g + 1) * 16 + (uint32_t)((c >> (4 * g + 4)) & 15u)];
-  }
-  return a ^ b;
+  for (int g = 0; g < 16; ++g) e[g] = tab[g * 16 + (uint32_t)((c >> (4 * g)) & 15u)];
+  // XOR of the 16 entries as a tree of three-input XORs (0x96: a ^ b ^ c)
+  const uint64_t x0 = lg_op3<0x96>(e[0], e[1], e[2]), x1 = lg_op3<0x96>(e[3], e[4], e[5]);
+  const uint64_t x2 = lg_op3<0x96>(e[6], e[7], e[8]), x3 = lg_op3<0x96>(e[9], e[10], e[11]);
+  const uint64_t x4 = lg_op3<0x96>(e[12], e[13], e[14]);
+  return lg_op3<0x96>(x0, x1, x2) ^ lg_op3<0x96>(x3, x4, e[15]);
 }
 // Fill one stream's window tables from its seeded start state.
 LG_HD inline void lg_build_tab(uint64_t s, uint64_t* __restrict tab) {
@@ -220,15 +221,17 @@ LG_HD inline uint64_t lg_draw_half(uint64_t st, uint32_t stride, uint32_t P[NK])
       R[x & 7] |= (k << x) | (k >> (32 - x));
     }
   }
+  // Digit b of cell x sits at bit (x + b) mod 32 of R[x % 8]: select those
+  // bits of every accumulator, then one rotate right by b lines them up.
   LG_UNROLL
   for (int b = 0; b < NK; ++b) {
     uint32_t acc = 0;
     LG_UNROLL
     for (int m = 0; m < 8; ++m) {
-      uint32_t r = b ? ((R[m] >> b) | (R[m] << (32 - b))) : R[m];
-      acc |= r & (0x01010101u << m);
+      const uint32_t sel = 0x01010101u << m;
+      acc |= R[m] & (b ? ((sel << b) | (sel >> (32 - b))) : sel);
     }
-    P[b] = acc;
+    P[b] = b ? ((acc >> b) | (acc << (32 - b))) : acc;
   }
   return st;
 }
@@ -466,6 +469,202 @@ LG_HD inline void lg_step_unit(const LgGeom& G, const uint64_t* __restrict src,
       uint64_t ist = lg_jump_tab(ijump[2 * ((size_t)y * G.iw + j) + (NH == 1 ? 0 : h)],
                                  tabs + (size_t)G.nbands * LG_WTAB);
       lg_inlet_word(G, ist, apply, res);
+    }
+  }
+  if (NH == 1) {
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) dst[(size_t)d * T + word] = res[d] & tail;
+  } else {
+    uint32_t* dst32 = reinterpret_cast<uint32_t*>(dst);
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d)
+      dst32[2 * ((size_t)d * T + word) + h] = (uint32_t)((res[d] & tail) >> (32 * h));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Two-pass step. The random draws of a step depend only on the seed, the step
+// number and which cells are fluid, never on the lattice state, so they can
+// be made ahead of the state update:
+//   draw pass:    for each 32-cell half, the collision column digits of its
+//                 cells (exact band stream), and for an inflow half the seven
+//                 re-seeded planes of its inflow cells (exact inflow stream);
+//   collide pass: gather, collide with those digits, bounce back, overwrite
+//                 the inflow cells, store.
+// The draw pass needs few registers, so many waves hide its serial chains;
+// the collide pass carries no random state. The draw pass visits the halves
+// in order-table sequence (its masked draws diverge), but files them by
+// position: half h of word w has its digits at index 2 w + h of each digit
+// plane (kstride halves apart) and, in an inflow word (y, j), its inflow
+// planes at index 2 (y iw + j) + h of each inflow plane (istride apart). The
+// collide pass, with no divergent paths, runs in plain word order and finds
+// them without a table lookup.
+// ---------------------------------------------------------------------------
+#define LG_KDIGITS 5  // digit planes reserved per step (3 used without rest)
+
+LG_HD inline uint64_t lg_inlet_apply(const LgGeom& G, uint32_t j, uint64_t fm) {
+  uint32_t span = G.inlet_cols - j * 64u;  // > 0 for j < iw
+  uint64_t imask = span >= 64 ? ~0ull : ((1ull << span) - 1ull);
+  return imask & fm;
+}
+
+// The collision digits of one 32-cell half (fluid mask fh) from state st.
+template <int NK>
+LG_HD inline uint64_t lg_draw_digits(uint64_t st, uint32_t fh, uint32_t stride, uint32_t K[NK]) {
+  if (fh == ~0u) return lg_draw_half<NK>(st, stride, K);
+  lg_draw_masked32<NK>(st, fh, stride, K);
+  return 0;  // the chain's end is not needed past a masked half
+}
+
+// Draw pass for word (y, j): both halves (DNH = 1; the upper half's chain
+// continues the lower one's when every cell of the lower half draws) or
+// half h alone (DNH = 2). tabs are this step's window tables,
+// kplanes/iplanes the step's digit and inflow planes.
+template <int NK, int DNH>
+LG_HD inline void lg_draw_unit_k(const LgGeom& G, uint64_t fm, const uint64_t* __restrict cj,
+                                 const uint64_t* __restrict tab, uint32_t h, uint32_t stride,
+                                 uint32_t* __restrict kout, size_t kstride) {
+  uint32_t K[NK];
+  const uint32_t h0 = DNH == 1 ? 0u : h;
+  const uint32_t f0 = (uint32_t)(fm >> (32 * h0));
+  uint64_t st = lg_draw_digits<NK>(lg_jump_tab(cj[h0], tab), f0, stride, K);
+  LG_UNROLL
+  for (int b = 0; b < NK; ++b) kout[b * kstride + h0] = K[b];
+  if (DNH == 1) {
+    if (f0 != ~0u) st = lg_jump_tab(cj[1], tab);
+    lg_draw_digits<NK>(st, (uint32_t)(fm >> 32), stride, K);
+    LG_UNROLL
+    for (int b = 0; b < NK; ++b) kout[b * kstride + 1] = K[b];
+  }
+}
+template <int DNH>
+LG_HD inline void lg_draw_unit(const LgGeom& G, const uint64_t* __restrict solid,
+                               const uint64_t* __restrict cjump, const uint32_t* __restrict row_band,
+                               const uint64_t* __restrict ijump, const uint64_t* __restrict tabs,
+                               uint32_t y, uint32_t j, uint32_t h, uint32_t* __restrict kplanes,
+                               size_t kstride, uint32_t* __restrict iplanes, size_t istride) {
+  const size_t word = (size_t)y * G.wpr + j;
+  uint32_t* __restrict kout = kplanes + 2 * word;
+  const uint64_t tail = (j == G.lastword) ? G.tailmask : ~0ull;
+  const uint64_t fm = ~solid[word] & tail;
+  const uint64_t* __restrict tab = tabs + (size_t)row_band[y] * LG_WTAB;
+  if (G.rest)
+    lg_draw_unit_k<LG_COLLIDE_REST_KBITS, DNH>(G, fm, cjump + 2 * word, tab, h,
+                                               LG_COLLIDE_REST_STRIDE, kout, kstride);
+  else
+    lg_draw_unit_k<LG_COLLIDE_NOREST_KBITS, DNH>(G, fm, cjump + 2 * word, tab, h,
+                                                 LG_COLLIDE_NOREST_STRIDE, kout, kstride);
+  if (j < G.iw) {
+    const uint64_t mine = DNH == 1 ? ~0ull : (0xFFFFFFFFull << (32 * h));
+    const uint64_t apply = lg_inlet_apply(G, j, fm) & mine;
+    if (apply) {
+      const size_t iword = (size_t)y * G.iw + j;
+      uint64_t ist = lg_jump_tab(ijump[2 * iword + (DNH == 1 ? 0 : h)],
+                                 tabs + (size_t)G.nbands * LG_WTAB);
+      uint64_t bits[7];
+      lg_inlet_bits(G, ist, apply, bits);
+      uint32_t* __restrict iout = iplanes + 2 * iword;
+      LG_UNROLL
+      for (int d = 0; d < 7; ++d) {
+        if (DNH == 1) {
+          iout[d * istride] = (uint32_t)bits[d];
+          iout[d * istride + 1] = (uint32_t)(bits[d] >> 32);
+        } else {
+          iout[d * istride + h] = (uint32_t)(bits[d] >> (32 * h));
+        }
+      }
+    }
+  }
+}
+
+// Collide pass for one unit of word (y, j): the whole word (NH = 1) or its
+// half h (NH = 2); kplanes/iplanes are the step's digit and inflow planes.
+template <int NH>
+LG_HD inline void lg_collide_unit(const LgGeom& G, const uint64_t* __restrict src,
+                                  uint64_t* __restrict dst, const uint64_t* __restrict solid,
+                                  const uint32_t* __restrict kplanes, size_t kstride,
+                                  const uint32_t* __restrict iplanes, size_t istride, uint32_t y,
+                                  uint32_t j, uint32_t h) {
+  const size_t word = (size_t)y * G.wpr + j;
+  const uint32_t* __restrict kin = kplanes + 2 * word + (NH == 1 ? 0 : h);
+  const uint32_t odd = y & 1u;
+  const size_t T = G.total;
+  const uint32_t ym = (y == 0) ? G.h - 1 : y - 1;
+  const uint32_t yp = (y + 1 == G.h) ? 0 : y + 1;
+  const uint32_t jl = (j == 0) ? G.lastword : j - 1;
+  const uint32_t jr = (j == G.lastword) ? 0 : j + 1;
+  const uint32_t cl = (j == 0) ? G.lastbit : 63u;
+  const uint32_t cr = (j == G.lastword) ? G.lastbit : 63u;
+  const size_t r0 = (size_t)y * G.wpr, rm = (size_t)ym * G.wpr, rp = (size_t)yp * G.wpr;
+  const uint64_t sm = solid[word];
+  const uint64_t tail = (j == G.lastword) ? G.tailmask : ~0ull;
+
+  uint32_t Kl[LG_KDIGITS], Kh[LG_KDIGITS];
+  const int nk = G.rest ? LG_COLLIDE_REST_KBITS : LG_COLLIDE_NOREST_KBITS;
+  LG_UNROLL
+  for (int b = 0; b < LG_KDIGITS; ++b) {
+    Kl[b] = b < nk ? kin[b * kstride] : 0u;
+    Kh[b] = (NH == 1 && b < nk) ? kin[b * kstride + 1] : Kl[b];
+  }
+
+  const uint64_t a0 = src[0 * T + r0 + j], a0l = src[0 * T + r0 + jl];
+  const uint64_t a1 = src[1 * T + rm + j], a1l = src[1 * T + rm + jl];
+  const uint64_t a2 = src[2 * T + rm + j], a2r = src[2 * T + rm + jr];
+  const uint64_t a3 = src[3 * T + r0 + j], a3r = src[3 * T + r0 + jr];
+  const uint64_t a4 = src[4 * T + rp + j], a4r = src[4 * T + rp + jr];
+  const uint64_t a5 = src[5 * T + rp + j], a5l = src[5 * T + rp + jl];
+  const uint64_t a6 = src[6 * T + word];
+
+  uint64_t n[7];
+  n[0] = lg_shl_in(a0, a0l, cl);
+  n[1] = odd ? a1 : lg_shl_in(a1, a1l, cl);
+  n[2] = odd ? lg_shr_in(a2, a2r, cr) : a2;
+  n[3] = lg_shr_in(a3, a3r, cr);
+  n[4] = odd ? lg_shr_in(a4, a4r, cr) : a4;
+  n[5] = odd ? a5 : lg_shl_in(a5, a5l, cl);
+  n[6] = a6;
+
+  uint64_t o[7];
+  {
+    uint32_t nh[7], oh[7];
+    if (NH == 1) {
+      LG_UNROLL
+      for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)n[d];
+      if (G.rest)
+        lg_collide_rest<uint32_t>(nh, Kl, oh);
+      else
+        lg_collide_norest<uint32_t>(nh, Kl, oh);
+      LG_UNROLL
+      for (int d = 0; d < 7; ++d) o[d] = oh[d];
+    }
+    const uint32_t sh = NH == 1 ? 32u : 32u * h;
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)(n[d] >> sh);
+    if (G.rest)
+      lg_collide_rest<uint32_t>(nh, Kh, oh);
+    else
+      lg_collide_norest<uint32_t>(nh, Kh, oh);
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) o[d] = (NH == 1 ? o[d] : 0ull) | ((uint64_t)oh[d] << sh);
+  }
+
+  // walls: bounce back, no collision (src/hex.rs:45-48, src/lattice.rs:358-359)
+  uint64_t res[7];
+  LG_UNROLL
+  for (int d = 0; d < 6; ++d) res[d] = (sm & n[(d + 3) % 6]) | (~sm & o[d]);
+  res[6] = (sm & n[6]) | (~sm & o[6]);
+
+  if (j < G.iw) {
+    const uint64_t mine = NH == 1 ? ~0ull : (0xFFFFFFFFull << (32 * h));
+    const uint64_t apply = lg_inlet_apply(G, j, ~sm & tail) & mine;
+    if (apply) {
+      const uint32_t* __restrict iin = iplanes + 2 * ((size_t)y * G.iw + j) + (NH == 1 ? 0 : h);
+      LG_UNROLL
+      for (int d = 0; d < 7; ++d) {
+        uint64_t bits = NH == 1 ? ((uint64_t)iin[d * istride + 1] << 32) | iin[d * istride]
+                                : (uint64_t)iin[d * istride] << (32 * h);
+        res[d] = (res[d] & ~apply) | (bits & apply);
+      }
     }
   }
   if (NH == 1) {

@@ -89,6 +89,48 @@ lg_step_kernel_h(LgGeom G, const uint64_t* __restrict src, uint64_t* __restrict 
   lg_step_body<2>(G, src, dst, solid, cjump, row_band, ijump, order, tabs);
 }
 
+// Two-pass step, one launch per step: collide blocks update the lattice for
+// step t with the digits drawn earlier (kcur/icur), draw blocks make step
+// t + 1's digits (knext/inext, from tabs_next) -- the draws never read the
+// lattice, so the two kinds of work share a launch and the draw blocks keep
+// the VALUs busy while collide blocks wait on memory. Roles alternate by
+// block index while both remain, so every CU gets a mix; a launch may carry
+// only one of them (cblocks or dblocks zero). Collide unit u is word u
+// (NH = 1) or half u % 2 of word u / 2 (NH = 2) in row-major order; draw unit
+// v is word order[v] (DNH = 1) or half v % 2 of word order[v / 2] (DNH = 2).
+template <int NH, int DNH>
+__global__ void __launch_bounds__(256)
+lg_step2_kernel(LgGeom G, const uint64_t* __restrict src, uint64_t* __restrict dst,
+                const uint64_t* __restrict solid, const uint64_t* __restrict cjump,
+                const uint32_t* __restrict row_band, const uint64_t* __restrict ijump,
+                const uint32_t* __restrict order, const uint64_t* __restrict tabs_next,
+                const uint32_t* __restrict kcur, const uint32_t* __restrict icur,
+                uint32_t* __restrict knext, uint32_t* __restrict inext, uint32_t cblocks,
+                uint32_t dblocks) {
+  const uint32_t m = min(cblocks, dblocks);
+  uint32_t b = blockIdx.x, draw;
+  if (b < 2 * m) {
+    draw = b & 1u;
+    b >>= 1;
+  } else {
+    draw = cblocks > dblocks ? 0u : 1u;
+    b -= m;
+  }
+  const size_t kstride = 2 * (size_t)G.total, istride = 2 * (size_t)G.iw * G.h;
+  const uint32_t u = b * blockDim.x + threadIdx.x;
+  if (draw) {
+    if (u >= G.total * DNH) return;
+    const uint32_t e = order[DNH == 1 ? u : u >> 1];
+    lg_draw_unit<DNH>(G, solid, cjump, row_band, ijump, tabs_next, e >> 16, e & 0xFFFFu,
+                      DNH == 1 ? 0u : (u & 1u), knext, kstride, inext, istride);
+  } else {
+    if (u >= G.total * NH) return;
+    const uint32_t word = NH == 1 ? u : u >> 1;
+    const uint32_t y = word / G.wpr;
+    lg_collide_unit<NH>(G, src, dst, solid, kcur, kstride, icur, istride, y, word - y * G.wpr,
+                        NH == 1 ? 0u : (u & 1u));
+  }
+}
 // Bytes -> planes (GpuLattice::load, src/gpu.rs:572-610), one thread per word.
 __global__ void __launch_bounds__(256) lg_pack_kernel(LgGeom G, const uint8_t* __restrict cells,
                                                       uint64_t* __restrict planes,
@@ -375,6 +417,7 @@ int rebuild_tables(lg_t* g, const std::vector<uint64_t>& solid_host) {
   if (!ij.empty())
     HIP_TRY(hipMemcpyAsync(g->ijump.p, ij.data(), ij.size() * 8, hipMemcpyHostToDevice, g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
+  g->tables_dirty = false;
   return LG_OK;
 }
 
@@ -400,11 +443,10 @@ int add_solid(lg_t* g, int kind, float a, float b, float c, float d) {
   hipLaunchKernelGGL(lg_solid_kernel, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream, g->G,
                      kind, a, b, c, d, g->a.p, g->solid.p);
   HIP_TRY(hipGetLastError());
-  std::vector<uint64_t> s;
-  int rc = download_solid(g, s);
-  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(g->stream));
   g->loaded = true;
-  return rebuild_tables(g, s);
+  g->tables_dirty = true;
+  return LG_OK;
 }
 
 int sample_now(lg_t* g, float alpha) {
@@ -417,16 +459,50 @@ int sample_now(lg_t* g, float alpha) {
   return LG_OK;
 }
 
+// Digit and inflow planes of the two-pass step: two slots of one step each
+// (the step being collided, the step being drawn), sized for the current
+// inflow width.
+int ensure_draw_buffers(lg_t* g) {
+  const size_t nhalf = (size_t)2 * g->G.total, nih = (size_t)2 * g->G.iw * g->G.h;
+  if (g->kbuf.n >= 2 * LG_KDIGITS * nhalf && g->ibuf.n >= std::max<size_t>(2 * 7 * nih, 1))
+    return LG_OK;
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  HIP_TRY(g->kbuf.alloc(2 * LG_KDIGITS * nhalf));
+  HIP_TRY(g->ibuf.alloc(std::max<size_t>(2 * 7 * nih, 1)));
+  return LG_OK;
+}
+
+// One two-pass launch: collide the current step from digit slot `cur` if
+// `collide`, draw into slot cur ^ 1 from tabs if `draw`.
+void launch_step2(lg_t* g, uint32_t cur, bool collide, bool draw, const uint64_t* tabs) {
+  const size_t nhalf = (size_t)2 * g->G.total, nih = (size_t)2 * g->G.iw * g->G.h;
+  const uint32_t cblocks = collide ? blocks_for((size_t)g->G.total * g->units) : 0;
+  const uint32_t dblocks = draw ? blocks_for((size_t)g->G.total * g->draw_units) : 0;
+  const uint32_t* kc = g->kbuf.p + (size_t)cur * LG_KDIGITS * nhalf;
+  const uint32_t* ic = g->ibuf.p + (size_t)cur * 7 * nih;
+  uint32_t* kn = g->kbuf.p + (size_t)(cur ^ 1u) * LG_KDIGITS * nhalf;
+  uint32_t* in = g->ibuf.p + (size_t)(cur ^ 1u) * 7 * nih;
+  auto k = g->units == 2 ? (g->draw_units == 2 ? lg_step2_kernel<2, 2> : lg_step2_kernel<2, 1>)
+                         : (g->draw_units == 2 ? lg_step2_kernel<1, 2> : lg_step2_kernel<1, 1>);
+  hipLaunchKernelGGL(k, dim3(cblocks + dblocks), dim3(256), 0, g->stream, g->G, g->a.p, g->b.p,
+                     g->solid.p, g->cjump.p, g->row_band.p, g->ijump.p, g->order.p, tabs, kc, ic,
+                     kn, in, cblocks, dblocks);
+}
+
 // Enqueue `n` steps (and samples every `every`), no synchronisation.
 int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) {
+  int trc = ensure_tables(g);
+  if (trc) return trc;
   uint64_t left = n, since = 0;
   const uint32_t per = g->G.nbands + 1;
   if (timed) g->tev_used = 0;
   while (left > 0) {
-    uint64_t chunk = std::min<uint64_t>(left, g->basis_steps);
+    const uint32_t extra = g->passes == 2 ? 1u : 0u;  // see the two-pass branch below
+    uint64_t chunk = std::min<uint64_t>(left, g->basis_steps - extra);
     if (every > 0) chunk = std::min<uint64_t>(chunk, every - since);
-    hipLaunchKernelGGL(lg_tables_kernel, dim3(blocks_for((size_t)chunk * per * 16)), dim3(256), 0,
-                       g->stream, g->seed, g->steps, (uint32_t)chunk, g->G.nbands, g->basis.p);
+    const uint32_t ntab = (uint32_t)chunk + extra;
+    hipLaunchKernelGGL(lg_tables_kernel, dim3(blocks_for((size_t)ntab * per * 16)), dim3(256), 0,
+                       g->stream, g->seed, g->steps, ntab, g->G.nbands, g->basis.p);
     HIP_TRY(hipGetLastError());
     if (timed) {
       while (g->tev.size() < 2 * (g->tev_used + 1)) {
@@ -436,17 +512,37 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
       }
       HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used], g->stream));
     }
-    for (uint64_t k = 0; k < chunk; ++k) {
-      if (g->units == 2)
-        hipLaunchKernelGGL(lg_step_kernel_h, dim3(blocks_for((size_t)g->G.total * 2)), dim3(256), 0,
-                           g->stream, g->G, g->a.p, g->b.p, g->solid.p, g->cjump.p, g->row_band.p,
-                           g->ijump.p, g->order.p, g->basis.p + (size_t)k * per * LG_WTAB);
-      else
-        hipLaunchKernelGGL(lg_step_kernel_w, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream,
-                           g->G, g->a.p, g->b.p, g->solid.p, g->cjump.p, g->row_band.p, g->ijump.p,
-                           g->order.p, g->basis.p + (size_t)k * per * LG_WTAB);
-      std::swap(g->a.p, g->b.p);
-      g->steps += 1;
+    if (g->passes == 1) {
+      for (uint64_t k = 0; k < chunk; ++k) {
+        if (g->units == 2)
+          hipLaunchKernelGGL(lg_step_kernel_h, dim3(blocks_for((size_t)g->G.total * 2)), dim3(256),
+                             0, g->stream, g->G, g->a.p, g->b.p, g->solid.p, g->cjump.p,
+                             g->row_band.p, g->ijump.p, g->order.p,
+                             g->basis.p + (size_t)k * per * LG_WTAB);
+        else
+          hipLaunchKernelGGL(lg_step_kernel_w, dim3(blocks_for(g->G.total)), dim3(256), 0,
+                             g->stream, g->G, g->a.p, g->b.p, g->solid.p, g->cjump.p,
+                             g->row_band.p, g->ijump.p, g->order.p,
+                             g->basis.p + (size_t)k * per * LG_WTAB);
+        std::swap(g->a.p, g->b.p);
+        g->steps += 1;
+      }
+    } else {
+      // The tables of this chunk cover one step more than it takes, so its
+      // last launch can draw the next chunk's first step.
+      int rc = ensure_draw_buffers(g);
+      if (rc) return rc;
+      if (left == n) {  // the call's first step: draw it on its own
+        launch_step2(g, 1, false, true, g->basis.p);
+        g->kslot = 0;
+      }
+      for (uint64_t k = 0; k < chunk; ++k) {
+        const bool more = k + 1 < chunk || left > chunk;
+        launch_step2(g, g->kslot, true, more, g->basis.p + (size_t)(k + 1) * per * LG_WTAB);
+        g->kslot ^= 1u;
+        std::swap(g->a.p, g->b.p);
+        g->steps += 1;
+      }
     }
     HIP_TRY(hipGetLastError());
     if (timed) HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used++ + 1], g->stream));
@@ -462,6 +558,14 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
 }
 
 }  // namespace
+
+int lgca_internal::ensure_tables(lg_t* g) {
+  if (!g->tables_dirty) return LG_OK;
+  std::vector<uint64_t> s;
+  int rc = download_solid(g, s);
+  if (rc) return rc;
+  return rebuild_tables(g, s);
+}
 
 extern "C" {
 
@@ -511,7 +615,7 @@ lg_t* lg_create(size_t w, size_t h, int rest, uint64_t seed, size_t bands) {
   g->seed = seed;
   const size_t per = G.nbands + 1;
   g->basis_steps = (uint32_t)std::max<size_t>(
-      1, std::min<size_t>(256, (32u << 20) / (per * LG_WTAB * 8)));
+      2, std::min<size_t>(256, (32u << 20) / (per * LG_WTAB * 8)));
   if (g->a.alloc((size_t)7 * G.total) || g->b.alloc((size_t)7 * G.total) ||
       g->solid.alloc(G.total) || g->basis.alloc((size_t)g->basis_steps * per * LG_WTAB) ||
       g->counter.alloc(1))
@@ -521,8 +625,8 @@ lg_t* lg_create(size_t w, size_t h, int rest, uint64_t seed, size_t bands) {
       hipMemsetAsync(g->solid.p, 0, g->solid.n * 8, g->stream) ||
       hipStreamSynchronize(g->stream))
     return bail("hipMemset failed");
-  std::vector<uint64_t> empty((size_t)G.total, 0);
-  if (rebuild_tables(g, empty) != LG_OK) return bail(g_err);
+  if (lgca::charpoly_low() == 0) return bail("jump polynomial not found");
+  g->tables_dirty = true;  // built before the first step
   return g;
 }
 
@@ -567,17 +671,8 @@ int lg_set_inlet(lg_t* g, size_t cols, const uint32_t thresh[7]) {
   uint32_t c = (uint32_t)std::min<size_t>(cols, g->G.w);
   g->G.inlet_cols = c;
   g->G.iw = (c + 63) / 64;
-  // inflow words move to the front of the grid, which needs wpr > iw or
-  // all words inflow; both are handled by the mapping.
-  if (!g->loaded) {
-    std::vector<uint64_t> empty((size_t)g->G.total, 0);
-    return rebuild_tables(g, empty);
-  }
-  // rebuild the inflow tables from the device's solid plane
-  std::vector<uint64_t> s;
-  int rc = download_solid(g, s);
-  if (rc) return rc;
-  return rebuild_tables(g, s);
+  g->tables_dirty = true;  // the inflow tables and word order change
+  return LG_OK;
 }
 
 int lg_set_inlet_flow(lg_t* g, size_t cols, float density, float ux, float uy) {
@@ -751,6 +846,21 @@ int lg_set_work_unit(lg_t* g, uint32_t cells_per_lane) {
   if (cells_per_lane != 64 && cells_per_lane != 32)
     return fail(LG_EINVAL, "cells per lane must be 64 or 32");
   g->units = cells_per_lane == 64 ? 1u : 2u;
+  return LG_OK;
+}
+
+int lg_set_draw_unit(lg_t* g, uint32_t cells_per_lane) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  if (cells_per_lane != 64 && cells_per_lane != 32)
+    return fail(LG_EINVAL, "cells per lane must be 64 or 32");
+  g->draw_units = cells_per_lane == 64 ? 1u : 2u;
+  return LG_OK;
+}
+
+int lg_set_passes(lg_t* g, uint32_t passes) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  if (passes != 1 && passes != 2) return fail(LG_EINVAL, "passes per step must be 1 or 2");
+  g->passes = passes;
   return LG_OK;
 }
 

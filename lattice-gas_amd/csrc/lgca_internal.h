@@ -10,6 +10,8 @@
 #include "../../include/lgca.h"
 #include "lgca_device.h"
 
+struct lg_lattice;
+
 namespace lgca_internal {
 
 extern thread_local std::string g_err;
@@ -32,9 +34,13 @@ struct DevBuf {
   size_t n = 0;
   hipError_t alloc(size_t count) {
     release();
-    n = count;
     if (count == 0) return hipSuccess;
-    return hipMalloc((void**)&p, count * sizeof(T));
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e == hipSuccess)
+      n = count;
+    else
+      p = nullptr;
+    return e;
   }
   void release() {
     if (p) (void)hipFree(p);
@@ -48,6 +54,10 @@ inline uint32_t blocks_for(size_t n, uint32_t t = 256) { return (uint32_t)((n + 
 
 // Equilibrium thresholds exactly as src/lattice.rs:59-82 computes them.
 void equilibrium_thresholds(float density, float ux, float uy, bool rest, uint32_t out[7]);
+
+// Rebuild the static jump and order tables of a lattice if its obstacle
+// layout or inflow width changed since they were last built.
+int ensure_tables(lg_lattice* g);
 
 }  // namespace lgca_internal
 
@@ -66,10 +76,18 @@ struct lg_lattice {
   uint64_t seed = 0, steps = 0;
   uint32_t bands = 1;
   bool loaded = false;
+  bool tables_dirty = true;  // cjump/ijump/row_band/order out of date
   // planes: a (current), b (next); solid; static tables
   DevBuf<uint64_t> a, b, solid, cjump, ijump, basis;
   DevBuf<uint32_t> row_band, order;
-  uint32_t units = 2;  // 1: a lane steps a 64-cell word; 2: a 32-cell half
+  uint32_t units = 1;  // 1: a lane steps a 64-cell word; 2: a 32-cell half
+  // passes per step: 1 = one fused kernel per step; 2 = one launch per step
+  // that collides step t and draws step t + 1's random digits into the other
+  // slot of kbuf/ibuf (kslot: the slot holding the current step's digits)
+  uint32_t passes = 2;
+  uint32_t draw_units = 1;  // draw lanes of the two-pass step: 1 = 64 cells, 2 = 32
+  uint32_t kslot = 0;
+  DevBuf<uint32_t> kbuf, ibuf;
   DevBuf<uint8_t> bytes;
   uint32_t basis_steps = 0;  // capacity of `basis` in steps
   // field

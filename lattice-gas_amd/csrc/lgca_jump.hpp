@@ -71,14 +71,46 @@ inline uint64_t charpoly_low() {
   return p;
 }
 
-// c(z) * z^f mod P(z)
-inline uint64_t poly_mulz(uint64_t c, uint32_t f, uint64_t p) {
+// c(z) * z^f mod P(z), one bit at a time
+inline uint64_t poly_mulz_slow(uint64_t c, uint32_t f, uint64_t p) {
   for (uint32_t i = 0; i < f; ++i) {
     uint64_t carry = c >> 63;
     c <<= 1;
     if (carry) c ^= p;
   }
   return c;
+}
+
+// c(z) * z^f mod P(z) for f <= 32: the f bits shifted past z^63 are folded
+// back with four byte tables, R[i][b] = b(z) z^(64 + 8i) mod P(z).
+struct MulzTables {
+  uint64_t p = 0;
+  uint64_t R[4][256];
+  explicit MulzTables(uint64_t poly) : p(poly) {
+    for (int i = 0; i < 4; ++i)
+      for (uint32_t b = 0; b < 256; ++b) {
+        uint64_t acc = 0;
+        for (int k = 0; k < 8; ++k)
+          if ((b >> k) & 1u) acc ^= poly_mulz_slow(1, 64 + 8 * i + k, poly);
+        R[i][b] = acc;
+      }
+  }
+  uint64_t mulz(uint64_t c, uint32_t f) const {
+    if (f == 0) return c;
+    const uint64_t over = c >> (64 - f);  // coefficients of z^64 .. z^(63+f)
+    return (c << f) ^ R[0][over & 255u] ^ R[1][(over >> 8) & 255u] ^ R[2][(over >> 16) & 255u] ^
+           R[3][(over >> 24) & 255u];
+  }
+};
+
+inline uint64_t poly_mulz(uint64_t c, uint32_t f, uint64_t p) {
+  static const MulzTables tabs(p);
+  if (p != tabs.p) return poly_mulz_slow(c, f, p);
+  while (f > 32) {
+    c = tabs.mulz(c, 32);
+    f -= 32;
+  }
+  return tabs.mulz(c, f);
 }
 
 // a(z) * b(z) mod P(z)

@@ -59,6 +59,8 @@ def load_library(path: str | None = None):
         "lg_steps": (u64, [vp]),
         "lg_set_steps": (i32, [vp, u64]),
         "lg_set_work_unit": (i32, [vp, C.c_uint32]),
+        "lg_set_passes": (i32, [vp, C.c_uint32]),
+        "lg_set_draw_unit": (i32, [vp, C.c_uint32]),
         "lg_batch_set_units": (i32, [vp, C.c_uint32]),
         "lg_batch_get_units": (i32, [vp, P(C.c_uint32)]),
         "lg_advance_timed": (i32, [vp, u64, P(f32)]),
@@ -201,9 +203,19 @@ class GpuLattice:
         """64 (a lane steps a u64 word) or 32 (half a word, twice the lanes)."""
         _check(load_library().lg_set_work_unit(self._p, int(cells_per_lane)))
 
+    def set_passes(self, passes: int):
+        """2 (each launch collides one step and draws the next step's random
+        columns; the default) or 1 (draw and collide fused per lane)."""
+        _check(load_library().lg_set_passes(self._p, int(passes)))
+
+    def set_draw_unit(self, cells_per_lane: int):
+        """64 or 32 cells per lane for the draw work of the two-pass step."""
+        _check(load_library().lg_set_draw_unit(self._p, int(cells_per_lane)))
+
     def advance_timed(self, n: int, sample_every: int = 0, alpha: float = 0.0) -> float:
-        """Advance and return the device milliseconds between HIP events
-        recorded on the engine's stream around the step launches."""
+        """Advance and return the device milliseconds of the step kernels,
+        from HIP event pairs on the engine's stream around each run of step
+        launches."""
         ms = C.c_float()
         _check(load_library().lg_advance_sampling_timed(self._p, int(n), int(sample_every),
                                                         float(alpha), C.byref(ms)))

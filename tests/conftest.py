@@ -18,8 +18,18 @@ HOSTSIM_SRC = os.path.join(ROOT, "tests", "hostsim", "lgca_hostsim.cpp")
 HOSTSIM_LIB = os.path.join(ROOT, "tests", "hostsim", "build", "liblgca_hostsim.so")
 
 
+def _hostsim_stale():
+    if not os.path.exists(HOSTSIM_LIB):
+        return True
+    built = os.path.getmtime(HOSTSIM_LIB)
+    csrc = os.path.join(ROOT, "lattice-gas_amd", "csrc")
+    deps = [HOSTSIM_SRC] + [os.path.join(csrc, f) for f in os.listdir(csrc)
+                            if f.endswith((".h", ".hpp"))]
+    return any(os.path.getmtime(d) > built for d in deps)
+
+
 def build_hostsim(force=False):
-    if force or not os.path.exists(HOSTSIM_LIB):
+    if force or _hostsim_stale():
         os.makedirs(os.path.dirname(HOSTSIM_LIB), exist_ok=True)
         subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
                         "-o", HOSTSIM_LIB, HOSTSIM_SRC], check=True)

@@ -2,6 +2,7 @@
 // the host and driven over a whole lattice, so CPU tests can compare the
 // exact device algorithm with the oracle without a GPU. Not part of the
 // product: nothing outside tests/ loads it.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -16,6 +17,9 @@ struct Sim {
   std::vector<uint64_t> a, b, solid, cjump, ijump, basis;
   std::vector<uint32_t> row_band, order;
   uint32_t units = 1;
+  uint32_t passes = 1;
+  uint32_t draw_units = 2;
+  std::vector<uint32_t> kbuf, ibuf;
 };
 }  // namespace
 
@@ -49,6 +53,8 @@ void* hs_new(uint32_t w, uint32_t h, int rest, uint64_t seed, uint32_t bands, ui
 
 void hs_free(void* p) { delete static_cast<Sim*>(p); }
 void hs_set_units(void* p, uint32_t units) { static_cast<Sim*>(p)->units = units; }
+void hs_set_passes(void* p, uint32_t passes) { static_cast<Sim*>(p)->passes = passes; }
+void hs_set_draw_units(void* p, uint32_t units) { static_cast<Sim*>(p)->draw_units = units; }
 
 void hs_load(void* p, const uint8_t* cells) {
   auto* S = static_cast<Sim*>(p);
@@ -91,6 +97,38 @@ void hs_advance(void* p, uint64_t n) {
       uint64_t st = lg_rng_init(s < G.nbands ? lg_band_seed(S->seed, t, s)
                                              : lg_inlet_seed(S->seed, t + 1));
       lg_build_tab(st, S->basis.data() + (size_t)s * LG_WTAB);
+    }
+    if (S->passes == 2) {
+      // the two-pass step: every half's draws, then every unit's collide
+      const size_t nhalf = (size_t)2 * G.total, nih = (size_t)2 * G.iw * G.h;
+      S->kbuf.assign(LG_KDIGITS * nhalf, 0xDEADBEEFu);
+      S->ibuf.assign(std::max<size_t>(7 * nih, 1), 0xDEADBEEFu);
+      for (size_t g = 0; g < (size_t)G.total * S->draw_units; ++g) {
+        if (S->draw_units == 1) {
+          uint32_t e = S->order[g];
+          lg_draw_unit<1>(G, S->solid.data(), S->cjump.data(), S->row_band.data(), S->ijump.data(),
+                          S->basis.data(), e >> 16, e & 0xFFFFu, 0, S->kbuf.data(), nhalf,
+                          S->ibuf.data(), nih);
+        } else {
+          uint32_t e = S->order[g >> 1];
+          lg_draw_unit<2>(G, S->solid.data(), S->cjump.data(), S->row_band.data(), S->ijump.data(),
+                          S->basis.data(), e >> 16, e & 0xFFFFu, (uint32_t)(g & 1u), S->kbuf.data(),
+                          nhalf, S->ibuf.data(), nih);
+        }
+      }
+      for (size_t u = 0; u < (size_t)G.total * S->units; ++u) {
+        uint32_t word = (uint32_t)(S->units == 1 ? u : u >> 1);
+        uint32_t y = word / G.wpr, j = word - y * G.wpr;
+        if (S->units == 1)
+          lg_collide_unit<1>(G, S->a.data(), S->b.data(), S->solid.data(), S->kbuf.data(), nhalf,
+                             S->ibuf.data(), nih, y, j, 0);
+        else
+          lg_collide_unit<2>(G, S->a.data(), S->b.data(), S->solid.data(), S->kbuf.data(), nhalf,
+                             S->ibuf.data(), nih, y, j, (uint32_t)(u & 1u));
+      }
+      std::swap(S->a, S->b);
+      S->steps += 1;
+      continue;
     }
     // visit the words in the device's order, each as S->units units
     for (uint32_t e : S->order)
@@ -136,6 +174,21 @@ void hs_collide_table(int rest, uint8_t* out) {
 }
 
 uint64_t hs_charpoly(void) { return lgca::charpoly_low(); }
+
+// Table-driven multiplication by z^f mod P against the bit-serial one, on n
+// pseudo-random (c, f) pairs with f up to 199; returns the mismatch count.
+int hs_mulz_check(uint64_t seed, int n) {
+  const uint64_t p = lgca::charpoly_low();
+  uint64_t x = seed | 1u;
+  int bad = 0;
+  for (int i = 0; i < n; ++i) {
+    x = lg_xs(x);
+    uint64_t c = x * 0x9E3779B97F4A7C15ull;
+    uint32_t f = (uint32_t)(x >> 40) % 200u;
+    bad += lgca::poly_mulz(c, f, p) != lgca::poly_mulz_slow(c, f, p);
+  }
+  return bad;
+}
 uint64_t hs_jump_state(uint64_t s, uint64_t n) { return lgca::jump_state(s, n); }
 
 }  // extern "C"

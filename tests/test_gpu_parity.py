@@ -73,9 +73,9 @@ def test_golden_setup_on_the_device_matches_the_cpu(oracle, case):
 
 
 def _compare_run(oracle, w, h, rest, bands, inlet, seed, steps, density=0.22, inflow=(0.3, 0.0),
-                 disc=None, box=None, check_every=1, cells_per_lane=64):
+                 disc=None, box=None, check_every=1, cells_per_lane=64, passes=(2, 32)):
     say(f"run {w}x{h} rest={rest} bands={bands} inlet={inlet} steps={steps} disc={disc} box={box} "
-        f"cells/lane={cells_per_lane}")
+        f"cells/lane={cells_per_lane} passes={passes}")
     lat = oracle.Lattice(w, h, density, inflow, inlet, rest, bands, seed)
     if disc:
         lat.add_solid_disc(*disc)
@@ -84,6 +84,8 @@ def _compare_run(oracle, w, h, rest, bands, inlet, seed, steps, density=0.22, in
     lat.init_equilibrium()
     g = GpuLattice(w, h, rest, seed, bands)
     g.set_work_unit(cells_per_lane)
+    g.set_passes(passes[0])
+    g.set_draw_unit(passes[1])
     g.set_inlet(inlet, density, inflow)
     g.load(lat.cells)
     done = 0
@@ -101,6 +103,10 @@ def _compare_run(oracle, w, h, rest, bands, inlet, seed, steps, density=0.22, in
     return lat, g
 
 
+# (kernels per step, draw cells per lane)
+PIPELINES = [(1, 32), (2, 32), (2, 64)]
+
+
 @pytest.mark.parametrize("w,h,rest,bands,inlet", [
     (65, 6, True, 1, 8),        # one cell past a word, tiny height
     (100, 18, True, 3, 8),      # ragged tail word, bands of 6
@@ -110,12 +116,13 @@ def _compare_run(oracle, w, h, rest, bands, inlet, seed, steps, density=0.22, in
     (320, 10, True, 10, 8),     # one row per band
     (80, 48, True, 2, 80),      # inflow over the whole width
 ])
+@pytest.mark.parametrize("passes", PIPELINES)
 @pytest.mark.parametrize("cells_per_lane", [64, 32])
-def test_ragged_shapes_every_step(oracle, w, h, rest, bands, inlet, cells_per_lane):
+def test_ragged_shapes_every_step(oracle, w, h, rest, bands, inlet, cells_per_lane, passes):
     yc = float(np.float32(np.float32(h) * oracle.SQRT3_2 / np.float32(2)))
     disc = (w / 3.0, yc, 3.5)
     _compare_run(oracle, w, h, rest, bands, inlet, 0xC0FFEE + w, 40, disc=disc,
-                 cells_per_lane=cells_per_lane)
+                 cells_per_lane=cells_per_lane, passes=passes)
 
 
 @pytest.mark.parametrize("bands", [1, 2, 3, 5, 8, 16])
@@ -259,16 +266,19 @@ def test_closed_box_conserves_at_full_config2_size():
     assert g.total_particles() == n0 == inv0[0]
 
 
+@pytest.mark.parametrize("passes", PIPELINES)
 @pytest.mark.parametrize("cells_per_lane", [64, 32])
-def test_config2_size_matches_the_cpu_replay(oracle, cells_per_lane):
+def test_config2_size_matches_the_cpu_replay(oracle, cells_per_lane, passes):
     """Config 2 geometry (4096x2304, plate H/8 at W/4, 8 inflow columns,
     bands 16) for 60 steps against the oracle, every 20 steps."""
     cfg = W.config2(bands=16)
-    say(f"config2 replay {cfg} cells/lane={cells_per_lane}")
+    say(f"config2 replay {cfg} cells/lane={cells_per_lane} passes={passes}")
     lat = config_oracle(oracle, cfg)
     lat.set_parallel(True)
     g = W.config_gpu(cfg)
     g.set_work_unit(cells_per_lane)
+    g.set_passes(passes[0])
+    g.set_draw_unit(passes[1])
     for _ in range(3):
         lat.advance(20)
         g.advance(20)

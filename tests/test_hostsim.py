@@ -19,6 +19,8 @@ def hs():
     L.hs_new.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, P(C.c_uint32)]
     L.hs_free.argtypes = [C.c_void_p]
     L.hs_set_units.argtypes = [C.c_void_p, C.c_uint32]
+    L.hs_set_passes.argtypes = [C.c_void_p, C.c_uint32]
+    L.hs_set_draw_units.argtypes = [C.c_void_p, C.c_uint32]
     L.hs_load.argtypes = [C.c_void_p, P(C.c_uint8)]
     L.hs_store.argtypes = [C.c_void_p, P(C.c_uint8)]
     L.hs_advance.argtypes = [C.c_void_p, C.c_uint64]
@@ -26,7 +28,15 @@ def hs():
     L.hs_charpoly.restype = C.c_uint64
     L.hs_jump_state.restype = C.c_uint64
     L.hs_jump_state.argtypes = [C.c_uint64, C.c_uint64]
+    L.hs_mulz_check.restype = C.c_int
+    L.hs_mulz_check.argtypes = [C.c_uint64, C.c_int]
     return L
+
+
+def test_table_multiplication_by_z_powers(hs):
+    """The byte-table c(z) z^f mod P used for the jump tables equals the
+    bit-serial product."""
+    assert hs.hs_mulz_check(12345, 200000) == 0
 
 
 def u8p(a):
@@ -41,6 +51,10 @@ class HostSim:
 
     def set_units(self, units):
         self.L.hs_set_units(self.p, units)
+
+    def set_passes(self, passes, draw_units=2):
+        self.L.hs_set_passes(self.p, passes)
+        self.L.hs_set_draw_units(self.p, draw_units)
 
     def load(self, cells):
         a = np.ascontiguousarray(cells, dtype=np.uint8)
@@ -95,7 +109,7 @@ def test_jump_equals_sequential_steps(oracle, hs):
 
 
 def compare(oracle, hs, w, h, rest, bands, inlet, seed, steps, disc=None, box=None,
-            density=0.22, inflow=(0.3, 0.0), units=1):
+            density=0.22, inflow=(0.3, 0.0), units=1, passes=(1, 2)):
     lat = oracle.Lattice(w, h, density, inflow, inlet, rest, bands, seed)
     if disc:
         lat.add_solid_disc(*disc)
@@ -104,6 +118,7 @@ def compare(oracle, hs, w, h, rest, bands, inlet, seed, steps, disc=None, box=No
     lat.init_equilibrium()
     sim = HostSim(hs, w, h, rest, seed, bands, inlet, oracle.thresholds(density, *inflow, rest))
     sim.set_units(units)
+    sim.set_passes(*passes)
     sim.load(lat.cells)
     for s in range(steps):
         lat.advance(1)
@@ -115,20 +130,26 @@ def compare(oracle, hs, w, h, rest, bands, inlet, seed, steps, disc=None, box=No
                         f"{[(int(b) % w, int(b) // w) for b in bad[:4]]}")
 
 
+PIPELINES = [(1, 2), (2, 2), (2, 1)]  # (passes, draw units per word)
+
+
+@pytest.mark.parametrize("passes", PIPELINES)
 @pytest.mark.parametrize("units", [1, 2])
 @pytest.mark.parametrize("idx", range(5))
-def test_golden_cases_every_step(oracle, hs, idx, units):
+def test_golden_cases_every_step(oracle, hs, idx, units, passes):
     name, w, h, rest, inlet, plate, threads, steps = oracle.GOLDEN_CASES[idx]
     box = oracle.golden_plate(w, h) if plate else None
-    compare(oracle, hs, w, h, rest, threads, inlet, oracle.GOLDEN_SEED, 60, box=box, units=units)
+    compare(oracle, hs, w, h, rest, threads, inlet, oracle.GOLDEN_SEED, 60, box=box, units=units,
+            passes=passes)
 
 
 @pytest.mark.parametrize("w,h,rest,bands,inlet", [
     (65, 6, True, 1, 8), (100, 18, True, 3, 8), (129, 64, False, 4, 70), (200, 40, True, 7, 130),
     (64, 32, True, 5, 0), (320, 10, True, 10, 8), (80, 48, True, 2, 80), (127, 8, False, 3, 127),
 ])
+@pytest.mark.parametrize("passes", PIPELINES)
 @pytest.mark.parametrize("units", [1, 2])
-def test_ragged_shapes_every_step(oracle, hs, w, h, rest, bands, inlet, units):
+def test_ragged_shapes_every_step(oracle, hs, w, h, rest, bands, inlet, units, passes):
     yc = float(np.float32(np.float32(h) * oracle.SQRT3_2 / np.float32(2)))
     compare(oracle, hs, w, h, rest, bands, inlet, 0xC0FFEE + w, 25, disc=(w / 3.0, yc, 3.5),
-            units=units)
+            units=units, passes=passes)
