@@ -142,7 +142,7 @@ def ensemble_leg(n, steps, bands, seed=4):
     b.advance(steps - used)
     t_batch = time.perf_counter() - t0
     shape = b.shape()
-    units = b.units()
+    passes, units, dunits = b.pipeline()
     b.close()
     # the plain loop over the same lattices (rebuilt), one at a time
     t_plain = 0.0
@@ -161,7 +161,8 @@ def ensemble_leg(n, steps, bands, seed=4):
             "batched_gcups": round(gc_b, 3), "plain_loop_gcups": round(gc_p, 3),
             "speedup": round(gc_b / gc_p, 3), "batched_s": round(t_batch, 4),
             "autotuned_shape": {"tile": shape[0], "grid": shape[1], "chunk": shape[2],
-                                "cells_per_lane": units},
+                                "passes": passes, "cells_per_lane": units,
+                                "draw_cells_per_lane": dunits},
             "plain_loop_sample": f"first {probe} lattices, {steps} steps each"}
 
 

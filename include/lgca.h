@@ -193,14 +193,22 @@ int lg_batch_get_shape(lg_batch_t* b, uint32_t* tile, uint32_t* grid, uint32_t* 
 /* Cells per lane (64 or 32) for the batch's step kernel, as lg_set_work_unit. */
 int lg_batch_set_units(lg_batch_t* b, uint32_t cells_per_lane);
 int lg_batch_get_units(lg_batch_t* b, uint32_t* cells_per_lane);
+/* Kernels per step (2, the default: collide while drawing the next step; 1:
+ * fused) and the draw work's cells per lane, as lg_set_passes /
+ * lg_set_draw_unit for one lattice. */
+int lg_batch_set_passes(lg_batch_t* b, uint32_t passes);
+int lg_batch_set_draw_unit(lg_batch_t* b, uint32_t cells_per_lane);
+int lg_batch_get_pipeline(lg_batch_t* b, uint32_t* passes, uint32_t* cells_per_lane,
+                          uint32_t* draw_cells_per_lane);
 /* lg_sample_field for every member with a field, issued together. */
 int lg_batch_sample(lg_batch_t* b, float alpha);
 /* lg_sample_field without the wait (the sample is queued on the handle's
  * stream; the next synchronous call on the handle sees it). */
 int lg_sample_field_async(lg_t* g, float alpha);
-/* Time each candidate shape (tile, grid, chunk, cells per lane) on the batch itself (those steps are real steps
- * of every member, reported in *used_steps) and keep the fastest.
- * budget_steps = 0: no limit. */
+/* Time candidate launch pipelines and shapes (kernels per step, cells per
+ * lane for the collide and draw work, tile, grid) on the batch itself, most
+ * promising first, and keep the fastest. The timed steps are real steps of
+ * every member, reported in *used_steps; budget_steps = 0: no limit. */
 int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps);
 
 /* Diagnostics. The generated collision circuit as the device evaluates it,

@@ -11,8 +11,12 @@
 //   - per chunk of K steps: one tables launch for every member's streams, K
 //     step launches over all tiles (grid-stride, so a fixed persistent grid
 //     can walk any number of tiles), one launch that bumps the counters.
-// Every member computes exactly what its own lg_advance would: the per-word
-// work is lg_step_word with the member's own geometry and streams.
+// With the two-pass step (the default) the tile table holds collide tiles
+// and draw tiles, alternating: launch k collides step k of every member and
+// draws its step k + 1 into the member's other digit slot, and a call begins
+// with one draw-only launch for its first step.
+// Every member computes exactly what its own lg_advance would: the per-unit
+// work is the same device code with the member's own geometry and streams.
 #include <chrono>
 #include <cstring>
 
@@ -29,11 +33,15 @@ struct LgDesc {
   const uint32_t* row_band;
   const uint64_t* ijump;
   const uint32_t* order;
-  uint64_t* tabs;  // >= K * (nbands + 1) * LG_WTAB
+  uint64_t* tabs;  // >= (K + 1) * (nbands + 1) * LG_WTAB
+  uint32_t* kb;    // two digit slots (two-pass step)
+  uint32_t* ib;    // two inflow slots
   uint64_t seed;
   uint64_t steps;      // step index of the current state
   uint64_t remaining;  // steps this member still takes in the current call
   uint32_t cur;        // plane set holding the current state
+  uint32_t kpar;       // digit slot holding the current step's digits
+  uint32_t kfresh;     // 1 until the call's first chunk: its first step is not drawn yet
   uint32_t pad;
   // coarse field (src/render.rs:21-48); fux == nullptr when none is attached
   float *fux, *fuy, *frho, *fsol;
@@ -89,13 +97,72 @@ __global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) 
   }
 }
 
-__global__ void lg_bump_kernel(LgDesc* __restrict D, uint32_t n, uint32_t count) {
+// Two-pass batch step. A tile is (member, first unit | role << 31): a
+// collide tile runs units of step k from digit slot kpar + k, a draw tile
+// draws units of step k + 1 into the other slot, both exactly as
+// lg_step2_kernel maps them for a single lattice.
+template <int TILE, int NH, int DNH>
+__global__ void __launch_bounds__(TILE)
+    lg_step2_batch_kernel(const LgDesc* __restrict D, const uint2* __restrict tiles,
+                          uint32_t ntiles, uint32_t k) {
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint2 tl = tiles[t];
+    const LgDesc& d = D[tl.x];
+    const uint32_t u = (tl.y & 0x7FFFFFFFu) + threadIdx.x;
+    const size_t kstride = 2 * (size_t)d.G.total, istride = 2 * (size_t)d.G.iw * d.G.h;
+    const uint32_t slot = (d.kpar + k) & 1u;
+    if (tl.y >> 31) {
+      if (k + 1 >= d.remaining || u >= d.G.total * DNH) continue;
+      const uint32_t e = d.order[DNH == 1 ? u : u >> 1];
+      const uint64_t* tabs = d.tabs + (size_t)(k + 1) * (d.G.nbands + 1) * LG_WTAB;
+      lg_draw_unit<DNH>(d.G, d.solid, d.cjump, d.row_band, d.ijump, tabs, e >> 16, e & 0xFFFFu,
+                        DNH == 1 ? 0u : (u & 1u), d.kb + (size_t)(slot ^ 1u) * LG_KDIGITS * kstride,
+                        kstride, d.ib + (size_t)(slot ^ 1u) * 7 * istride, istride);
+    } else {
+      if (k >= d.remaining || u >= d.G.total * NH) continue;
+      const uint32_t par = (d.cur + k) & 1u;
+      const uint64_t* src = par ? d.p1 : d.p0;
+      uint64_t* dst = par ? d.p0 : d.p1;
+      const uint32_t word = NH == 1 ? u : u >> 1;
+      const uint32_t y = word / d.G.wpr;
+      lg_collide_unit<NH>(d.G, src, dst, d.solid, d.kb + (size_t)slot * LG_KDIGITS * kstride,
+                          kstride, d.ib + (size_t)slot * 7 * istride, istride, y,
+                          word - y * d.G.wpr, NH == 1 ? 0u : (u & 1u));
+    }
+  }
+}
+
+// The call's first step for members still fresh: its draw tiles only.
+template <int TILE, int DNH>
+__global__ void __launch_bounds__(TILE)
+    lg_draw0_batch_kernel(const LgDesc* __restrict D, const uint2* __restrict tiles,
+                          uint32_t ntiles) {
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint2 tl = tiles[t];
+    const LgDesc& d = D[tl.x];
+    const uint32_t u = (tl.y & 0x7FFFFFFFu) + threadIdx.x;
+    if (!(tl.y >> 31) || !d.kfresh || d.remaining == 0 || u >= d.G.total * DNH) continue;
+    const size_t kstride = 2 * (size_t)d.G.total, istride = 2 * (size_t)d.G.iw * d.G.h;
+    const uint32_t e = d.order[DNH == 1 ? u : u >> 1];
+    lg_draw_unit<DNH>(d.G, d.solid, d.cjump, d.row_band, d.ijump, d.tabs, e >> 16, e & 0xFFFFu,
+                      DNH == 1 ? 0u : (u & 1u), d.kb + (size_t)d.kpar * LG_KDIGITS * kstride,
+                      kstride, d.ib + (size_t)d.kpar * 7 * istride, istride);
+  }
+}
+
+// After a chunk: advance each member's counters; fresh_after = 1 when the
+// chunk ran the fused step (no digits were drawn ahead), so that a following
+// two-pass chunk draws its own first step.
+__global__ void lg_bump_kernel(LgDesc* __restrict D, uint32_t n, uint32_t count,
+                               uint32_t fresh_after) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint64_t adv = D[i].remaining < count ? D[i].remaining : count;
   D[i].steps += adv;
   D[i].cur ^= (uint32_t)(adv & 1u);
   D[i].remaining -= adv;
+  D[i].kpar ^= count & 1u;
+  D[i].kfresh = fresh_after;
 }
 
 // Coarse-grain every member with a field (blockIdx.y = member), exactly as
@@ -124,18 +191,20 @@ struct lg_batch {
   DevBuf<LgDesc> desc;
   DevBuf<uint2> tiles128, tiles256;
   uint32_t ntiles128 = 0, ntiles256 = 0;
-  uint32_t tiles_units = 0;  // NH the tile tables were built for
+  uint32_t tiles_key = 0;   // pipeline the tile tables were built for (0: none)
   uint32_t max_tab_blocks_per_step = 0;  // tables launch width per step of chunk
   uint32_t kmax = 0;                     // largest chunk the members' tables hold
   // tuned launch shape
   uint32_t tile = 256;   // words per tile (128 or 256)
   uint32_t grid = 0;     // step-kernel workgroups; 0 = one per tile
   uint32_t chunk = 32;   // steps per graph replay (even)
-  uint32_t units = 1;    // 1: a lane steps 64 cells, 2: 32 cells
+  uint32_t units = 1;    // collide (or fused) lanes: 1 = 64 cells, 2 = 32 cells
+  uint32_t passes = 2;   // 2: two-pass step, 1: fused step
+  uint32_t draw_units = 1;  // draw lanes of the two-pass step
   // graph for `chunk` steps with the current shape
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  uint32_t graph_chunk = 0, graph_tile = 0, graph_grid = 0, graph_units = 0;
+  uint32_t graph_chunk = 0, graph_tile = 0, graph_grid = 0, graph_key = 0;
   int ncu = 256;
   // host staging of the last uploads, alive until the stream has consumed them
   std::vector<LgDesc> host_desc;
@@ -157,10 +226,16 @@ void drop_graph(lg_batch* b) {
   b->graph_chunk = 0;
 }
 
+// The launch pipeline as one key (tile tables and graphs depend on it).
+uint32_t pipeline_key(const lg_batch* b) {
+  return b->passes * 100 + b->units * 10 + (b->passes == 2 ? b->draw_units : 0);
+}
+
 int upload_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
   for (lg_t* g : b->members) {
     int rc = ensure_tables(g);
     if (rc) return rc;
+    if ((rc = ensure_draw_buffers(g))) return rc;  // the pipeline may change within a call
   }
   std::vector<LgDesc>& h = b->host_desc;
   h.assign(b->members.size(), LgDesc{});
@@ -178,10 +253,14 @@ int upload_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
     d.ijump = g->ijump.p;
     d.order = g->order.p;
     d.tabs = g->basis.p;
+    d.kb = g->kbuf.p;
+    d.ib = g->ibuf.p;
     d.seed = g->seed;
     d.steps = g->steps;
     d.remaining = limits ? std::min<uint64_t>(limits[i], n) : n;
     d.cur = 0;
+    d.kpar = 0;
+    d.kfresh = 1;
     if (g->has_field) {
       d.fux = g->fux.p;
       d.fuy = g->fuy.p;
@@ -202,12 +281,30 @@ int upload_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
   return LG_OK;
 }
 
+// Tiles of TILE units: for the fused step, every member's units; for the
+// two-pass step, collide tiles and draw tiles (role bit 31) alternating.
 template <int TILE>
 int build_tiles(lg_batch* b, DevBuf<uint2>& buf, uint32_t& n, std::vector<uint2>& t) {
   t.clear();
+  std::vector<uint2> draws;
   for (size_t i = 0; i < b->members.size(); ++i) {
     uint32_t total = b->members[i]->G.total * b->units;
     for (uint32_t g0 = 0; g0 < total; g0 += TILE) t.push_back(make_uint2((uint32_t)i, g0));
+    if (b->passes == 2) {
+      uint32_t dtotal = b->members[i]->G.total * b->draw_units;
+      for (uint32_t g0 = 0; g0 < dtotal; g0 += TILE)
+        draws.push_back(make_uint2((uint32_t)i, g0 | 0x80000000u));
+    }
+  }
+  if (b->passes == 2) {
+    std::vector<uint2> mixed;
+    mixed.reserve(t.size() + draws.size());
+    size_t a = 0, d = 0;
+    while (a < t.size() || d < draws.size()) {
+      if (a < t.size()) mixed.push_back(t[a++]);
+      if (d < draws.size()) mixed.push_back(draws[d++]);
+    }
+    t.swap(mixed);
   }
   n = (uint32_t)t.size();
   HIP_TRY(buf.alloc(std::max<size_t>(t.size(), 1)));
@@ -218,48 +315,78 @@ int build_tiles(lg_batch* b, DevBuf<uint2>& buf, uint32_t& n, std::vector<uint2>
 }
 
 // Enqueue `count` steps (count <= kmax) on the batch stream.
+template <int TILE>
+void launch_steps(lg_batch* b, uint32_t grid, const uint2* tiles, uint32_t ntiles, uint32_t k,
+                  bool first) {
+  const dim3 gd(grid), bd(TILE);
+  if (b->passes == 1) {
+    if (b->units == 1)
+      hipLaunchKernelGGL((lg_step_batch_kernel<TILE, 1>), gd, bd, 0, b->stream, b->desc.p, tiles,
+                         ntiles, k);
+    else
+      hipLaunchKernelGGL((lg_step_batch_kernel<TILE, 2>), gd, bd, 0, b->stream, b->desc.p, tiles,
+                         ntiles, k);
+    return;
+  }
+  if (first) {
+    if (b->draw_units == 1)
+      hipLaunchKernelGGL((lg_draw0_batch_kernel<TILE, 1>), gd, bd, 0, b->stream, b->desc.p, tiles,
+                         ntiles);
+    else
+      hipLaunchKernelGGL((lg_draw0_batch_kernel<TILE, 2>), gd, bd, 0, b->stream, b->desc.p, tiles,
+                         ntiles);
+  }
+  if (b->units == 1 && b->draw_units == 1)
+    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 1, 1>), gd, bd, 0, b->stream, b->desc.p, tiles,
+                       ntiles, k);
+  else if (b->units == 1)
+    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 1, 2>), gd, bd, 0, b->stream, b->desc.p, tiles,
+                       ntiles, k);
+  else if (b->draw_units == 1)
+    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 2, 1>), gd, bd, 0, b->stream, b->desc.p, tiles,
+                       ntiles, k);
+  else
+    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 2, 2>), gd, bd, 0, b->stream, b->desc.p, tiles,
+                       ntiles, k);
+}
+
 int enqueue_chunk(lg_batch* b, uint32_t count) {
   const uint32_t nm = (uint32_t)b->members.size();
-  dim3 tgrid(b->max_tab_blocks_per_step * count, nm);
-  hipLaunchKernelGGL(lg_tables_batch_kernel, tgrid, dim3(256), 0, b->stream, b->desc.p, count);
+  // two-pass: one table more, for the draws of the next chunk's first step
+  const uint32_t ntab = count + (b->passes == 2 ? 1u : 0u);
+  dim3 tgrid(b->max_tab_blocks_per_step * ntab, nm);
+  hipLaunchKernelGGL(lg_tables_batch_kernel, tgrid, dim3(256), 0, b->stream, b->desc.p, ntab);
   HIP_TRY(hipGetLastError());
   const uint32_t ntiles = b->tile == 128 ? b->ntiles128 : b->ntiles256;
   const uint2* tiles = b->tile == 128 ? b->tiles128.p : b->tiles256.p;
   uint32_t grid = b->grid ? std::min(b->grid, ntiles) : ntiles;
   for (uint32_t k = 0; k < count; ++k) {
-    if (b->tile == 128 && b->units == 1)
-      hipLaunchKernelGGL((lg_step_batch_kernel<128, 1>), dim3(grid), dim3(128), 0, b->stream,
-                         b->desc.p, tiles, ntiles, k);
-    else if (b->tile == 128)
-      hipLaunchKernelGGL((lg_step_batch_kernel<128, 2>), dim3(grid), dim3(128), 0, b->stream,
-                         b->desc.p, tiles, ntiles, k);
-    else if (b->units == 1)
-      hipLaunchKernelGGL((lg_step_batch_kernel<256, 1>), dim3(grid), dim3(256), 0, b->stream,
-                         b->desc.p, tiles, ntiles, k);
+    if (b->tile == 128)
+      launch_steps<128>(b, grid, tiles, ntiles, k, k == 0);
     else
-      hipLaunchKernelGGL((lg_step_batch_kernel<256, 2>), dim3(grid), dim3(256), 0, b->stream,
-                         b->desc.p, tiles, ntiles, k);
+      launch_steps<256>(b, grid, tiles, ntiles, k, k == 0);
   }
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(lg_bump_kernel, dim3(blocks_for(nm)), dim3(256), 0, b->stream, b->desc.p, nm,
-                     count);
+                     count, b->passes == 1 ? 1u : 0u);
   HIP_TRY(hipGetLastError());
   return LG_OK;
 }
 
 int ensure_tiles(lg_batch* b) {
-  if (b->tiles_units == b->units) return LG_OK;
+  if (b->tiles_key == pipeline_key(b)) return LG_OK;
+  HIP_TRY(hipStreamSynchronize(b->stream));  // the old tables may still be read
   int rc = build_tiles<128>(b, b->tiles128, b->ntiles128, b->host_tiles128);
   if (rc) return rc;
   rc = build_tiles<256>(b, b->tiles256, b->ntiles256, b->host_tiles256);
   if (rc) return rc;
-  b->tiles_units = b->units;
+  b->tiles_key = pipeline_key(b);
   return LG_OK;
 }
 
 int ensure_graph(lg_batch* b) {
   if (b->exec && b->graph_chunk == b->chunk && b->graph_tile == b->tile &&
-      b->graph_grid == b->grid && b->graph_units == b->units)
+      b->graph_grid == b->grid && b->graph_key == pipeline_key(b))
     return LG_OK;
   drop_graph(b);
   HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -276,7 +403,7 @@ int ensure_graph(lg_batch* b) {
   b->graph_chunk = b->chunk;
   b->graph_tile = b->tile;
   b->graph_grid = b->grid;
-  b->graph_units = b->units;
+  b->graph_key = pipeline_key(b);
   return LG_OK;
 }
 
@@ -341,7 +468,7 @@ lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
   if (hipGetDeviceProperties(&p, members[0]->device) == hipSuccess) b->ncu = p.multiProcessorCount;
   b->kmax = ~0u;
   for (lg_t* g : b->members) {
-    b->kmax = std::min(b->kmax, g->basis_steps);
+    b->kmax = std::min(b->kmax, g->basis_steps - 1);  // one table spare for the two-pass step
     uint32_t per = g->G.nbands + 1;
     b->max_tab_blocks_per_step = std::max(b->max_tab_blocks_per_step, blocks_for((size_t)per * 16));
   }
@@ -419,6 +546,31 @@ int lg_batch_set_units(lg_batch_t* b, uint32_t cells_per_lane) {
   return LG_OK;
 }
 
+int lg_batch_set_passes(lg_batch_t* b, uint32_t passes) {
+  if (!b) return fail(LG_EINVAL, "null batch");
+  if (passes != 1 && passes != 2) return fail(LG_EINVAL, "passes per step must be 1 or 2");
+  b->passes = passes;
+  return LG_OK;
+}
+
+int lg_batch_set_draw_unit(lg_batch_t* b, uint32_t cells_per_lane) {
+  if (!b) return fail(LG_EINVAL, "null batch");
+  if (cells_per_lane != 64 && cells_per_lane != 32)
+    return fail(LG_EINVAL, "cells per lane must be 64 or 32");
+  b->draw_units = cells_per_lane == 64 ? 1u : 2u;
+  return LG_OK;
+}
+
+int lg_batch_get_pipeline(lg_batch_t* b, uint32_t* passes, uint32_t* cells_per_lane,
+                          uint32_t* draw_cells_per_lane) {
+  if (!b || !passes || !cells_per_lane || !draw_cells_per_lane)
+    return fail(LG_EINVAL, "null argument");
+  *passes = b->passes;
+  *cells_per_lane = b->units == 1 ? 64u : 32u;
+  *draw_cells_per_lane = b->draw_units == 1 ? 64u : 32u;
+  return LG_OK;
+}
+
 int lg_batch_set_shape(lg_batch_t* b, uint32_t tile, uint32_t grid, uint32_t chunk) {
   if (!b) return fail(LG_EINVAL, "null batch");
   if (tile != 128 && tile != 256) return fail(LG_EINVAL, "tile must be 128 or 256 words");
@@ -451,34 +603,34 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
   int rc = upload_desc(b, nullptr, ~0ull);
   if (rc) return rc;
   struct Cand {
-    uint32_t tile, grid, chunk, units;
+    uint32_t passes, units, draw_units, tile, grid;
   };
-  std::vector<Cand> cands;
-  for (uint32_t units : {1u, 2u})
-    for (uint32_t tile : {256u, 128u})
-      for (uint32_t m : {0u, 4u, 16u})
-        for (uint32_t chunk : {8u, 32u}) {
-          uint32_t c = std::min(chunk, b->kmax & ~1u);
-          if (c == 0) continue;
-          cands.push_back({tile, m ? m * (uint32_t)b->ncu : 0u, c, units});
-        }
+  // most promising first: a budget that runs out keeps the best so far
+  const uint32_t g16 = 16u * (uint32_t)b->ncu;
+  const Cand cands[] = {
+      {2, 1, 1, 256, 0},   {2, 1, 1, 128, 0},   {2, 1, 2, 256, 0}, {1, 1, 1, 256, 0},
+      {2, 2, 2, 256, 0},   {2, 1, 1, 256, g16}, {1, 2, 1, 256, 0}, {1, 1, 1, 128, 0},
+      {2, 2, 1, 256, 0},   {1, 1, 1, 256, g16},
+  };
+  const uint32_t chunk = std::max<uint32_t>(2, std::min<uint32_t>(32, b->kmax & ~1u));
   double best = 1e300;
-  Cand bestc = {b->tile, b->grid, b->chunk, b->units};
+  Cand bestc = {b->passes, b->units, b->draw_units, b->tile, b->grid};
   uint64_t used = 0;
   for (const Cand& c : cands) {
-    uint64_t trial = 2ull * c.chunk;
+    const uint64_t trial = chunk;  // one warm replay (captures the graph), one timed
     if (budget_steps && used + 2 * trial > budget_steps) break;
+    b->passes = c.passes;
+    b->units = c.units;
+    b->draw_units = c.draw_units;
     b->tile = c.tile;
     b->grid = c.grid;
-    b->chunk = c.chunk;
-    b->units = c.units;
-    if ((rc = run_steps(b, trial, true))) return rc;  // warm (captures the graph)
+    b->chunk = chunk;
+    if ((rc = run_steps(b, trial, true))) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));
     auto t0 = std::chrono::steady_clock::now();
     if ((rc = run_steps(b, trial, true))) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));
-    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() /
-                (double)trial;
+    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     used += 2 * trial;
     if (dt < best) {
       best = dt;
@@ -486,10 +638,12 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
     }
   }
   commit_members(b, nullptr, used);
+  b->passes = bestc.passes;
+  b->units = bestc.units;
+  b->draw_units = bestc.draw_units;
   b->tile = bestc.tile;
   b->grid = bestc.grid;
-  b->chunk = bestc.chunk;
-  b->units = bestc.units;
+  b->chunk = chunk;
   *used_steps = used;
   return LG_OK;
 }

@@ -459,10 +459,12 @@ int sample_now(lg_t* g, float alpha) {
   return LG_OK;
 }
 
+}  // namespace
+
 // Digit and inflow planes of the two-pass step: two slots of one step each
 // (the step being collided, the step being drawn), sized for the current
 // inflow width.
-int ensure_draw_buffers(lg_t* g) {
+int lgca_internal::ensure_draw_buffers(lg_t* g) {
   const size_t nhalf = (size_t)2 * g->G.total, nih = (size_t)2 * g->G.iw * g->G.h;
   if (g->kbuf.n >= 2 * LG_KDIGITS * nhalf && g->ibuf.n >= std::max<size_t>(2 * 7 * nih, 1))
     return LG_OK;
@@ -471,6 +473,8 @@ int ensure_draw_buffers(lg_t* g) {
   HIP_TRY(g->ibuf.alloc(std::max<size_t>(2 * 7 * nih, 1)));
   return LG_OK;
 }
+
+namespace {
 
 // One two-pass launch: collide the current step from digit slot `cur` if
 // `collide`, draw into slot cur ^ 1 from tabs if `draw`.

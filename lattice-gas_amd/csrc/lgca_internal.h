@@ -59,6 +59,9 @@ void equilibrium_thresholds(float density, float ux, float uy, bool rest, uint32
 // layout or inflow width changed since they were last built.
 int ensure_tables(lg_lattice* g);
 
+// Allocate the two digit/inflow slots of the two-pass step if needed.
+int ensure_draw_buffers(lg_lattice* g);
+
 }  // namespace lgca_internal
 
 // The handle behind lg_t: owns every device allocation of one lattice.

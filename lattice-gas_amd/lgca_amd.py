@@ -63,6 +63,9 @@ def load_library(path: str | None = None):
         "lg_set_draw_unit": (i32, [vp, C.c_uint32]),
         "lg_batch_set_units": (i32, [vp, C.c_uint32]),
         "lg_batch_get_units": (i32, [vp, P(C.c_uint32)]),
+        "lg_batch_set_passes": (i32, [vp, C.c_uint32]),
+        "lg_batch_set_draw_unit": (i32, [vp, C.c_uint32]),
+        "lg_batch_get_pipeline": (i32, [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "lg_advance_timed": (i32, [vp, u64, P(f32)]),
         "lg_advance_sampling_timed": (i32, [vp, u64, u64, f32, P(f32)]),
         "lg_debug_collide_table": (i32, [i32, P(C.c_uint8), sz]),
@@ -317,6 +320,18 @@ class Batch:
         u = C.c_uint32()
         _check(load_library().lg_batch_get_units(self._p, C.byref(u)))
         return u.value
+
+    def set_passes(self, passes: int):
+        _check(load_library().lg_batch_set_passes(self._p, int(passes)))
+
+    def set_draw_unit(self, cells_per_lane: int):
+        _check(load_library().lg_batch_set_draw_unit(self._p, int(cells_per_lane)))
+
+    def pipeline(self):
+        """(kernels per step, collide cells per lane, draw cells per lane)."""
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(load_library().lg_batch_get_pipeline(self._p, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def shape(self):
         t, g, c = C.c_uint32(), C.c_uint32(), C.c_uint32()

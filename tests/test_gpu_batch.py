@@ -32,16 +32,23 @@ def build_all():
     return [member(*s) for s in SPECS]
 
 
-@pytest.mark.parametrize("units", [64, 32])
+# (kernels per step, collide cells per lane, draw cells per lane)
+PIPELINES = [(2, 64, 64), (2, 32, 32), (2, 64, 32), (2, 32, 64), (1, 64, 64), (1, 32, 64)]
+
+
+@pytest.mark.parametrize("pipeline", PIPELINES)
 @pytest.mark.parametrize("shape", [None, (128, 0, 8), (256, 1024, 2), (128, 4096, 16)])
-def test_batch_equals_solo_runs(shape, units):
-    print(f"batch shape={shape} cells/lane={units}", flush=True)
+def test_batch_equals_solo_runs(shape, pipeline):
+    print(f"batch shape={shape} pipeline={pipeline}", flush=True)
     solo = build_all()
     for g in solo:
         g.advance(45)
     batched = build_all()
     b = Batch(batched)
-    b.set_units(units)
+    b.set_passes(pipeline[0])
+    b.set_units(pipeline[1])
+    b.set_draw_unit(pipeline[2])
+    assert b.pipeline() == pipeline
     if shape:
         b.set_shape(*shape)
     b.advance(20)          # graph replays + remainder
@@ -74,7 +81,7 @@ def test_autotune_keeps_results_exact():
     ref = build_all()
     bat = build_all()
     b = Batch(bat)
-    used = b.autotune(budget_steps=400)
+    used = b.autotune(budget_steps=0)  # every candidate: pipelines change within the call
     assert used > 0 and b.shape()[0] in (128, 256)
     b.advance(30)
     for r in ref:
@@ -104,3 +111,20 @@ def test_limited_advance_and_batched_snapshot():
             assert np.array_equal(g.store(), r.store())
             for x, y in zip(snap, r.read_field()):
                 assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+def test_pipeline_changes_between_calls():
+    """Fused and two-pass calls interleaved on one batch (each call starts
+    its own digit stream) give the members' own runs."""
+    print("pipeline changes", flush=True)
+    solo = build_all()
+    for g in solo:
+        g.advance(5 + 8 + 3 + 10)
+    bat = build_all()
+    b = Batch(bat)
+    for passes, n in ((2, 5), (1, 8), (2, 3), (1, 10)):
+        b.set_passes(passes)
+        b.advance(n)
+    for s_, g in zip(solo, bat):
+        assert g.steps == s_.steps
+        assert np.array_equal(g.store(), s_.store())
