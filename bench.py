@@ -176,6 +176,32 @@ def ensemble_leg(n, steps, bands, seed=4):
             "plain_loop_sample": f"first {probe} lattices, {steps} steps each"}
 
 
+def shear_leg(steps, bands, passes, units, dunits, ahead):
+    """The transport-measure workload at full size (32768 x 16384, 537M cells,
+    periodic, density 0.3, shear wave from seed 7): step rate and algorithmic
+    HBM rate of the step kernels, with the lattice (940 MB of planes, beyond
+    the 256 MB MALL) resident before the timed region."""
+    import workloads as W
+
+    cfg = W.config3(bands=bands)
+    g = W.config_gpu(cfg)
+    g.set_work_unit(units)
+    g.set_passes(passes)
+    g.set_draw_unit(dunits)
+    g.set_draw_ahead(ahead)
+    g.advance(2)  # warm-up (builds the tables, first launches)
+    t0 = time.perf_counter()
+    dev_ms = g.advance_timed(steps)
+    wall = time.perf_counter() - t0
+    cells = cfg["w"] * cfg["h"]
+    g.close()
+    step_ms = dev_ms / steps
+    ach = cells * BYTES_PER_CELL_UPDATE / (step_ms * 1e-3) / 1e9
+    return {"workload": cfg["name"], "cells": cells, "steps": steps,
+            "gcups": round(cells * steps / wall / 1e9, 3), "mean_step_ms": round(step_ms, 4),
+            "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+
+
 def serving_leg(n, rate, seed=5):
     """Served stream: Poisson arrivals, log-uniform sizes 256x256..4096x2304,
     steps 500-20000, random plate and drift, HBM budget 200 GB, 32x32-block
@@ -202,6 +228,9 @@ def main():
                     help="kernels per step: draw pass + collide pass (2) or one fused kernel (1)")
     ap.add_argument("--draw-cells-per-lane", type=int, default=64, choices=(64, 32),
                     help="work unit of the two-pass step's draw work")
+    ap.add_argument("--draw-ahead", type=int, default=0, metavar="N",
+                    help="two-pass step: draws of up to N steps per launch on a second stream "
+                         "(0: draws share the collide launches)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--ensemble", type=int, default=256, metavar="N",
@@ -210,6 +239,8 @@ def main():
     ap.add_argument("--serve", type=int, default=64, metavar="N",
                     help="requests in the serving leg (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--shear-steps", type=int, default=400, metavar="N",
+                    help="steps of the 537M-cell shear-wave leg (0 skips it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB megabytes (to scale rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE) and exit")
@@ -246,6 +277,7 @@ def main():
     g.set_work_unit(args.cells_per_lane)
     g.set_passes(args.passes)
     g.set_draw_unit(args.draw_cells_per_lane)
+    g.set_draw_ahead(args.draw_ahead)
     g.attach_field(cfg["block"], cfg["block"])
     if args.warmup:
         g.advance_sampling(args.warmup, every, 1.0)
@@ -260,8 +292,13 @@ def main():
     traffic_alg = cells * BYTES_PER_CELL_UPDATE
     achieved = traffic_alg / (step_ms * 1e-3) / 1e9
     unit = "h" if args.cells_per_lane == 32 else "w"
-    kernel = (f"lg_step_kernel_{unit}" if args.passes == 1 else
-              f"lg_step2_kernel<{3 - args.cells_per_lane // 32},{3 - args.draw_cells_per_lane // 32}>")
+    nh, dnh = 3 - args.cells_per_lane // 32, 3 - args.draw_cells_per_lane // 32
+    if args.passes == 1:
+        kernel = f"lg_step_kernel_{unit}"
+    elif args.draw_ahead:
+        kernel = f"lg_collide_kernel<{nh}>+lg_draw_kernel<{dnh}>"
+    else:
+        kernel = f"lg_step2_kernel<{nh},{dnh}>"
 
     out = {
         "metric": METRIC,
@@ -280,7 +317,8 @@ def main():
                    "rest": True, "bands": bands, "inlet_cols": cfg["inlet"],
                    "sample_every": every, "block": cfg["block"], "seed": cfg["seed"],
                    "cells_per_lane": args.cells_per_lane, "passes": args.passes,
-                   "draw_cells_per_lane": args.draw_cells_per_lane if args.passes == 2 else None},
+                   "draw_cells_per_lane": args.draw_cells_per_lane if args.passes == 2 else None,
+                   "draw_ahead": args.draw_ahead if args.passes == 2 else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(kernel, cfg["name"]),
@@ -292,6 +330,12 @@ def main():
         gpu_cells = g.store()
         out["cpu_baseline"] = cpu_baseline(cfg, total, gpu_cells, args.cpu_budget)
     g.close()
+    if args.shear_steps:
+        try:
+            out["shear_537m"] = shear_leg(args.shear_steps, bands, args.passes, args.cells_per_lane,
+                                          args.draw_cells_per_lane, args.draw_ahead)
+        except Exception as e:
+            out["shear_537m"] = {"error": repr(e)}
     if args.ensemble:
         try:
             out["ensemble"] = ensemble_leg(args.ensemble, args.ensemble_steps, bands)

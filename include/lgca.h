@@ -120,6 +120,13 @@ int lg_set_work_unit(lg_t* g, uint32_t cells_per_lane);
  * one step into each lane. The results are identical either way. */
 int lg_set_passes(lg_t* g, uint32_t passes);
 
+/* Draw-ahead for the two-pass step: 0 runs each step's draw work in the
+ * launch that collides the step before it; N > 0 (at most 64) runs the draws
+ * of up to N steps per launch on a second HIP stream, double-buffered ahead of
+ * the one-collide-per-step launches on the handle's stream, so the draws fill
+ * the gaps of the serial collide chain. Results are identical either way. */
+int lg_set_draw_ahead(lg_t* g, uint32_t steps);
+
 /* Cells per lane (64, the default, or 32) of the two-pass step's draw work,
  * as lg_set_work_unit sets them for its collide work. Results are identical. */
 int lg_set_draw_unit(lg_t* g, uint32_t cells_per_lane);

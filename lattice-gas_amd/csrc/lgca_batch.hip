@@ -101,8 +101,11 @@ __global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) 
 // collide tile runs units of step k from digit slot kpar + k, a draw tile
 // draws units of step k + 1 into the other slot, both exactly as
 // lg_step2_kernel maps them for a single lattice.
+// Register target as the single-lattice kernel's (about 100 VGPRs); left to
+// itself the compiler unrolls the tile loop into 225 VGPRs for whole-word
+// units, 2 waves per SIMD.
 template <int TILE, int NH, int DNH>
-__global__ void __launch_bounds__(TILE)
+__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(5)))
     lg_step2_batch_kernel(const LgDesc* __restrict D, const uint2* __restrict tiles,
                           uint32_t ntiles, uint32_t k) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {

@@ -229,7 +229,8 @@ LG_HD inline uint64_t lg_draw_half(uint64_t st, uint32_t stride, uint32_t P[NK])
     LG_UNROLL
     for (int m = 0; m < 8; ++m) {
       const uint32_t sel = 0x01010101u << m;
-      acc |= R[m] & (b ? ((sel << b) | (sel >> (32 - b))) : sel);
+      // (R & mask) | acc as one three-input op (0xEA)
+      acc = lg_op3<0xEA>(R[m], b ? ((sel << b) | (sel >> (32 - b))) : sel, acc);
     }
     P[b] = b ? ((acc >> b) | (acc << (32 - b))) : acc;
   }

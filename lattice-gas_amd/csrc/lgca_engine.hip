@@ -131,6 +131,39 @@ lg_step2_kernel(LgGeom G, const uint64_t* __restrict src, uint64_t* __restrict d
                         NH == 1 ? 0u : (u & 1u));
   }
 }
+// Draw-ahead pipeline, draw side: step blockIdx.y of this launch for draw
+// unit u (word order[u], DNH = 1, or half u % 2 of word order[u / 2]);
+// tabs0, kb0, ib0: the launch's first step's tables and slots.
+template <int DNH>
+__global__ void __launch_bounds__(256)
+lg_draw_kernel(LgGeom G, const uint64_t* __restrict solid, const uint64_t* __restrict cjump,
+               const uint32_t* __restrict row_band, const uint64_t* __restrict ijump,
+               const uint32_t* __restrict order, const uint64_t* __restrict tabs0, uint32_t per,
+               uint32_t* __restrict kb0, uint32_t* __restrict ib0) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= G.total * DNH) return;
+  const size_t kstride = 2 * (size_t)G.total, istride = 2 * (size_t)G.iw * G.h;
+  const uint32_t s = blockIdx.y;
+  const uint32_t e = order[DNH == 1 ? u : u >> 1];
+  lg_draw_unit<DNH>(G, solid, cjump, row_band, ijump, tabs0 + (size_t)s * per * LG_WTAB, e >> 16,
+                    e & 0xFFFFu, DNH == 1 ? 0u : (u & 1u), kb0 + (size_t)s * LG_KDIGITS * kstride,
+                    kstride, ib0 + (size_t)s * 7 * istride, istride);
+}
+
+// Draw-ahead pipeline, collide side: one step from its digit and inflow slot.
+template <int NH>
+__global__ void __launch_bounds__(256)
+lg_collide_kernel(LgGeom G, const uint64_t* __restrict src, uint64_t* __restrict dst,
+                  const uint64_t* __restrict solid, const uint32_t* __restrict kc,
+                  const uint32_t* __restrict ic) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= G.total * NH) return;
+  const uint32_t word = NH == 1 ? u : u >> 1;
+  const uint32_t y = word / G.wpr;
+  lg_collide_unit<NH>(G, src, dst, solid, kc, 2 * (size_t)G.total, ic, 2 * (size_t)G.iw * G.h, y,
+                      word - y * G.wpr, NH == 1 ? 0u : (u & 1u));
+}
+
 // Bytes -> planes (GpuLattice::load, src/gpu.rs:572-610), one thread per word.
 __global__ void __launch_bounds__(256) lg_pack_kernel(LgGeom G, const uint8_t* __restrict cells,
                                                       uint64_t* __restrict planes,
@@ -464,13 +497,15 @@ int sample_now(lg_t* g, float alpha) {
 // Digit and inflow planes of the two-pass step: two slots of one step each
 // (the step being collided, the step being drawn), sized for the current
 // inflow width.
+// With draw-ahead: two groups of `ahead` steps each.
 int lgca_internal::ensure_draw_buffers(lg_t* g) {
   const size_t nhalf = (size_t)2 * g->G.total, nih = (size_t)2 * g->G.iw * g->G.h;
-  if (g->kbuf.n >= 2 * LG_KDIGITS * nhalf && g->ibuf.n >= std::max<size_t>(2 * 7 * nih, 1))
+  const size_t slots = 2 * (size_t)std::max<uint32_t>(1, g->ahead);
+  if (g->kbuf.n >= slots * LG_KDIGITS * nhalf && g->ibuf.n >= std::max<size_t>(slots * 7 * nih, 1))
     return LG_OK;
   HIP_TRY(hipStreamSynchronize(g->stream));
-  HIP_TRY(g->kbuf.alloc(2 * LG_KDIGITS * nhalf));
-  HIP_TRY(g->ibuf.alloc(std::max<size_t>(2 * 7 * nih, 1)));
+  HIP_TRY(g->kbuf.alloc(slots * LG_KDIGITS * nhalf));
+  HIP_TRY(g->ibuf.alloc(std::max<size_t>(slots * 7 * nih, 1)));
   return LG_OK;
 }
 
@@ -493,6 +528,54 @@ void launch_step2(lg_t* g, uint32_t cur, bool collide, bool draw, const uint64_t
                      kn, in, cblocks, dblocks);
 }
 
+// The second stream and the events of the draw-ahead pipeline (once).
+int ensure_ahead_stream(lg_t* g) {
+  if (g->dstream) return LG_OK;
+  HIP_TRY(hipStreamCreateWithFlags(&g->dstream, hipStreamNonBlocking));
+  for (int q = 0; q < 2; ++q) {
+    HIP_TRY(hipEventCreateWithFlags(&g->ev_drawn[q], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&g->ev_used[q], hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(g->ev_used[q], g->stream));  // both groups start free
+  }
+  HIP_TRY(hipEventCreate(&g->ev_start));
+  return LG_OK;
+}
+
+// Draw-ahead: launch the draws of steps [s0, s0 + nd) of the chunk whose
+// tables are in basis into the next slot group on dstream, then their
+// collides on the main stream.
+void enqueue_ahead_group(lg_t* g, uint32_t s0, uint32_t nd) {
+  const uint32_t per = g->G.nbands + 1;
+  const size_t nhalf = (size_t)2 * g->G.total, nih = (size_t)2 * g->G.iw * g->G.h;
+  const uint32_t q = (uint32_t)(g->dgroup++ & 1u);
+  uint32_t* kb = g->kbuf.p + (size_t)q * g->ahead * LG_KDIGITS * nhalf;
+  uint32_t* ib = g->ibuf.p + (size_t)q * g->ahead * 7 * nih;
+  (void)hipStreamWaitEvent(g->dstream, g->ev_used[q], 0);
+  const dim3 dg(blocks_for((size_t)g->G.total * g->draw_units), nd);
+  const uint64_t* tabs0 = g->basis.p + (size_t)s0 * per * LG_WTAB;
+  if (g->draw_units == 2)
+    hipLaunchKernelGGL(lg_draw_kernel<2>, dg, dim3(256), 0, g->dstream, g->G, g->solid.p,
+                       g->cjump.p, g->row_band.p, g->ijump.p, g->order.p, tabs0, per, kb, ib);
+  else
+    hipLaunchKernelGGL(lg_draw_kernel<1>, dg, dim3(256), 0, g->dstream, g->G, g->solid.p,
+                       g->cjump.p, g->row_band.p, g->ijump.p, g->order.p, tabs0, per, kb, ib);
+  (void)hipEventRecord(g->ev_drawn[q], g->dstream);
+  (void)hipStreamWaitEvent(g->stream, g->ev_drawn[q], 0);
+  for (uint32_t k = 0; k < nd; ++k) {
+    const uint32_t* kc = kb + (size_t)k * LG_KDIGITS * nhalf;
+    const uint32_t* ic = ib + (size_t)k * 7 * nih;
+    if (g->units == 2)
+      hipLaunchKernelGGL(lg_collide_kernel<2>, dim3(blocks_for(nhalf)), dim3(256), 0, g->stream,
+                         g->G, g->a.p, g->b.p, g->solid.p, kc, ic);
+    else
+      hipLaunchKernelGGL(lg_collide_kernel<1>, dim3(blocks_for(g->G.total)), dim3(256), 0,
+                         g->stream, g->G, g->a.p, g->b.p, g->solid.p, kc, ic);
+    std::swap(g->a.p, g->b.p);
+    g->steps += 1;
+  }
+  (void)hipEventRecord(g->ev_used[q], g->stream);
+}
+
 // Enqueue `n` steps (and samples every `every`), no synchronisation.
 int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) {
   int trc = ensure_tables(g);
@@ -500,15 +583,37 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
   uint64_t left = n, since = 0;
   const uint32_t per = g->G.nbands + 1;
   if (timed) g->tev_used = 0;
+  const bool ahead = g->passes == 2 && g->ahead > 0;
+  if (ahead) {
+    int rc = ensure_ahead_stream(g);
+    if (!rc) rc = ensure_draw_buffers(g);
+    if (rc) return rc;
+    // the draw stream starts with this call's work on the main stream
+    HIP_TRY(hipEventRecord(g->ev_start, g->stream));
+    HIP_TRY(hipStreamWaitEvent(g->dstream, g->ev_start, 0));
+  }
   while (left > 0) {
-    const uint32_t extra = g->passes == 2 ? 1u : 0u;  // see the two-pass branch below
+    const uint32_t extra = (g->passes == 2 && !ahead) ? 1u : 0u;  // see the two-pass branch
     uint64_t chunk = std::min<uint64_t>(left, g->basis_steps - extra);
     if (every > 0) chunk = std::min<uint64_t>(chunk, every - since);
     const uint32_t ntab = (uint32_t)chunk + extra;
     hipLaunchKernelGGL(lg_tables_kernel, dim3(blocks_for((size_t)ntab * per * 16)), dim3(256), 0,
-                       g->stream, g->seed, g->steps, ntab, g->G.nbands, g->basis.p);
+                       ahead ? g->dstream : g->stream, g->seed, g->steps, ntab, g->G.nbands,
+                       g->basis.p);
     HIP_TRY(hipGetLastError());
-    if (timed) {
+    if (timed && ahead) {
+      // one interval for the whole call: the draw stream runs ahead of the
+      // main stream's step boundaries
+      if (g->tev_used == 0) {
+        while (g->tev.size() < 2) {
+          hipEvent_t e;
+          HIP_TRY(hipEventCreate(&e));
+          g->tev.push_back(e);
+        }
+        HIP_TRY(hipEventRecord(g->tev[0], g->stream));
+        g->tev_used = 1;
+      }
+    } else if (timed) {
       while (g->tev.size() < 2 * (g->tev_used + 1)) {
         hipEvent_t e;
         HIP_TRY(hipEventCreate(&e));
@@ -516,7 +621,11 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
       }
       HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used], g->stream));
     }
-    if (g->passes == 1) {
+    if (ahead) {
+      for (uint64_t s0 = 0; s0 < chunk; s0 += g->ahead)
+        enqueue_ahead_group(g, (uint32_t)s0,
+                            (uint32_t)std::min<uint64_t>(g->ahead, chunk - s0));
+    } else if (g->passes == 1) {
       for (uint64_t k = 0; k < chunk; ++k) {
         if (g->units == 2)
           hipLaunchKernelGGL(lg_step_kernel_h, dim3(blocks_for((size_t)g->G.total * 2)), dim3(256),
@@ -549,7 +658,11 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
       }
     }
     HIP_TRY(hipGetLastError());
-    if (timed) HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used++ + 1], g->stream));
+    if (timed && ahead) {
+      if (left == chunk) HIP_TRY(hipEventRecord(g->tev[1], g->stream));  // the call's last chunk
+    } else if (timed) {
+      HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used++ + 1], g->stream));
+    }
     left -= chunk;
     since += chunk;
     if (every > 0 && (since >= every || left == 0)) {
@@ -637,6 +750,7 @@ lg_t* lg_create(size_t w, size_t h, int rest, uint64_t seed, size_t bands) {
 void lg_destroy(lg_t* g) {
   if (!g) return;
   (void)hipStreamSynchronize(g->stream);
+  if (g->dstream) (void)hipStreamSynchronize(g->dstream);
   g->a.release();
   g->b.release();
   g->solid.release();
@@ -653,6 +767,12 @@ void lg_destroy(lg_t* g) {
   g->fsums.release();
   g->counter.release();
   for (hipEvent_t e : g->tev) (void)hipEventDestroy(e);
+  for (int q = 0; q < 2; ++q) {
+    if (g->ev_drawn[q]) (void)hipEventDestroy(g->ev_drawn[q]);
+    if (g->ev_used[q]) (void)hipEventDestroy(g->ev_used[q]);
+  }
+  if (g->ev_start) (void)hipEventDestroy(g->ev_start);
+  if (g->dstream) (void)hipStreamDestroy(g->dstream);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
 }
@@ -850,6 +970,14 @@ int lg_set_work_unit(lg_t* g, uint32_t cells_per_lane) {
   if (cells_per_lane != 64 && cells_per_lane != 32)
     return fail(LG_EINVAL, "cells per lane must be 64 or 32");
   g->units = cells_per_lane == 64 ? 1u : 2u;
+  return LG_OK;
+}
+
+int lg_set_draw_ahead(lg_t* g, uint32_t steps) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  if (steps > 64) return fail(LG_EINVAL, "draw-ahead is at most 64 steps");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->ahead = steps;
   return LG_OK;
 }
 

@@ -90,6 +90,14 @@ struct lg_lattice {
   uint32_t passes = 2;
   uint32_t draw_units = 1;  // draw lanes of the two-pass step: 1 = 64 cells, 2 = 32
   uint32_t kslot = 0;
+  // Draw-ahead (ahead > 0): the draw work runs on `dstream` in launches of up
+  // to `ahead` steps into two groups of slots; `stream` runs one collide
+  // launch per step. ev_drawn[q] / ev_used[q]: group q drawn / collided.
+  uint32_t ahead = 0;
+  hipStream_t dstream = nullptr;
+  hipEvent_t ev_drawn[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
+  hipEvent_t ev_start = nullptr;
+  uint64_t dgroup = 0;
   DevBuf<uint32_t> kbuf, ibuf;
   DevBuf<uint8_t> bytes;
   uint32_t basis_steps = 0;  // capacity of `basis` in steps

@@ -61,6 +61,7 @@ def load_library(path: str | None = None):
         "lg_set_work_unit": (i32, [vp, C.c_uint32]),
         "lg_set_passes": (i32, [vp, C.c_uint32]),
         "lg_set_draw_unit": (i32, [vp, C.c_uint32]),
+        "lg_set_draw_ahead": (i32, [vp, C.c_uint32]),
         "lg_batch_set_units": (i32, [vp, C.c_uint32]),
         "lg_batch_get_units": (i32, [vp, P(C.c_uint32)]),
         "lg_batch_set_passes": (i32, [vp, C.c_uint32]),
@@ -214,6 +215,11 @@ class GpuLattice:
     def set_draw_unit(self, cells_per_lane: int):
         """64 or 32 cells per lane for the draw work of the two-pass step."""
         _check(load_library().lg_set_draw_unit(self._p, int(cells_per_lane)))
+
+    def set_draw_ahead(self, steps: int):
+        """0: draws share the collide launches; N: draws of up to N steps per
+        launch on a second stream, ahead of the collides."""
+        _check(load_library().lg_set_draw_ahead(self._p, int(steps)))
 
     def advance_timed(self, n: int, sample_every: int = 0, alpha: float = 0.0) -> float:
         """Advance and return the device milliseconds of the step kernels,

@@ -86,6 +86,7 @@ def _compare_run(oracle, w, h, rest, bands, inlet, seed, steps, density=0.22, in
     g.set_work_unit(cells_per_lane)
     g.set_passes(passes[0])
     g.set_draw_unit(passes[1])
+    g.set_draw_ahead(passes[2] if len(passes) > 2 else 0)
     g.set_inlet(inlet, density, inflow)
     g.load(lat.cells)
     done = 0
@@ -103,8 +104,8 @@ def _compare_run(oracle, w, h, rest, bands, inlet, seed, steps, density=0.22, in
     return lat, g
 
 
-# (kernels per step, draw cells per lane)
-PIPELINES = [(1, 32), (2, 32), (2, 64)]
+# (kernels per step, draw cells per lane[, draw-ahead steps])
+PIPELINES = [(1, 32), (2, 32), (2, 64), (2, 64, 4), (2, 32, 3)]
 
 
 @pytest.mark.parametrize("w,h,rest,bands,inlet", [
@@ -279,7 +280,36 @@ def test_config2_size_matches_the_cpu_replay(oracle, cells_per_lane, passes):
     g.set_work_unit(cells_per_lane)
     g.set_passes(passes[0])
     g.set_draw_unit(passes[1])
+    g.set_draw_ahead(passes[2] if len(passes) > 2 else 0)
     for _ in range(3):
         lat.advance(20)
         g.advance(20)
         assert np.array_equal(g.store(), lat.cells)
+
+
+@pytest.mark.parametrize("ahead", [1, 2, 8, 64])
+def test_draw_ahead_sampling_and_mode_changes(oracle, ahead):
+    """Draw-ahead across chunk and sampling boundaries (every 7 steps), with
+    the pipeline switched between calls, against the oracle's cells and
+    exact block sums."""
+    say(f"draw-ahead {ahead}")
+    w, h = 320, 96
+    lat = oracle.Lattice(w, h, 0.25, (0.2, 0.0), 8, True, 5, 0xABCD)
+    lat.add_solid_box(100.0, 104.0, 30.0, 60.0)
+    lat.init_equilibrium()
+    g = GpuLattice(w, h, True, 0xABCD, 5)
+    g.set_inlet(8, 0.25, (0.2, 0.0))
+    g.load(lat.cells)
+    g.attach_field(16, 16)
+    field = oracle.Field(w, h, 16, 16)
+    for rep, (mode, n) in enumerate(((ahead, 45), (0, 13), (ahead, 30), (ahead, 1))):
+        g.set_draw_ahead(mode)
+        g.advance_sampling(n, 7, 1.0)
+        done = 0
+        while done < n:
+            k = min(7, n - done)
+            lat.advance(k)
+            done += k
+        field.sample(lat, 1.0)
+        assert np.array_equal(g.store(), lat.cells), f"call {rep}"
+        assert [list(x) for x in g.read_block_sums()] == [list(x) for x in field.block_sums(lat)]
