@@ -18,6 +18,12 @@ from HIP event pairs on the engine's stream around each run of step launches.
 `cpu_baseline` is the CPU oracle (a restatement of the reference's CPU path,
 one thread per row band, spawned per step as the reference does) replaying
 the same run on this host; its final cells are compared with the GPU's.
+
+Further objects in the same line: `small_512x256` (the reference's 512 x 256
+CPU-runnable case on the GPU; its CPU rates and check sit under
+`cpu_baseline`), `shear_537m` (the 32768 x 16384 transport workload's step
+rate), `ensemble` (256 uneven lattices, batched vs one at a time) and
+`serving` (a Poisson request stream, continuous batching vs the plain loop).
 """
 from __future__ import annotations
 
@@ -176,6 +182,54 @@ def ensemble_leg(n, steps, bands, seed=4):
             "plain_loop_sample": f"first {probe} lattices, {steps} steps each"}
 
 
+def small_leg(bands, passes, units, dunits, ahead):
+    """The reference's CPU-runnable case (512 x 256, plate 64 rows at x = 128,
+    2000 steps, 16 x 16 block sums) on the GPU: its rate, and its final cells
+    and exact block sums for the CPU baseline to check."""
+    import workloads as W
+
+    cfg = W.config1(bands=bands)
+    cells = cfg["w"] * cfg["h"]
+    g = W.config_gpu(cfg)
+    g.set_work_unit(units)
+    g.set_passes(passes)
+    g.set_draw_unit(dunits)
+    g.set_draw_ahead(ahead)
+    g.attach_field(cfg["block"], cfg["block"])
+    t0 = time.perf_counter()
+    g.advance_sampling(cfg["steps"], cfg["steps"], 1.0)
+    wall = time.perf_counter() - t0
+    final = (g.store(), g.read_block_sums())
+    g.close()
+    return ({"workload": cfg["name"], "cells": cells, "steps": cfg["steps"],
+             "gpu_gcups": round(cells * cfg["steps"] / wall / 1e9, 3)}, cfg, final)
+
+
+def cpu_small(cfg, final):
+    """The 512 x 256 case on the oracle, all host cores (one thread per band)
+    and one thread, each over the full 2000 steps; the GPU's final cells and
+    block sums compared with the oracle's."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from oracle.workloads_cpu import config_oracle
+
+    cells = cfg["w"] * cfg["h"]
+    rates = {}
+    for par in (True, False):
+        lat = config_oracle(O, cfg)
+        lat.set_parallel(par)
+        t0 = time.perf_counter()
+        lat.advance(cfg["steps"])
+        rates[par] = cells * cfg["steps"] / (time.perf_counter() - t0) / 1e9
+    f = O.Field(cfg["w"], cfg["h"], cfg["block"], cfg["block"])
+    same = bool(np.array_equal(final[0], lat.cells)) and all(
+        np.array_equal(a, b) for a, b in zip(final[1], f.block_sums(lat)))
+    return {"workload": cfg["name"], "value": round(rates[True], 4), "unit": "GCUPS",
+            "cores": cfg["bands"], "single_thread": round(rates[False], 4),
+            "verified_bit_exact": same}
+
+
 def shear_leg(steps, bands, passes, units, dunits, ahead):
     """The transport-measure workload at full size (32768 x 16384, 537M cells,
     periodic, density 0.3, shear wave from seed 7): step rate and algorithmic
@@ -330,6 +384,13 @@ def main():
         gpu_cells = g.store()
         out["cpu_baseline"] = cpu_baseline(cfg, total, gpu_cells, args.cpu_budget)
     g.close()
+    try:
+        out["small_512x256"], small_cfg, small_final = small_leg(
+            bands, args.passes, args.cells_per_lane, args.draw_cells_per_lane, args.draw_ahead)
+        if "cpu_baseline" in out:
+            out["cpu_baseline"]["small_512x256"] = cpu_small(small_cfg, small_final)
+    except Exception as e:
+        out["small_512x256"] = {"error": repr(e)}
     if args.shear_steps:
         try:
             out["shear_537m"] = shear_leg(args.shear_steps, bands, args.passes, args.cells_per_lane,

@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(256) lg_tables_batch_kernel(const LgDesc* __re
 }
 
 // Step k of the chunk for every tile. A tile is TILE consecutive thread
-// indices of one member, mapped to work units exactly as lg_step_kernel maps
+// indices of one member, mapped to work units exactly as lg_step_kernel_{w,h} map
 // them (the member's static word order, NH units per word).
 template <int TILE, int NH>
 __global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) lg_step_batch_kernel(const LgDesc* __restrict D,

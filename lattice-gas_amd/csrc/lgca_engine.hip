@@ -1,11 +1,14 @@
 // MI355X FHP lattice-gas engine: HIP kernels and the C ABI of include/lgca.h.
 //
-// Hot path per step (one launch of lg_step_kernel, one thread per u64 word):
-// gather six direction planes with in-word shifts and cross-word carries,
-// jump each word's position in its CPU band stream, draw the 64 collision
-// indices, evaluate the generated collision circuit, bounce back on solid
-// sites, re-seed the inflow columns, mask the ragged tail. The per-step
-// stream jump tables come from lg_tables_kernel, one launch per batch of
+// Hot path per step (default: one launch of lg_step2_kernel): draw work --
+// jump each word's position in its CPU band stream, draw the collision
+// columns of its cells and, in inflow words, the re-seeded states, for the
+// next step -- and collide work -- gather six direction planes with in-word
+// shifts and cross-word carries, evaluate the generated collision circuit with
+// the digits drawn one launch earlier, bounce back on solid sites, overwrite
+// the inflow cells, mask the ragged tail. The alternatives (one fused kernel
+// per step; draws on a second stream) give the same cells. The per-step
+// stream jump tables come from lg_tables_kernel, one launch per chunk of
 // steps.
 #include <hip/hip_runtime.h>
 
