@@ -75,11 +75,16 @@ __global__ void __launch_bounds__(256) lg_tables_batch_kernel(const LgDesc* __re
   }
 }
 
+// The batch kernels are flattened: with a dozen kernels sharing the per-unit
+// device functions the compiler would otherwise reach them through calls,
+// paying register saves and stack on every unit (measured 1.8x slower).
+#define LG_FLAT __attribute__((flatten))
+
 // Step k of the chunk for every tile. A tile is TILE consecutive thread
 // indices of one member, mapped to work units exactly as lg_step_kernel_{w,h} map
 // them (the member's static word order, NH units per word).
 template <int TILE, int NH>
-__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) lg_step_batch_kernel(const LgDesc* __restrict D,
+__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) LG_FLAT lg_step_batch_kernel(const LgDesc* __restrict D,
                                                              const uint2* __restrict tiles,
                                                              uint32_t ntiles, uint32_t k) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -105,7 +110,7 @@ __global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) 
 // itself the compiler unrolls the tile loop into 225 VGPRs for whole-word
 // units, 2 waves per SIMD.
 template <int TILE, int NH, int DNH>
-__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(5))) LG_FLAT
     lg_step2_batch_kernel(const LgDesc* __restrict D, const uint2* __restrict tiles,
                           uint32_t ntiles, uint32_t k) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -137,7 +142,7 @@ __global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(5)))
 
 // The call's first step for members still fresh: its draw tiles only.
 template <int TILE, int DNH>
-__global__ void __launch_bounds__(TILE)
+__global__ void __launch_bounds__(TILE) LG_FLAT
     lg_draw0_batch_kernel(const LgDesc* __restrict D, const uint2* __restrict tiles,
                           uint32_t ntiles) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {

@@ -157,6 +157,15 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
     clock = lambda: time.perf_counter() - t0
     quanta = 0
     peak = 0
+    spent = {"admit": 0.0, "batch_setup": 0.0, "advance": 0.0, "snapshot": 0.0,
+             "retire": 0.0, "idle": 0.0}     # wall seconds by activity
+
+    def timed(key, fn, *a):
+        t = time.perf_counter()
+        r = fn(*a)
+        spent[key] += time.perf_counter() - t
+        return r
+
     while pending or active:
         now = clock()
         changed = False
@@ -166,30 +175,33 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
             if used + fp > hbm_budget and active:
                 break                                # wait for memory to free up
             pending.popleft()
-            g = build(r, block)
+            g = timed("admit", build, r, block)
             used += fp
             results[r.rid].start = clock()
             active.append({"req": r, "g": g, "done": 0, "bytes": fp})
             changed = True
         if not active:
             if pending:
-                time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
+                timed("idle", time.sleep, max(0.0, min(pending[0].arrival - clock(), 0.01)))
             continue
         peak = max(peak, len(active))
         if changed or batch is None:
+            ts = time.perf_counter()
             if batch is not None:
                 batch.close()
             batch = Batch([a["g"] for a in active])
             if shape:
                 batch.set_shape(*shape)
+            spent["batch_setup"] += time.perf_counter() - ts
         limits = [a["req"].steps - a["done"] for a in active]
-        batch.advance_limited(every, limits)
+        timed("advance", batch.advance_limited, every, limits)
         quanta += 1
-        snaps = batch.snapshot(1.0)
+        snaps = timed("snapshot", batch.snapshot, 1.0)
         for a, lim, snap in zip(active, limits, snaps):
             a["done"] += min(every, lim)
             results[a["req"].rid].snapshots.append((a["done"],) + tuple(np.array(x) for x in snap))
         finished = [a for a in active if a["done"] >= a["req"].steps]
+        tr = time.perf_counter()
         if finished:
             for a in finished:
                 res = results[a["req"].rid]
@@ -202,10 +214,12 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
             batch = None
             for a in finished:
                 a["g"].close()
+        spent["retire"] += time.perf_counter() - tr
     t_last = max(res.done for res in results.values())
     s = _summary(trace, results, 0.0, t_last, "batched")
     s["quanta"] = quanta
     s["peak_resident"] = peak
+    s["wall_s_by_activity"] = {k: round(v, 4) for k, v in spent.items()}
     return s, results
 
 
@@ -214,23 +228,33 @@ def serve_plain(trace: list[Request], every: int = 100, block: int = 32, keep_ce
     results = {r.rid: Result(r.rid, r.arrival) for r in trace}
     t0 = time.perf_counter()
     clock = lambda: time.perf_counter() - t0
+    spent = {"admit": 0.0, "advance": 0.0, "snapshot": 0.0, "idle": 0.0}
     for r in sorted(trace, key=lambda r: r.arrival):
         wait = r.arrival - clock()
         if wait > 0:
             time.sleep(wait)
+            spent["idle"] += wait
         res = results[r.rid]
         res.start = clock()
+        t = time.perf_counter()
         g = build(r, block)
+        spent["admit"] += time.perf_counter() - t
         done = 0
         while done < r.steps:
             q = min(every, r.steps - done)
+            t = time.perf_counter()
             g.advance(q)
+            spent["advance"] += time.perf_counter() - t
             done += q
+            t = time.perf_counter()
             g.sample_field(1.0)
             res.snapshots.append(_snapshot(g, done))
+            spent["snapshot"] += time.perf_counter() - t
         if keep_cells:
             res.final_cells = g.store()
         res.done = clock()
         g.close()
     t_last = max(res.done for res in results.values())
-    return _summary(trace, results, 0.0, t_last, "plain"), results
+    s = _summary(trace, results, 0.0, t_last, "plain")
+    s["wall_s_by_activity"] = {k: round(v, 4) for k, v in spent.items()}
+    return s, results
