@@ -182,6 +182,10 @@ def ensemble_leg(n, steps, bands, seed=4):
             "plain_loop_sample": f"first {probe} lattices, {steps} steps each"}
 
 
+class _Skip(Exception):
+    pass
+
+
 def small_leg(bands, passes, units, dunits, ahead):
     """The reference's CPU-runnable case (512 x 256, plate 64 rows at x = 128,
     2000 steps, 16 x 16 block sums) on the GPU: its rate, and its final cells
@@ -293,6 +297,8 @@ def main():
     ap.add_argument("--serve", type=int, default=64, metavar="N",
                     help="requests in the serving leg (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the 512 x 256 leg (keeps profiles to the headline lattice)")
     ap.add_argument("--shear-steps", type=int, default=400, metavar="N",
                     help="steps of the 537M-cell shear-wave leg (0 skips it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
@@ -385,10 +391,14 @@ def main():
         out["cpu_baseline"] = cpu_baseline(cfg, total, gpu_cells, args.cpu_budget)
     g.close()
     try:
+        if args.no_small:
+            raise _Skip()
         out["small_512x256"], small_cfg, small_final = small_leg(
             bands, args.passes, args.cells_per_lane, args.draw_cells_per_lane, args.draw_ahead)
         if "cpu_baseline" in out:
             out["cpu_baseline"]["small_512x256"] = cpu_small(small_cfg, small_final)
+    except _Skip:
+        pass
     except Exception as e:
         out["small_512x256"] = {"error": repr(e)}
     if args.shear_steps:

@@ -547,13 +547,13 @@ int ensure_ahead_stream(lg_t* g) {
 // Draw-ahead: launch the draws of steps [s0, s0 + nd) of the chunk whose
 // tables are in basis into the next slot group on dstream, then their
 // collides on the main stream.
-void enqueue_ahead_group(lg_t* g, uint32_t s0, uint32_t nd) {
+int enqueue_ahead_group(lg_t* g, uint32_t s0, uint32_t nd) {
   const uint32_t per = g->G.nbands + 1;
   const size_t nhalf = (size_t)2 * g->G.total, nih = (size_t)2 * g->G.iw * g->G.h;
   const uint32_t q = (uint32_t)(g->dgroup++ & 1u);
   uint32_t* kb = g->kbuf.p + (size_t)q * g->ahead * LG_KDIGITS * nhalf;
   uint32_t* ib = g->ibuf.p + (size_t)q * g->ahead * 7 * nih;
-  (void)hipStreamWaitEvent(g->dstream, g->ev_used[q], 0);
+  HIP_TRY(hipStreamWaitEvent(g->dstream, g->ev_used[q], 0));
   const dim3 dg(blocks_for((size_t)g->G.total * g->draw_units), nd);
   const uint64_t* tabs0 = g->basis.p + (size_t)s0 * per * LG_WTAB;
   if (g->draw_units == 2)
@@ -562,8 +562,9 @@ void enqueue_ahead_group(lg_t* g, uint32_t s0, uint32_t nd) {
   else
     hipLaunchKernelGGL(lg_draw_kernel<1>, dg, dim3(256), 0, g->dstream, g->G, g->solid.p,
                        g->cjump.p, g->row_band.p, g->ijump.p, g->order.p, tabs0, per, kb, ib);
-  (void)hipEventRecord(g->ev_drawn[q], g->dstream);
-  (void)hipStreamWaitEvent(g->stream, g->ev_drawn[q], 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(g->ev_drawn[q], g->dstream));
+  HIP_TRY(hipStreamWaitEvent(g->stream, g->ev_drawn[q], 0));
   for (uint32_t k = 0; k < nd; ++k) {
     const uint32_t* kc = kb + (size_t)k * LG_KDIGITS * nhalf;
     const uint32_t* ic = ib + (size_t)k * 7 * nih;
@@ -576,7 +577,9 @@ void enqueue_ahead_group(lg_t* g, uint32_t s0, uint32_t nd) {
     std::swap(g->a.p, g->b.p);
     g->steps += 1;
   }
-  (void)hipEventRecord(g->ev_used[q], g->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(g->ev_used[q], g->stream));
+  return LG_OK;
 }
 
 // Enqueue `n` steps (and samples every `every`), no synchronisation.
@@ -625,9 +628,11 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
       HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used], g->stream));
     }
     if (ahead) {
-      for (uint64_t s0 = 0; s0 < chunk; s0 += g->ahead)
-        enqueue_ahead_group(g, (uint32_t)s0,
-                            (uint32_t)std::min<uint64_t>(g->ahead, chunk - s0));
+      for (uint64_t s0 = 0; s0 < chunk; s0 += g->ahead) {
+        int rc = enqueue_ahead_group(g, (uint32_t)s0,
+                                     (uint32_t)std::min<uint64_t>(g->ahead, chunk - s0));
+        if (rc) return rc;
+      }
     } else if (g->passes == 1) {
       for (uint64_t k = 0; k < chunk; ++k) {
         if (g->units == 2)
