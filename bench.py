@@ -252,10 +252,13 @@ def shear_leg(steps, bands, passes, units, dunits, ahead):
     dev_ms = g.advance_timed(steps)
     wall = time.perf_counter() - t0
     cells = cfg["w"] * cfg["h"]
+    p = g.pipeline()
+    pipe = {"passes": p[0], "cells_per_lane": p[1],
+            "draw_cells_per_lane": p[2] if p[0] == 2 else None}
     g.close()
     step_ms = dev_ms / steps
     ach = cells * BYTES_PER_CELL_UPDATE / (step_ms * 1e-3) / 1e9
-    return {"workload": cfg["name"], "cells": cells, "steps": steps,
+    return {"workload": cfg["name"], "cells": cells, "steps": steps, "pipeline": pipe,
             "gcups": round(cells * steps / wall / 1e9, 3), "mean_step_ms": round(step_ms, 4),
             "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
 
@@ -282,8 +285,9 @@ def main():
     ap.add_argument("--bands", type=int, default=0, help="CPU bands (default: host threads, <= 16)")
     ap.add_argument("--cells-per-lane", type=int, default=64, choices=(64, 32),
                     help="step-kernel work unit: a whole u64 word or half of it")
-    ap.add_argument("--passes", type=int, default=2, choices=(1, 2),
-                    help="kernels per step: draw pass + collide pass (2) or one fused kernel (1)")
+    ap.add_argument("--passes", type=int, default=0, choices=(0, 1, 2),
+                    help="kernels per step: draw pass + collide pass (2), one fused kernel (1) "
+                         "or chosen by lattice size (0)")
     ap.add_argument("--draw-cells-per-lane", type=int, default=64, choices=(64, 32),
                     help="work unit of the two-pass step's draw work")
     ap.add_argument("--draw-ahead", type=int, default=0, metavar="N",
@@ -351,11 +355,12 @@ def main():
     step_ms = dev_ms / args.steps
     traffic_alg = cells * BYTES_PER_CELL_UPDATE
     achieved = traffic_alg / (step_ms * 1e-3) / 1e9
-    unit = "h" if args.cells_per_lane == 32 else "w"
-    nh, dnh = 3 - args.cells_per_lane // 32, 3 - args.draw_cells_per_lane // 32
-    if args.passes == 1:
+    passes, cpl, dcpl, ahead = g.pipeline()  # what the timed steps ran
+    unit = "h" if cpl == 32 else "w"
+    nh, dnh = 3 - cpl // 32, 3 - dcpl // 32
+    if passes == 1:
         kernel = f"lg_step_kernel_{unit}"
-    elif args.draw_ahead:
+    elif ahead:
         kernel = f"lg_collide_kernel<{nh}>+lg_draw_kernel<{dnh}>"
     else:
         kernel = f"lg_step2_kernel<{nh},{dnh}>"
@@ -376,9 +381,9 @@ def main():
         "config": {"workload": cfg["name"], "w": cfg["w"], "h": cfg["h"], "cells": cells,
                    "rest": True, "bands": bands, "inlet_cols": cfg["inlet"],
                    "sample_every": every, "block": cfg["block"], "seed": cfg["seed"],
-                   "cells_per_lane": args.cells_per_lane, "passes": args.passes,
-                   "draw_cells_per_lane": args.draw_cells_per_lane if args.passes == 2 else None,
-                   "draw_ahead": args.draw_ahead if args.passes == 2 else None},
+                   "cells_per_lane": cpl, "passes": passes,
+                   "draw_cells_per_lane": dcpl if passes == 2 else None,
+                   "draw_ahead": ahead if passes == 2 else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(kernel, cfg["name"]),

@@ -15,8 +15,9 @@
  *     the message is then available from lg_last_error() (thread-local). The
  *     library never aborts on bad input; where the reference asserts
  *     (src/gpu.rs:480-481,573,631,727-731) it returns LG_EINVAL.
- *   - A handle owns all its device memory and its own HIP stream; use one
- *     handle from one thread at a time. Distinct handles may run concurrently.
+ *   - A handle owns all its device memory and its own HIP stream (a second one
+ *     for draw-ahead, lg_set_draw_ahead); use one handle from one thread at a
+ *     time. Distinct handles may run concurrently.
  *   - Host arrays are borrowed for the duration of the call only.
  *   - Cells are one byte per site in the reference's layout (src/hex.rs:7-23):
  *     bits 0-5 moving directions, bit 6 rest, bit 7 solid; row-major, y * w + x.
@@ -112,13 +113,20 @@ int lg_set_steps(lg_t* g, uint64_t steps);
  * either way. */
 int lg_set_work_unit(lg_t* g, uint32_t cells_per_lane);
 
-/* Work per step launch: 2 (the default) splits the step into a draw pass
- * (the random collision columns and inflow samples, which depend only on the
- * seed, the step and the obstacle layout) and a gather-collide-store pass;
- * each launch collides step t while it draws step t + 1, so memory waits
- * of the one overlap the arithmetic of the other. 1 fuses both passes of
- * one step into each lane. The results are identical either way. */
+/* Work per step launch: 2 splits the step into a draw pass (the random
+ * collision columns and inflow samples, which depend only on the seed, the
+ * step and the obstacle layout) and a gather-collide-store pass; each launch
+ * collides step t while it draws step t + 1, so memory waits of the one
+ * overlap the arithmetic of the other. 1 fuses both passes of one step into
+ * each lane. 0 (the default) picks by size: the two-pass step for lattices
+ * whose one-launch step holds too few waves to keep every CU busy to its end,
+ * the fused step for larger ones. The results are identical either way. */
 int lg_set_passes(lg_t* g, uint32_t passes);
+
+/* The pipeline the next step will use: kernels per step (1 or 2), cells per
+ * lane of the collide (or fused) and draw work, draw-ahead steps. */
+int lg_get_pipeline(lg_t* g, uint32_t* passes, uint32_t* cells_per_lane,
+                    uint32_t* draw_cells_per_lane, uint32_t* draw_ahead);
 
 /* Draw-ahead for the two-pass step: 0 runs each step's draw work in the
  * launch that collides the step before it; N > 0 (at most 64) runs the draws

@@ -589,7 +589,8 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
   uint64_t left = n, since = 0;
   const uint32_t per = g->G.nbands + 1;
   if (timed) g->tev_used = 0;
-  const bool ahead = g->passes == 2 && g->ahead > 0;
+  const uint32_t passes = step_passes(g);
+  const bool ahead = passes == 2 && g->ahead > 0;
   if (ahead) {
     int rc = ensure_ahead_stream(g);
     if (!rc) rc = ensure_draw_buffers(g);
@@ -599,7 +600,7 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
     HIP_TRY(hipStreamWaitEvent(g->dstream, g->ev_start, 0));
   }
   while (left > 0) {
-    const uint32_t extra = (g->passes == 2 && !ahead) ? 1u : 0u;  // see the two-pass branch
+    const uint32_t extra = (passes == 2 && !ahead) ? 1u : 0u;  // see the two-pass branch
     uint64_t chunk = std::min<uint64_t>(left, g->basis_steps - extra);
     if (every > 0) chunk = std::min<uint64_t>(chunk, every - since);
     const uint32_t ntab = (uint32_t)chunk + extra;
@@ -633,7 +634,7 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
                                      (uint32_t)std::min<uint64_t>(g->ahead, chunk - s0));
         if (rc) return rc;
       }
-    } else if (g->passes == 1) {
+    } else if (passes == 1) {
       for (uint64_t k = 0; k < chunk; ++k) {
         if (g->units == 2)
           hipLaunchKernelGGL(lg_step_kernel_h, dim3(blocks_for((size_t)g->G.total * 2)), dim3(256),
@@ -683,6 +684,11 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
 }
 
 }  // namespace
+
+uint32_t lgca_internal::step_passes(const lg_t* g) {
+  if (g->passes) return g->passes;
+  return g->G.total >= LG_AUTO_FUSED_WORDS ? 1u : 2u;
+}
 
 int lgca_internal::ensure_tables(lg_t* g) {
   if (!g->tables_dirty) return LG_OK;
@@ -999,8 +1005,19 @@ int lg_set_draw_unit(lg_t* g, uint32_t cells_per_lane) {
 
 int lg_set_passes(lg_t* g, uint32_t passes) {
   if (!g) return fail(LG_EINVAL, "null handle");
-  if (passes != 1 && passes != 2) return fail(LG_EINVAL, "passes per step must be 1 or 2");
+  if (passes > 2) return fail(LG_EINVAL, "passes per step must be 0 (by size), 1 or 2");
   g->passes = passes;
+  return LG_OK;
+}
+
+int lg_get_pipeline(lg_t* g, uint32_t* passes, uint32_t* cells_per_lane,
+                    uint32_t* draw_cells_per_lane, uint32_t* draw_ahead) {
+  if (!g || !passes || !cells_per_lane || !draw_cells_per_lane || !draw_ahead)
+    return fail(LG_EINVAL, "null argument");
+  *passes = step_passes(g);
+  *cells_per_lane = g->units == 1 ? 64u : 32u;
+  *draw_cells_per_lane = g->draw_units == 1 ? 64u : 32u;
+  *draw_ahead = *passes == 2 ? g->ahead : 0u;
   return LG_OK;
 }
 

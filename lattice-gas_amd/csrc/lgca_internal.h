@@ -55,6 +55,18 @@ inline uint32_t blocks_for(size_t n, uint32_t t = 256) { return (uint32_t)((n + 
 // Equilibrium thresholds exactly as src/lattice.rs:59-82 computes them.
 void equilibrium_thresholds(float density, float ux, float uy, bool rest, uint32_t out[7]);
 
+// Words per plane from which the default step is the fused one (measured on
+// MI355X: at 147,456 words, 4096 x 2304, the two-pass step is faster -- 647 vs
+// ~500 GCUPS with a plate and inflow, 686 vs 673 without; from 331,776 words,
+// 6144 x 3456, the fused step is, up to 1,134 vs 991 GCUPS at 8.4M words).
+#define LG_AUTO_FUSED_WORDS (1u << 18)
+
+// Kernels per step actually used: lg_set_passes's choice, or for 0 (the
+// default) the fused step from LG_AUTO_FUSED_WORDS words up -- where one
+// launch per step holds enough waves to fill the chip -- and the two-pass
+// step below it.
+uint32_t step_passes(const lg_lattice* g);
+
 // Rebuild the static jump and order tables of a lattice if its obstacle
 // layout or inflow width changed since they were last built.
 int ensure_tables(lg_lattice* g);
@@ -86,8 +98,9 @@ struct lg_lattice {
   uint32_t units = 1;  // 1: a lane steps a 64-cell word; 2: a 32-cell half
   // passes per step: 1 = one fused kernel per step; 2 = one launch per step
   // that collides step t and draws step t + 1's random digits into the other
-  // slot of kbuf/ibuf (kslot: the slot holding the current step's digits)
-  uint32_t passes = 2;
+  // slot of kbuf/ibuf (kslot: the slot holding the current step's digits);
+  // 0 = chosen by size (step_passes)
+  uint32_t passes = 0;
   uint32_t draw_units = 1;  // draw lanes of the two-pass step: 1 = 64 cells, 2 = 32
   uint32_t kslot = 0;
   // Draw-ahead (ahead > 0): the draw work runs on `dstream` in launches of up

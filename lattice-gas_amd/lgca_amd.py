@@ -62,6 +62,7 @@ def load_library(path: str | None = None):
         "lg_set_passes": (i32, [vp, C.c_uint32]),
         "lg_set_draw_unit": (i32, [vp, C.c_uint32]),
         "lg_set_draw_ahead": (i32, [vp, C.c_uint32]),
+        "lg_get_pipeline": (i32, [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "lg_batch_set_units": (i32, [vp, C.c_uint32]),
         "lg_batch_get_units": (i32, [vp, P(C.c_uint32)]),
         "lg_batch_set_passes": (i32, [vp, C.c_uint32]),
@@ -209,8 +210,16 @@ class GpuLattice:
 
     def set_passes(self, passes: int):
         """2 (each launch collides one step and draws the next step's random
-        columns; the default) or 1 (draw and collide fused per lane)."""
+        columns), 1 (draw and collide fused per lane) or 0 (by size, the
+        default)."""
         _check(load_library().lg_set_passes(self._p, int(passes)))
+
+    def pipeline(self):
+        """(kernels per step, cells per lane, draw cells per lane, draw-ahead)
+        the next step will use."""
+        v = [C.c_uint32() for _ in range(4)]
+        _check(load_library().lg_get_pipeline(self._p, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def set_draw_unit(self, cells_per_lane: int):
         """64 or 32 cells per lane for the draw work of the two-pass step."""

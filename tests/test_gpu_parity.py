@@ -313,3 +313,20 @@ def test_draw_ahead_sampling_and_mode_changes(oracle, ahead):
         field.sample(lat, 1.0)
         assert np.array_equal(g.store(), lat.cells), f"call {rep}"
         assert [list(x) for x in g.read_block_sums()] == [list(x) for x in field.block_sums(lat)]
+
+
+def test_default_pipeline_follows_the_lattice_size():
+    """passes = 0 (the default) is the two-pass step for the 4096 x 2304 plate
+    lattice and the fused step for the 32768 x 16384 shear lattice; an explicit
+    choice overrides it."""
+    say("default pipeline by size")
+    small = GpuLattice(4096, 2304, True, 1, 16)
+    assert small.pipeline()[0] == 2
+    small.set_passes(1)
+    assert small.pipeline()[0] == 1
+    small.close()
+    big = GpuLattice(32768, 16384, True, 1, 16)
+    assert big.pipeline()[0] == 1
+    big.set_passes(2)
+    assert big.pipeline()[:3] == (2, 64, 64)
+    big.close()
