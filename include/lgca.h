@@ -208,8 +208,9 @@ int lg_batch_get_shape(lg_batch_t* b, uint32_t* tile, uint32_t* grid, uint32_t* 
 /* Cells per lane (64 or 32) for the batch's step kernel, as lg_set_work_unit. */
 int lg_batch_set_units(lg_batch_t* b, uint32_t cells_per_lane);
 int lg_batch_get_units(lg_batch_t* b, uint32_t* cells_per_lane);
-/* Kernels per step (2, the default: collide while drawing the next step; 1:
- * fused) and the draw work's cells per lane, as lg_set_passes /
+/* Kernels per step (2: collide while drawing the next step; 1: fused; the
+ * default follows lg_set_passes's size rule applied to all members' words
+ * together) and the draw work's cells per lane, as lg_set_passes /
  * lg_set_draw_unit for one lattice. */
 int lg_batch_set_passes(lg_batch_t* b, uint32_t passes);
 int lg_batch_set_draw_unit(lg_batch_t* b, uint32_t cells_per_lane);

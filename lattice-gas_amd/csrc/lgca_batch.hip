@@ -207,7 +207,7 @@ struct lg_batch {
   uint32_t grid = 0;     // step-kernel workgroups; 0 = one per tile
   uint32_t chunk = 32;   // steps per graph replay (even)
   uint32_t units = 1;    // collide (or fused) lanes: 1 = 64 cells, 2 = 32 cells
-  uint32_t passes = 2;   // 2: two-pass step, 1: fused step
+  uint32_t passes = 2;   // 2: two-pass step, 1: fused step (set by size at creation)
   uint32_t draw_units = 1;  // draw lanes of the two-pass step
   // graph for `chunk` steps with the current shape
   hipGraph_t graph = nullptr;
@@ -481,6 +481,10 @@ lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
     b->max_tab_blocks_per_step = std::max(b->max_tab_blocks_per_step, blocks_for((size_t)per * 16));
   }
   b->chunk = std::min<uint32_t>(32, b->kmax & ~1u);
+  // the same size rule as one lattice's default step, over all members' words
+  size_t words = 0;
+  for (lg_t* g : b->members) words += g->G.total;
+  b->passes = words >= LG_AUTO_FUSED_WORDS ? 1u : 2u;
   if (b->chunk == 0) b->chunk = 2;
   if (b->desc.alloc(n)) return bail("hipMalloc failed for descriptors");
   if (ensure_tiles(b)) return bail(g_err);
