@@ -175,6 +175,19 @@ void hs_collide_table(int rest, uint8_t* out) {
 
 uint64_t hs_charpoly(void) { return lgca::charpoly_low(); }
 
+// The device source's generator: Rng::new(seed), then n next_u64 outputs.
+uint64_t hs_rng_state(uint64_t seed) { return lg_rng_init(seed); }
+void hs_rng_u64(uint64_t seed, size_t n, uint64_t* out) {
+  uint64_t s = lg_rng_init(seed);
+  for (size_t i = 0; i < n; ++i) {
+    s = lg_xs(s);
+    out[i] = s * LG_RNG_MUL;
+  }
+}
+// The device source's stream seeds (band b of step t; inflow after step t).
+uint64_t hs_band_seed(uint64_t seed, uint64_t t, uint32_t b) { return lg_band_seed(seed, t, b); }
+uint64_t hs_inlet_seed(uint64_t seed, uint64_t t) { return lg_inlet_seed(seed, t + 1); }
+
 // Table-driven multiplication by z^f mod P against the bit-serial one, on n
 // pseudo-random (c, f) pairs with f up to 199; returns the mismatch count.
 int hs_mulz_check(uint64_t seed, int n) {
