@@ -181,16 +181,18 @@ __global__ void __launch_bounds__(256) lg_snapshot_kernel(const LgDesc* __restri
   const LgDesc& d = D[blockIdx.y];
   if (!d.fux) return;
   const uint32_t nb = d.bw * d.bh;
-  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t g = blockIdx.x * 4 + (threadIdx.x >> 6);  // one wavefront per block
   if (g >= nb) return;
   const uint64_t* planes = d.cur ? d.p1 : d.p0;
-  lg_sample_block(d.G, planes, d.solid, d.bx, d.by, d.bw, alpha, d.fux, d.fuy, d.frho, d.fsol,
-                  d.fsums, g);
-  float* out = staging + d.snap_off;
-  out[g] = d.fux[g];
-  out[nb + g] = d.fuy[g];
-  out[2 * nb + g] = d.frho[g];
-  out[3 * nb + g] = d.fsol[g];
+  lg_sample_block_wave(d.G, planes, d.solid, d.bx, d.by, d.bw, alpha, d.fux, d.fuy, d.frho,
+                       d.fsol, d.fsums, g, threadIdx.x & 63u);
+  if ((threadIdx.x & 63u) == 0) {
+    float* out = staging + d.snap_off;
+    out[g] = d.fux[g];
+    out[nb + g] = d.fuy[g];
+    out[2 * nb + g] = d.frho[g];
+    out[3 * nb + g] = d.fsol[g];
+  }
 }
 
 struct lg_batch {
@@ -533,7 +535,7 @@ int lg_batch_snapshot(lg_batch_t* b, float alpha, float* out, size_t nfloats) {
   uint32_t maxnb = 0;
   for (lg_t* g : b->members)
     if (g->has_field) maxnb = std::max(maxnb, g->bw * g->bh);
-  dim3 grid(blocks_for(maxnb), (uint32_t)b->members.size());
+  dim3 grid(blocks_for(maxnb, 4), (uint32_t)b->members.size());
   hipLaunchKernelGGL(lg_snapshot_kernel, grid, dim3(256), 0, b->stream, b->desc.p, alpha,
                      b->staging.p);
   HIP_TRY(hipGetLastError());

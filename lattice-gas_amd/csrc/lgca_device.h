@@ -143,35 +143,48 @@ struct LgGeom {
 // with every operation rounded on its own (no fused multiply-add), as the CPU
 // computes it. sums[4 g ..] receives (mass, px2, py2, nsolid).
 #if defined(__HIPCC__)
-__device__ inline void lg_sample_block(const LgGeom& G, const uint64_t* __restrict planes,
-                                       const uint64_t* __restrict solid, uint32_t bx, uint32_t by,
-                                       uint32_t bw, float alpha, float* __restrict ux,
-                                       float* __restrict uy, float* __restrict rho,
-                                       float* __restrict sol, int64_t* __restrict sums, uint32_t g) {
-  uint32_t jb = g / bw, ib = g - jb * bw;
+// Exact partial sums over the (row, word) pieces item0, item0 + istep, ...
+// of block g (by rows x the words the block's bx columns touch).
+__device__ inline void lg_block_sums(const LgGeom& G, const uint64_t* __restrict planes,
+                                     const uint64_t* __restrict solid, uint32_t bx, uint32_t by,
+                                     uint32_t bw, uint32_t g, uint32_t item0, uint32_t istep,
+                                     int64_t s[4]) {
+  const uint32_t jb = g / bw, ib = g - jb * bw;
   const int CX2[6] = {2, 1, -1, -2, -1, 1};
   const int CY2[6] = {0, 1, 1, 0, -1, -1};
-  uint32_t x0 = ib * bx, x1 = x0 + bx;
-  uint32_t w0 = x0 >> 6, w1 = (x1 - 1) >> 6;
+  const uint32_t x0 = ib * bx, x1 = x0 + bx;
+  const uint32_t w0 = x0 >> 6, nwb = ((x1 - 1) >> 6) - w0 + 1;
   int64_t mass = 0, px = 0, py = 0, ns = 0;
-  for (uint32_t y = jb * by; y < (jb + 1) * by; ++y) {
-    size_t base = (size_t)y * G.wpr;
-    for (uint32_t wi = w0; wi <= w1; ++wi) {
-      uint32_t lo = max(x0, wi << 6) - (wi << 6);
-      uint32_t hi = min(x1, (wi + 1) << 6) - (wi << 6);
-      uint64_t m = (hi - lo >= 64) ? ~0ull : (((1ull << (hi - lo)) - 1ull) << lo);
-      uint64_t sm = solid[base + wi] & m;
-      ns += __popcll(sm);
-      uint64_t fm = m & ~sm;
-      for (int d = 0; d < 6; ++d) {
-        int64_t c = __popcll(planes[(size_t)d * G.total + base + wi] & fm);
-        mass += c;
-        px += CX2[d] * c;
-        py += CY2[d] * c;
-      }
-      mass += __popcll(planes[(size_t)6 * G.total + base + wi] & fm);
+  for (uint32_t it = item0; it < by * nwb; it += istep) {
+    const uint32_t r = it / nwb, wi = w0 + (it - r * nwb);
+    const size_t base = (size_t)(jb * by + r) * G.wpr;
+    const uint32_t lo = max(x0, wi << 6) - (wi << 6);
+    const uint32_t hi = min(x1, (wi + 1) << 6) - (wi << 6);
+    const uint64_t m = (hi - lo >= 64) ? ~0ull : (((1ull << (hi - lo)) - 1ull) << lo);
+    const uint64_t sm = solid[base + wi] & m;
+    ns += __popcll(sm);
+    const uint64_t fm = m & ~sm;
+    for (int d = 0; d < 6; ++d) {
+      const int64_t c = __popcll(planes[(size_t)d * G.total + base + wi] & fm);
+      mass += c;
+      px += CX2[d] * c;
+      py += CY2[d] * c;
     }
+    mass += __popcll(planes[(size_t)6 * G.total + base + wi] & fm);
   }
+  s[0] = mass;
+  s[1] = px;
+  s[2] = py;
+  s[3] = ns;
+}
+
+// Block g's exact sums into sums[4 g ..] and the f32 running average of
+// src/render.rs:107-115, every operation rounded on its own as the CPU does.
+__device__ inline void lg_block_field(uint32_t bx, uint32_t by, float alpha, const int64_t s[4],
+                                      float* __restrict ux, float* __restrict uy,
+                                      float* __restrict rho, float* __restrict sol,
+                                      int64_t* __restrict sums, uint32_t g) {
+  const int64_t mass = s[0], px = s[1], py = s[2], ns = s[3];
   sums[(size_t)g * 4 + 0] = mass;
   sums[(size_t)g * 4 + 1] = px;
   sums[(size_t)g * 4 + 2] = py;
@@ -188,6 +201,21 @@ __device__ inline void lg_sample_block(const LgGeom& G, const uint64_t* __restri
   uy[g] = __fadd_rn(uy[g], __fmul_rn(alpha, __fsub_rn(vy, uy[g])));
   rho[g] = __fadd_rn(rho[g], __fmul_rn(alpha, __fsub_rn(__fdiv_rn((float)mass, n), rho[g])));
   sol[g] = __fdiv_rn((float)ns, n);
+}
+
+// One wavefront per block: each lane takes every 64th (row, word) piece, the
+// partial sums meet by a butterfly of lane exchanges, lane 0 writes.
+__device__ inline void lg_sample_block_wave(const LgGeom& G, const uint64_t* __restrict planes,
+                                            const uint64_t* __restrict solid, uint32_t bx,
+                                            uint32_t by, uint32_t bw, float alpha,
+                                            float* __restrict ux, float* __restrict uy,
+                                            float* __restrict rho, float* __restrict sol,
+                                            int64_t* __restrict sums, uint32_t g, uint32_t lane) {
+  int64_t s[4];
+  lg_block_sums(G, planes, solid, bx, by, bw, g, lane, 64, s);
+  for (int off = 32; off > 0; off >>= 1)
+    for (int i = 0; i < 4; ++i) s[i] += __shfl_xor(s[i], off, 64);
+  if (lane == 0) lg_block_field(bx, by, alpha, s, ux, uy, rho, sol, sums, g);
 }
 #endif
 

@@ -212,13 +212,15 @@ __global__ void __launch_bounds__(256) lg_unpack_kernel(LgGeom G, const uint64_t
 // over fluid cells (src/moments.rs:37-65 FLUID, src/render.rs:93-106), then
 // the f32 running average of src/render.rs:107-115 with every operation
 // rounded on its own (no fused multiply-add), as the CPU computes it.
+// One wavefront per coarse-grain block (four per workgroup).
 __global__ void __launch_bounds__(256) lg_sample_kernel(
     LgGeom G, const uint64_t* __restrict planes, const uint64_t* __restrict solid, uint32_t bx,
     uint32_t by, uint32_t bw, uint32_t bh, float alpha, float* __restrict ux, float* __restrict uy,
     float* __restrict rho, float* __restrict sol, int64_t* __restrict sums) {
-  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= bw * bh) return;
-  lg_sample_block(G, planes, solid, bx, by, bw, alpha, ux, uy, rho, sol, sums, g);
+  const uint32_t g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= bw * bh) return;  // whole wavefronts leave together
+  lg_sample_block_wave(G, planes, solid, bx, by, bw, alpha, ux, uy, rho, sol, sums, g,
+                       threadIdx.x & 63u);
 }
 
 // Obstacles (src/lattice.rs:210-220): every site whose f32 position
@@ -488,7 +490,7 @@ int add_solid(lg_t* g, int kind, float a, float b, float c, float d) {
 int sample_now(lg_t* g, float alpha) {
   // a lattice smaller than one block has an empty field (src/render.rs:73-75)
   if (!g->has_field || g->bw == 0 || g->bh == 0) return LG_OK;
-  hipLaunchKernelGGL(lg_sample_kernel, dim3(blocks_for((size_t)g->bw * g->bh)), dim3(256), 0,
+  hipLaunchKernelGGL(lg_sample_kernel, dim3(blocks_for((size_t)g->bw * g->bh, 4)), dim3(256), 0,
                      g->stream, g->G, g->a.p, g->solid.p, g->bx, g->by, g->bw, g->bh, alpha,
                      g->fux.p, g->fuy.p, g->frho.p, g->fsol.p, g->fsums.p);
   HIP_TRY(hipGetLastError());
