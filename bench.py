@@ -22,8 +22,10 @@ the same run on this host; its final cells are compared with the GPU's.
 Further objects in the same line: `small_512x256` (the reference's 512 x 256
 CPU-runnable case on the GPU; its CPU rates and check sit under
 `cpu_baseline`), `shear_537m` (the 32768 x 16384 transport workload's step
-rate), `ensemble` (256 uneven lattices, batched vs one at a time) and
-`serving` (a Poisson request stream, continuous batching vs the plain loop).
+rate), `ensemble` (256 uneven lattices, batched vs one at a time),
+`serving` (a Poisson request stream, continuous batching vs the plain loop) and
+`transport_measure` (the reference's --measure run on the engine: viscosity and
+advection factor with their wall time).
 """
 from __future__ import annotations
 
@@ -263,6 +265,21 @@ def shear_leg(steps, bands, passes, units, dunits, ahead):
             "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
 
 
+def transport_leg():
+    """The reference's --measure (src/main.rs:527-548 → src/transport.rs:291-319)
+    on the engine: viscosity and advection factor of the 256 x 256 boxes at
+    density 0.22 with rest particles, CPU threads 12, seed 0x5EED, whose
+    published values are 0.2989 and 0.4405 (docs/transport.md:14-15)."""
+    import lgca_amd
+
+    t0 = time.perf_counter()
+    nu, g = lgca_amd.measure_transport(0.22, True, 12, 0x5EED, 256)
+    wall = time.perf_counter() - t0
+    return {"nu": round(nu, 6), "g": round(g, 6), "wall_s": round(wall, 3),
+            "published": {"nu": 0.2989, "g": 0.4405},
+            "matches_published": f"{nu:.4f}" == "0.2989" and f"{g:.4f}" == "0.4405"}
+
+
 def serving_leg(n, rate, seed=5):
     """Served stream: Poisson arrivals, log-uniform sizes 256x256..4096x2304,
     steps 500-20000, random plate and drift, HBM budget 200 GB, 32x32-block
@@ -301,6 +318,8 @@ def main():
     ap.add_argument("--serve", type=int, default=64, metavar="N",
                     help="requests in the serving leg (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--no-transport", action="store_true",
+                    help="skip the transport-measure leg (the reference's --measure)")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 512 x 256 leg (keeps profiles to the headline lattice)")
     ap.add_argument("--shear-steps", type=int, default=400, metavar="N",
@@ -412,6 +431,11 @@ def main():
                                           args.draw_cells_per_lane, args.draw_ahead)
         except Exception as e:
             out["shear_537m"] = {"error": repr(e)}
+    if not args.no_transport:
+        try:
+            out["transport_measure"] = transport_leg()
+        except Exception as e:
+            out["transport_measure"] = {"error": repr(e)}
     if args.ensemble:
         try:
             out["ensemble"] = ensemble_leg(args.ensemble, args.ensemble_steps, bands)

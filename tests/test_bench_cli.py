@@ -19,5 +19,5 @@ def test_help_lists_the_pipeline_options():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0
     for opt in ("--passes", "--cells-per-lane", "--draw-cells-per-lane", "--draw-ahead",
-                "--shear-steps", "--no-small", "--ensemble", "--serve", "--copy-calibration"):
+                "--shear-steps", "--no-small", "--no-transport", "--ensemble", "--serve", "--copy-calibration"):
         assert opt in r.stdout
