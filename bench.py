@@ -23,9 +23,11 @@ Further objects in the same line: `small_512x256` (the reference's 512 x 256
 CPU-runnable case on the GPU; its CPU rates and check sit under
 `cpu_baseline`), `shear_537m` (the 32768 x 16384 transport workload's step
 rate), `ensemble` (256 uneven lattices, batched vs one at a time),
-`serving` (a Poisson request stream, continuous batching vs the plain loop) and
+`serving` (a Poisson request stream, continuous batching vs the plain loop),
 `transport_measure` (the reference's --measure run on the engine: viscosity and
-advection factor with their wall time).
+advection factor with their wall time) and `frames` (the frame writers on the
+reference's render-benchmark field, and what queueing a frame every 500 steps
+to the writer pool costs the headline run's stepping loop).
 """
 from __future__ import annotations
 
@@ -280,6 +282,121 @@ def transport_leg():
             "matches_published": f"{nu:.4f}" == "0.2989" and f"{g:.4f}" == "0.4405"}
 
 
+def frames_leg(bands, passes, units, dunits, ahead, workers=4, frames=10, frame_every=500):
+    """Frame output (src/main.rs:196-281), measured twice.
+
+    Writers: the field the reference times its encoders on (2048 x 1280 plate
+    run, 50 steps, blocks of 20), written by the library's write_vorticity
+    (scale 8, wmax 0.01) and write_arrows (1.6 in the frame (0.4, 0)), best
+    and median of 15 writes each, beside the reference's published 6.490 ms
+    and 2.988 ms (docs/benchmarks.md:312-313, an M3 Pro).
+
+    Run loop: the headline 4096 x 2304 workload stepped `frames` x
+    `frame_every` steps (coarse-grain every 100 steps) with and without a
+    frame queued after each chunk to `workers` writer threads (blocks of 32,
+    scale 8): what writing frames costs the stepping loop. Returns the
+    measures and the writers' field, for the CPU baseline's encoders."""
+    import shutil
+    import tempfile
+
+    import lgca_amd
+    import workloads as W
+
+    rc = W.render_bench(bands)
+    g = W.config_gpu(rc)
+    g.attach_field(rc["block"], rc["block"])
+    g.advance(rc["steps"])
+    g.sample_field(1.0)
+    ux, uy, _, solid = g.read_field()
+    bw, bh, bx, by = g.field_dims()
+    g.close()
+    field = (ux, uy, solid, bw, bh, bx, by)
+    d = tempfile.mkdtemp(prefix="lgca-frames-")
+    try:
+        t_png, t_svg = [], []
+        for _ in range(15):
+            t = time.perf_counter()
+            lgca_amd.write_vorticity(os.path.join(d, "v.png"), field, 8, 0.01)
+            t_png.append(time.perf_counter() - t)
+            t = time.perf_counter()
+            lgca_amd.write_arrows(os.path.join(d, "a.svg"), field, 1.6, (0.4, 0.0))
+            t_svg.append(time.perf_counter() - t)
+        t_png.sort()
+        t_svg.sort()
+        py = max(int(round(8 * 0.8660254 * by / bx)), 1)
+        pixels = bw * 8 * bh * py
+        writers = {"workload": rc["name"], "field": f"{bw}x{bh} blocks of {bx}",
+                   "image": f"{bw * 8}x{bh * py}",
+                   "png_ms": round(t_png[0] * 1e3, 3), "png_median_ms": round(t_png[7] * 1e3, 3),
+                   "png_mpx_per_s": round(pixels / t_png[0] / 1e6, 1),
+                   "svg_ms": round(t_svg[0] * 1e3, 3), "svg_median_ms": round(t_svg[7] * 1e3, 3),
+                   "reference_published_m3pro": {"png_ms": 6.490, "svg_ms": 2.988}}
+
+        cfg = W.config2(bands=bands)
+        every = cfg["sample_every"]
+        g = W.config_gpu(cfg)
+        g.set_work_unit(units)
+        g.set_passes(passes)
+        g.set_draw_unit(dunits)
+        g.set_draw_ahead(ahead)
+        g.attach_field(cfg["block"], cfg["block"])
+        g.advance_sampling(every, every, 1.0)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            g.advance_sampling(frame_every, every, 1.0)
+        bare = time.perf_counter() - t0
+        fr = lgca_amd.Frames(workers)
+        t0 = time.perf_counter()
+        for k in range(frames):
+            g.advance_sampling(frame_every, every, 1.0)
+            fr.write_lattice(g, os.path.join(d, f"vorticity-{k:04}.png"),
+                             os.path.join(d, f"arrows-{k:04}.svg"), 8, 1.6)
+        queued = time.perf_counter() - t0
+        fr.finish()
+        on_disk = time.perf_counter() - t0
+        bwf, bhf, _, _ = g.field_dims()
+        g.close()
+        loop = {"workload": cfg["name"], "frames": frames, "frame_every": frame_every,
+                "writers": workers, "image": f"{bwf * 8}x{bhf * 7}",
+                "steps_only_s": round(bare, 4), "with_frames_s": round(queued, 4),
+                "frames_on_disk_s": round(on_disk, 4),
+                "loop_overhead": round(queued / bare - 1.0, 4)}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"writers": writers, "run_loop": loop}, field
+
+
+def cpu_frames(field):
+    """The oracle's encoders (a plain restatement of src/render.rs and
+    src/png.rs, one thread) on the writers' field, best of 5 each, and the
+    library's files compared with them byte for byte."""
+    import shutil
+    import tempfile
+
+    import lgca_amd
+    from oracle import oracle as O
+
+    ux, uy, solid, bw, bh, bx, by = field
+    t_png, t_svg = [], []
+    for _ in range(5):
+        t = time.perf_counter()
+        png = O.vorticity_png(ux, uy, solid, bw, bh, bx, by, 8, 0.01)
+        t_png.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        svg = O.arrows_svg(ux, uy, solid, bw, bh, bx, by, 1.6, (0.4, 0.0))
+        t_svg.append(time.perf_counter() - t)
+    d = tempfile.mkdtemp(prefix="lgca-frames-")
+    try:
+        lgca_amd.write_vorticity(os.path.join(d, "v.png"), field, 8, 0.01)
+        lgca_amd.write_arrows(os.path.join(d, "a.svg"), field, 1.6, (0.4, 0.0))
+        with open(os.path.join(d, "v.png"), "rb") as f1, open(os.path.join(d, "a.svg"), "rb") as f2:
+            same = f1.read() == png and f2.read() == svg
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"png_ms": round(min(t_png) * 1e3, 3), "svg_ms": round(min(t_svg) * 1e3, 3),
+            "cores": 1, "kind": "port", "verified_bit_exact": same}
+
+
 def serving_leg(n, rate, seed=5):
     """Served stream: Poisson arrivals, log-uniform sizes 256x256..4096x2304,
     steps 500-20000, random plate and drift, HBM budget 200 GB, 32x32-block
@@ -320,6 +437,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--no-transport", action="store_true",
                     help="skip the transport-measure leg (the reference's --measure)")
+    ap.add_argument("--no-frames", action="store_true",
+                    help="skip the frame-output leg")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 512 x 256 leg (keeps profiles to the headline lattice)")
     ap.add_argument("--shear-steps", type=int, default=400, metavar="N",
@@ -436,6 +555,14 @@ def main():
             out["transport_measure"] = transport_leg()
         except Exception as e:
             out["transport_measure"] = {"error": repr(e)}
+    if not args.no_frames:
+        try:
+            out["frames"], frame_field = frames_leg(bands, args.passes, args.cells_per_lane,
+                                                    args.draw_cells_per_lane, args.draw_ahead)
+            if "cpu_baseline" in out:
+                out["cpu_baseline"]["frames"] = cpu_frames(frame_field)
+        except Exception as e:
+            out["frames"] = {"error": repr(e)}
     if args.ensemble:
         try:
             out["ensemble"] = ensemble_leg(args.ensemble, args.ensemble_steps, bands)

@@ -36,6 +36,7 @@ extern "C" {
 #define LG_EINVAL (-1)   /* bad argument (the reference would assert) */
 #define LG_EDEVICE (-2)  /* HIP error, or no device */
 #define LG_ESTATE (-3)   /* call out of order, e.g. read_field before attach_field */
+#define LG_EIO (-4)      /* a frame file could not be written */
 
 typedef struct lg_lattice lg_t;
 
@@ -86,6 +87,10 @@ int lg_advance_sampling(lg_t* g, uint64_t n, uint64_t every, float alpha);
 
 /* GpuLattice::sample_field (src/gpu.rs:710-721). No-op without a field. */
 int lg_sample_field(lg_t* g, float alpha);
+
+/* New: the block grid of the attached field (Field's bw, bh, bx, by,
+ * src/render.rs:21-34), so that callers can size the arrays of lg_read_field. */
+int lg_field_dims(lg_t* g, size_t* bw, size_t* bh, size_t* bx, size_t* by);
 
 /* GpuLattice::read_field (src/gpu.rs:725-736): copy the four f32 arrays out.
  * nblocks must be (w / bx) * (h / by). */
@@ -226,6 +231,76 @@ int lg_sample_field_async(lg_t* g, float alpha);
  * promising first, and keep the fastest. The timed steps are real steps of
  * every member, reported in *used_steps; budget_steps = 0: no limit. */
 int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps);
+
+/* ---- Frame output (host code; no GPU except lg_frames_write_lattice) ------
+ *
+ * The reference's frame writers, byte-identical with its files for the same
+ * field. A field here is bw x bh blocks of bx x by cells, as lg_read_field
+ * returns it: row-major f32 arrays ux, uy, solid of bw * bh values. */
+
+/* Field::vorticity (src/render.rs:122-139): centred differences, one-sided
+ * at the x ends, periodic in y; out holds bw * bh values. */
+int lg_field_vorticity(const float* ux, const float* uy, size_t bw, size_t bh, size_t bx,
+                       size_t by, float* out);
+
+/* Field::mean_velocity (src/render.rs:142-157): mean of ux and uy over the
+ * blocks whose solid fraction is below 0.5, summed in f32 in block order;
+ * (0, 0) when there are none. */
+int lg_field_mean_velocity(const float* ux, const float* uy, const float* solid, size_t n,
+                           float* mx, float* my);
+
+/* The colour scale of write_frame (src/main.rs:219-229): element n' * 98 /
+ * 100 of the ascending |vort| over the n' blocks whose solid fraction is below
+ * 0.12, at least 1e-5, and 1e-3 when n' is 0. LG_EINVAL on a NaN, where the
+ * reference's sort would panic. */
+int lg_vorticity_scale(const float* vort, const float* solid, size_t n, float* wmax);
+
+/* png::deflate (src/png.rs:103-162): one fixed-Huffman block with greedy
+ * LZ77. *len receives the compressed size; the bytes are copied to out when
+ * out is not NULL (LG_EINVAL if cap is smaller than *len). */
+int lg_png_deflate(const uint8_t* data, size_t n, uint8_t* out, size_t cap, size_t* len);
+
+/* png::write_rgb (src/png.rs:174-197): an 8-bit RGB PNG of w x h pixels,
+ * rows top to bottom. LG_EIO names the path when the file cannot be written. */
+int lg_write_png_rgb(const char* path, size_t w, size_t h, const uint8_t* rgb);
+
+/* render::write_vorticity (src/render.rs:176-204): each block is scale
+ * pixels wide and round(scale * sqrt(3)/2 * by / bx) high, coloured by
+ * vorticity / wmax on a blue-white-red map, solid blocks dark. */
+int lg_write_vorticity(const char* path, const float* ux, const float* uy, const float* solid,
+                       size_t bw, size_t bh, size_t bx, size_t by, size_t scale, float wmax);
+
+/* render::write_arrows (src/render.rs:209-267): an SVG arrow per fluid block
+ * of the velocity minus (frame_x, frame_y), bx * scale lattice units per unit
+ * of speed; arrows shorter than 0.35 are left out. */
+int lg_write_arrows(const char* path, const float* ux, const float* uy, const float* solid,
+                    size_t bw, size_t bh, size_t bx, size_t by, float scale, float frame_x,
+                    float frame_y);
+
+/* Frames (src/main.rs:214-281): writer threads behind a queue that holds at
+ * most `workers` frames (1..256), so that a run outpacing its writers waits
+ * for them. Each frame writes the vorticity PNG, scaled by the 98th
+ * percentile above, then the arrow SVG (write_frame, src/main.rs:220-234). */
+typedef struct lg_frames lg_frames_t;
+lg_frames_t* lg_frames_create(size_t workers);
+
+/* Frames::write (src/main.rs:262-268): copy the field and queue the frame,
+ * waiting while the queue is full. Returns LG_EIO with "<png path>: <error>"
+ * of the first frame that failed, if any has; that failure surfaces a frame
+ * or two late, as in the reference. */
+int lg_frames_write(lg_frames_t* fr, const float* ux, const float* uy, const float* solid,
+                    size_t bw, size_t bh, size_t bx, size_t by, const char* png, const char* svg,
+                    size_t scale, float arrow_scale, float mean_x, float mean_y);
+
+/* New: the run loop's frame (src/main.rs:676-688) from a lattice's attached
+ * field: read it (as lg_read_field), take its mean velocity as the arrows'
+ * frame, and queue it as lg_frames_write does. */
+int lg_frames_write_lattice(lg_frames_t* fr, lg_t* g, const char* png, const char* svg,
+                            size_t scale, float arrow_scale);
+
+/* Frames::finish (src/main.rs:271-280): write everything queued, stop the
+ * writers and free the pool; returns the first failure as lg_frames_write. */
+int lg_frames_finish(lg_frames_t* fr);
 
 /* Diagnostics. The generated collision circuit as the device evaluates it,
  * for every state and every draw index k < stride, into out[state * stride +

@@ -21,13 +21,26 @@ HIPCC_FLAGS = [
 ]
 
 
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+              "-Wall", "-Wextra"]
+
+
 def build(verbose: bool = False) -> str:
     subprocess.run([sys.executable, os.path.join(HERE, "gen_collide.py"),
                     "--out", os.path.join(CSRC, "lgca_collide.gen.h")], check=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB + ".tmp"
     srcs = [os.path.join(CSRC, f) for f in ("lgca_engine.hip", "lgca_batch.hip")]
-    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + srcs
+    # The frame writers are host-only C++: built by the host compiler into an
+    # object that hipcc links into the same library.
+    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    obj = os.path.join(HERE, "_build", "lgca_frames.o")
+    r = subprocess.run(["g++"] + HOST_FLAGS + ["-c", "-o", obj, os.path.join(CSRC, "lgca_frames.cpp")],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed for lgca_frames.cpp")
+    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + srcs + ["-x", "none", obj, "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -925,6 +925,13 @@ int lg_sample_field(lg_t* g, float alpha) {
   return LG_OK;
 }
 
+int lg_field_dims(lg_t* g, size_t* bw, size_t* bh, size_t* bx, size_t* by) {
+  if (!g || !bw || !bh || !bx || !by) return fail(LG_EINVAL, "null argument");
+  if (!g->has_field) return fail(LG_ESTATE, "no field attached to this lattice");
+  *bw = g->bw, *bh = g->bh, *bx = g->bx, *by = g->by;
+  return LG_OK;
+}
+
 int lg_read_field(lg_t* g, float* ux, float* uy, float* rho, float* solid, size_t nblocks) {
   if (!g || !ux || !uy || !rho || !solid) return fail(LG_EINVAL, "null argument");
   if (!g->has_field) return fail(LG_ESTATE, "no field attached to this lattice");

@@ -9,17 +9,11 @@
 
 #include "../../include/lgca.h"
 #include "lgca_device.h"
+#include "lgca_error.h"
 
 struct lg_lattice;
 
 namespace lgca_internal {
-
-extern thread_local std::string g_err;
-
-inline int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
 
 #define HIP_TRY(expr)                                                                  \
   do {                                                                                 \

@@ -89,6 +89,19 @@ def load_library(path: str | None = None):
         "lg_measure_viscosity": (i32, [f32, i32, sz, u64, sz, P(f32)]),
         "lg_measure_advection": (i32, [f32, i32, sz, u64, sz, P(f32)]),
         "lg_measure_transport": (i32, [f32, i32, sz, u64, sz, P(f32), P(f32)]),
+        "lg_field_dims": (i32, [vp, P(sz), P(sz), P(sz), P(sz)]),
+        "lg_field_vorticity": (i32, [P(f32), P(f32), sz, sz, sz, sz, P(f32)]),
+        "lg_field_mean_velocity": (i32, [P(f32), P(f32), P(f32), sz, P(f32), P(f32)]),
+        "lg_vorticity_scale": (i32, [P(f32), P(f32), sz, P(f32)]),
+        "lg_png_deflate": (i32, [P(C.c_uint8), sz, P(C.c_uint8), sz, P(sz)]),
+        "lg_write_png_rgb": (i32, [C.c_char_p, sz, sz, P(C.c_uint8)]),
+        "lg_write_vorticity": (i32, [C.c_char_p, P(f32), P(f32), P(f32), sz, sz, sz, sz, sz, f32]),
+        "lg_write_arrows": (i32, [C.c_char_p, P(f32), P(f32), P(f32), sz, sz, sz, sz, f32, f32, f32]),
+        "lg_frames_create": (vp, [sz]),
+        "lg_frames_write": (i32, [vp, P(f32), P(f32), P(f32), sz, sz, sz, sz, C.c_char_p,
+                                  C.c_char_p, sz, f32, f32, f32]),
+        "lg_frames_write_lattice": (i32, [vp, vp, C.c_char_p, C.c_char_p, sz, f32]),
+        "lg_frames_finish": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -248,6 +261,12 @@ class GpuLattice:
             raise LgcaError("no field attached to this lattice")
         return self._field[2] * self._field[3]
 
+    def field_dims(self):
+        """(bw, bh, bx, by) of the attached field."""
+        v = [C.c_size_t() for _ in range(4)]
+        _check(load_library().lg_field_dims(self._p, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
     # src/gpu.rs:725-736 (returns the four arrays instead of filling a Field)
     def read_field(self):
         n = self._nblocks()
@@ -370,6 +389,121 @@ def measure_transport(density: float, rest: bool, threads: int, seed: int, size:
                                                int(seed) & (2**64 - 1), int(size),
                                                C.byref(nu), C.byref(g)))
     return nu.value, g.value
+
+
+# ---- frame output (src/render.rs:121-266, src/png.rs, src/main.rs:196-281) ----
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _field_arrays(*arrs):
+    return [np.ascontiguousarray(a, dtype=np.float32).ravel() for a in arrs]
+
+
+def _path(p) -> bytes:
+    return os.fsencode(p)
+
+
+def field_vorticity(ux, uy, bw, bh, bx, by) -> np.ndarray:
+    """Field::vorticity (src/render.rs:122-139)."""
+    ux, uy = _field_arrays(ux, uy)
+    out = np.zeros(bw * bh, dtype=np.float32)
+    _check(load_library().lg_field_vorticity(_fptr(ux), _fptr(uy), bw, bh, bx, by, _fptr(out)))
+    return out
+
+
+def field_mean_velocity(ux, uy, solid):
+    """Field::mean_velocity (src/render.rs:142-157)."""
+    ux, uy, solid = _field_arrays(ux, uy, solid)
+    mx, my = C.c_float(), C.c_float()
+    _check(load_library().lg_field_mean_velocity(_fptr(ux), _fptr(uy), _fptr(solid), ux.size,
+                                                 C.byref(mx), C.byref(my)))
+    return mx.value, my.value
+
+
+def vorticity_scale(vort, solid) -> float:
+    """The colour scale of write_frame (src/main.rs:219-229)."""
+    vort, solid = _field_arrays(vort, solid)
+    w = C.c_float()
+    _check(load_library().lg_vorticity_scale(_fptr(vort), _fptr(solid), vort.size, C.byref(w)))
+    return w.value
+
+
+def png_deflate(data) -> bytes:
+    """png::deflate (src/png.rs:103-162)."""
+    a = np.ascontiguousarray(np.frombuffer(bytes(data), dtype=np.uint8))
+    n = C.c_size_t()
+    L = load_library()
+    _check(L.lg_png_deflate(_u8(a), a.size, None, 0, C.byref(n)))
+    out = np.zeros(max(n.value, 1), dtype=np.uint8)
+    _check(L.lg_png_deflate(_u8(a), a.size, _u8(out), out.size, C.byref(n)))
+    return out[:n.value].tobytes()
+
+
+def write_png_rgb(path, rgb: np.ndarray):
+    """png::write_rgb (src/png.rs:174-197) of an (h, w, 3) u8 image."""
+    img = np.ascontiguousarray(rgb, dtype=np.uint8)
+    h, w = img.shape[:2]
+    _check(load_library().lg_write_png_rgb(_path(path), w, h, _u8(img)))
+
+
+def write_vorticity(path, field, scale: int, wmax: float):
+    """render::write_vorticity (src/render.rs:176-204). `field` is (ux, uy,
+    solid, bw, bh, bx, by)."""
+    ux, uy, solid, bw, bh, bx, by = field
+    ux, uy, solid = _field_arrays(ux, uy, solid)
+    _check(load_library().lg_write_vorticity(_path(path), _fptr(ux), _fptr(uy), _fptr(solid),
+                                             bw, bh, bx, by, int(scale), float(wmax)))
+
+
+def write_arrows(path, field, scale: float, frame):
+    """render::write_arrows (src/render.rs:209-267)."""
+    ux, uy, solid, bw, bh, bx, by = field
+    ux, uy, solid = _field_arrays(ux, uy, solid)
+    _check(load_library().lg_write_arrows(_path(path), _fptr(ux), _fptr(uy), _fptr(solid),
+                                          bw, bh, bx, by, float(scale), float(frame[0]),
+                                          float(frame[1])))
+
+
+class Frames:
+    """Frames (src/main.rs:214-281): writer threads in the library, behind a
+    queue of at most `workers` frames."""
+
+    def __init__(self, workers: int):
+        self._p = load_library().lg_frames_create(int(workers))
+        if not self._p:
+            raise LgcaError(load_library().lg_last_error().decode())
+
+    def write(self, field, png, svg, scale: int, arrow_scale: float, mean):
+        """Frames::write (src/main.rs:262-268); `field` as for write_vorticity."""
+        ux, uy, solid, bw, bh, bx, by = field
+        ux, uy, solid = _field_arrays(ux, uy, solid)
+        _check(load_library().lg_frames_write(self._p, _fptr(ux), _fptr(uy), _fptr(solid), bw, bh,
+                                              bx, by, _path(png), _path(svg), int(scale),
+                                              float(arrow_scale), float(mean[0]), float(mean[1])))
+
+    def write_lattice(self, lattice: "GpuLattice", png, svg, scale: int, arrow_scale: float):
+        """The run loop's frame (src/main.rs:676-688) from the lattice's field."""
+        _check(load_library().lg_frames_write_lattice(self._p, lattice._p, _path(png), _path(svg),
+                                                      int(scale), float(arrow_scale)))
+
+    def finish(self):
+        """Frames::finish (src/main.rs:271-280)."""
+        p, self._p = self._p, None
+        if p:
+            _check(load_library().lg_frames_finish(p))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.finish()
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            load_library().lg_frames_finish(self._p)
+            self._p = None
 
 
 def measure_viscosity(density, rest, threads, seed, size=256) -> float:

@@ -74,6 +74,22 @@ def config2(bands: int = 1) -> dict:
                 sample_every=100)
 
 
+def render_bench(bands: int = 1) -> dict:
+    """The field the reference times its frame writers on (benches/speed.rs:
+    330-357, 633-657): 2048 x 1280, rest particles, density 0.22 at drift 0.4
+    (also the 8 inflow columns'), seed 0x5EED, a plate 4 wide at x = w / 5 and
+    h * 173 / 1280 tall about yc, 50 steps, blocks of 20 sampled at alpha 1."""
+    w, h = 2048, 1280
+    yc = f32(f32(h) * SQRT3_2 / f32(2.0))
+    size = f32(f32(h) * f32(173.0) / f32(1280.0))
+    x0 = f32(f32(w) / f32(5.0))
+    half = f32(size / f32(2.0))
+    plate = (float(x0), float(f32(x0 + f32(4.0))), float(f32(yc - half)), float(f32(yc + half)))
+    return dict(name="fhp-2048x1280-render", w=w, h=h, rest=True, density=0.22,
+                init_u=(0.4, 0.0), inflow=(0.4, 0.0), inlet=8, seed=0x5EED, bands=bands,
+                plate=plate, steps=50, block=20)
+
+
 def shear_profile(w: int, amp: float = 0.05):
     """u_y(x) = amp * sin(2 pi x / W) in f32, as src/transport.rs:234-242
     evaluates its column profile (k = 2 pi / W in f32, then k * x)."""

@@ -1,7 +1,7 @@
 // CPU oracle for the FHP lattice-gas step: a line-by-line restatement of the
 // reference's byte-per-cell CPU path.
 //
-// TEST INFRASTRUCTURE ONLY. This file is the checker that the HIP engine is
+// TEST INFRASTRUCTURE ONLY. This file is what the HIP engine's results are
 // compared against; it is never part of the product. Only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
 //

@@ -1,9 +1,11 @@
 // C ABI over the CPU oracle, for the Python tests and the CPU baseline.
 //
 // TEST INFRASTRUCTURE ONLY: the checker, never the thing measured or shipped.
-// Each entry point forwards to the restatement in lgca_oracle.hpp, which cites
-// the reference file:line it follows.
+// Each entry point forwards to the restatement in lgca_oracle.hpp (the
+// lattice), lgca_transport.hpp (transport measurement) or lgca_render.hpp
+// (frame output), which cite the reference file:line they follow.
 #include "lgca_oracle.hpp"
+#include "lgca_render.hpp"
 #include "lgca_transport.hpp"
 
 using namespace oracle;
@@ -169,6 +171,66 @@ void orc_row_ux(const uint8_t* cells, size_t w, size_t h, float* out) {
 void orc_column_uy(const uint8_t* cells, size_t w, size_t h, float* out) {
   auto u = column_uy(cells, w, h);
   std::memcpy(out, u.data(), w * 4);
+}
+
+// ---- frame output (lgca_render.hpp) --------------------------------------
+// The byte outputs are returned by size: the call copies them to `out` when
+// `cap` is large enough and always returns the full size.
+static size_t copy_out(const void* src, size_t n, void* out, size_t cap) {
+  if (out && cap >= n) std::memcpy(out, src, n);
+  return n;
+}
+
+// Field::vorticity (src/render.rs:122-139)
+void orc_vorticity(const float* ux, const float* uy, size_t bw, size_t bh, size_t bx, size_t by,
+                   float* out) {
+  auto w = oracle_render::vorticity(bw, bh, bx, by, ux, uy);
+  std::memcpy(out, w.data(), w.size() * 4);
+}
+// write_frame's colour scale (src/main.rs:219-229)
+float orc_vorticity_scale(const float* vort, const float* solid, size_t n) {
+  return oracle_render::vorticity_scale(std::vector<float>(vort, vort + n), solid);
+}
+// Field::mean_velocity (src/render.rs:142-157)
+void orc_field_mean_velocity(const float* ux, const float* uy, const float* solid, size_t n,
+                             float* mx, float* my) {
+  oracle_render::mean_velocity(ux, uy, solid, n, mx, my);
+}
+// png::deflate (src/png.rs:103-162)
+size_t orc_deflate(const uint8_t* data, size_t n, uint8_t* out, size_t cap) {
+  auto z = oracle_render::deflate(data, n);
+  return copy_out(z.data(), z.size(), out, cap);
+}
+// png::crc32, png::adler32 (src/png.rs:4-22)
+uint32_t orc_crc32(const uint8_t* d, size_t n) { return oracle_render::crc32(d, n); }
+uint32_t orc_adler32(const uint8_t* d, size_t n) { return oracle_render::adler32(d, n); }
+// png::write_rgb (src/png.rs:174-197), the file's bytes
+size_t orc_png_rgb(size_t w, size_t h, const uint8_t* rgb, uint8_t* out, size_t cap) {
+  auto p = oracle_render::png_rgb(w, h, rgb);
+  return copy_out(p.data(), p.size(), out, cap);
+}
+// render::write_vorticity (src/render.rs:176-204), the file's bytes
+size_t orc_vorticity_png(const float* ux, const float* uy, const float* solid, size_t bw,
+                         size_t bh, size_t bx, size_t by, size_t scale, float wmax, uint8_t* out,
+                         size_t cap) {
+  auto p = oracle_render::vorticity_png(bw, bh, bx, by, ux, uy, solid, scale, wmax);
+  return copy_out(p.data(), p.size(), out, cap);
+}
+// render::write_arrows (src/render.rs:209-267), the file's bytes
+size_t orc_arrows_svg(const float* ux, const float* uy, const float* solid, size_t bw, size_t bh,
+                      size_t bx, size_t by, float scale, float fx, float fy, char* out,
+                      size_t cap) {
+  auto s = oracle_render::arrows_svg(bw, bh, bx, by, ux, uy, solid, scale, fx, fy);
+  return copy_out(s.data(), s.size(), out, cap);
+}
+// Rust's `{:.1}` and `{}` of an f32
+size_t orc_fmt1(float x, char* out, size_t cap) {
+  auto s = oracle_render::fmt1(x);
+  return copy_out(s.c_str(), s.size() + 1, out, cap);
+}
+size_t orc_fmt_short(float x, char* out, size_t cap) {
+  auto s = oracle_render::fmt_short(x);
+  return copy_out(s.c_str(), s.size() + 1, out, cap);
 }
 
 }  // extern "C"

@@ -2,8 +2,8 @@
 // the decay of a transverse shear wave, advection factor from the phase drift
 // of a wave carried by a uniform stream.
 //
-// TEST INFRASTRUCTURE ONLY (the checker). f32 expressions are evaluated in
-// source order with no contraction; f32 sin/cos are the C library's sinf/cosf
+// TEST INFRASTRUCTURE ONLY: what results are compared with. f32 expressions
+// are evaluated in source order with no contraction; f32 sin/cos are the C library's sinf/cosf
 // (the reference calls f32::sin/cos, i.e. the platform libm); f64 as in Rust.
 #pragma once
 #include <cmath>

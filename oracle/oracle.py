@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/lgca_oracle.hpp).
 
-TEST INFRASTRUCTURE ONLY. The oracle is the checker the HIP engine is compared
-with; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+TEST INFRASTRUCTURE ONLY. The oracle is what the HIP engine's results are
+compared with; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
 this module, and never as the thing measured or shipped.
 
 The wrapper mirrors the reference's own types: ``Lattice`` restates
@@ -75,6 +75,19 @@ def lib():
             "orc_advection_factor": (f32, [f32, i32, sz, u64, sz]),
             "orc_row_ux": (None, [P(C.c_uint8), sz, sz, P(f32)]),
             "orc_column_uy": (None, [P(C.c_uint8), sz, sz, P(f32)]),
+            "orc_vorticity": (None, [P(f32), P(f32), sz, sz, sz, sz, P(f32)]),
+            "orc_vorticity_scale": (f32, [P(f32), P(f32), sz]),
+            "orc_field_mean_velocity": (None, [P(f32), P(f32), P(f32), sz, P(f32), P(f32)]),
+            "orc_deflate": (sz, [P(C.c_uint8), sz, P(C.c_uint8), sz]),
+            "orc_crc32": (C.c_uint32, [P(C.c_uint8), sz]),
+            "orc_adler32": (C.c_uint32, [P(C.c_uint8), sz]),
+            "orc_png_rgb": (sz, [sz, sz, P(C.c_uint8), P(C.c_uint8), sz]),
+            "orc_vorticity_png": (sz, [P(f32), P(f32), P(f32), sz, sz, sz, sz, sz, f32,
+                                       P(C.c_uint8), sz]),
+            "orc_arrows_svg": (sz, [P(f32), P(f32), P(f32), sz, sz, sz, sz, f32, f32, f32,
+                                    C.c_char_p, sz]),
+            "orc_fmt1": (sz, [f32, C.c_char_p, sz]),
+            "orc_fmt_short": (sz, [f32, C.c_char_p, sz]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -115,6 +128,93 @@ def rng_u64(seed: int, n: int) -> np.ndarray:
     out = np.zeros(n, dtype=np.uint64)
     lib().orc_rng_u64(seed, n, _ptr(out, C.c_uint64))
     return out
+
+
+# ---- frame output (oracle/lgca_render.hpp) -----------------------------------
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32).ravel()
+
+
+def _bytes_call(fn, *args) -> bytes:
+    n = fn(*args, None, 0)
+    out = C.create_string_buffer(max(n, 1))
+    fn(*args, C.cast(out, fn.argtypes[-2]), n)
+    return out.raw[:n]
+
+
+def vorticity(ux, uy, bw, bh, bx, by) -> np.ndarray:
+    """Field::vorticity (src/render.rs:122-139)."""
+    ux, uy = _f32(ux), _f32(uy)
+    out = np.zeros(bw * bh, dtype=np.float32)
+    lib().orc_vorticity(_ptr(ux, C.c_float), _ptr(uy, C.c_float), bw, bh, bx, by, _ptr(out, C.c_float))
+    return out
+
+
+def vorticity_scale(vort, solid) -> float:
+    """write_frame's colour scale (src/main.rs:219-229)."""
+    v, s = _f32(vort), _f32(solid)
+    return float(lib().orc_vorticity_scale(_ptr(v, C.c_float), _ptr(s, C.c_float), v.size))
+
+
+def field_mean_velocity(ux, uy, solid):
+    """Field::mean_velocity (src/render.rs:142-157)."""
+    ux, uy, s = _f32(ux), _f32(uy), _f32(solid)
+    mx, my = C.c_float(), C.c_float()
+    lib().orc_field_mean_velocity(_ptr(ux, C.c_float), _ptr(uy, C.c_float), _ptr(s, C.c_float),
+                                  ux.size, C.byref(mx), C.byref(my))
+    return mx.value, my.value
+
+
+def _u8buf(data) -> np.ndarray:
+    return np.ascontiguousarray(np.frombuffer(bytes(data), dtype=np.uint8))
+
+
+def deflate(data) -> bytes:
+    """png::deflate (src/png.rs:103-162)."""
+    a = _u8buf(data)
+    return _bytes_call(lib().orc_deflate, _ptr(a, C.c_uint8), a.size)
+
+
+def crc32(data) -> int:
+    a = _u8buf(data)
+    return int(lib().orc_crc32(_ptr(a, C.c_uint8), a.size))
+
+
+def adler32(data) -> int:
+    a = _u8buf(data)
+    return int(lib().orc_adler32(_ptr(a, C.c_uint8), a.size))
+
+
+def png_rgb(rgb: np.ndarray) -> bytes:
+    """png::write_rgb (src/png.rs:174-197) of an (h, w, 3) u8 image: the file's bytes."""
+    img = np.ascontiguousarray(rgb, dtype=np.uint8)
+    h, w = img.shape[:2]
+    return _bytes_call(lib().orc_png_rgb, w, h, _ptr(img, C.c_uint8))
+
+
+def vorticity_png(ux, uy, solid, bw, bh, bx, by, scale, wmax) -> bytes:
+    """render::write_vorticity (src/render.rs:176-204): the file's bytes."""
+    ux, uy, s = _f32(ux), _f32(uy), _f32(solid)
+    return _bytes_call(lib().orc_vorticity_png, _ptr(ux, C.c_float), _ptr(uy, C.c_float),
+                       _ptr(s, C.c_float), bw, bh, bx, by, scale, wmax)
+
+
+def arrows_svg(ux, uy, solid, bw, bh, bx, by, scale, frame) -> bytes:
+    """render::write_arrows (src/render.rs:209-267): the file's bytes."""
+    ux, uy, s = _f32(ux), _f32(uy), _f32(solid)
+    return _bytes_call(lib().orc_arrows_svg, _ptr(ux, C.c_float), _ptr(uy, C.c_float),
+                       _ptr(s, C.c_float), bw, bh, bx, by, scale, frame[0], frame[1])
+
+
+def fmt1(x) -> str:
+    """Rust's `{:.1}` of an f32."""
+    return _bytes_call(lib().orc_fmt1, float(np.float32(x))).rstrip(b"\0").decode()
+
+
+def fmt_short(x) -> str:
+    """Rust's `{}` of an f32."""
+    return _bytes_call(lib().orc_fmt_short, float(np.float32(x))).rstrip(b"\0").decode()
 
 
 def collision_table(rest: bool):
