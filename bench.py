@@ -354,10 +354,11 @@ def frames_leg(bands, passes, units, dunits, ahead, workers=4, frames=10, frame_
         queued = time.perf_counter() - t0
         fr.finish()
         on_disk = time.perf_counter() - t0
-        bwf, bhf, _, _ = g.field_dims()
+        bwf, bhf, bxf, byf = g.field_dims()
         g.close()
+        pyf = max(int(round(8 * 0.8660254 * byf / bxf)), 1)
         loop = {"workload": cfg["name"], "frames": frames, "frame_every": frame_every,
-                "writers": workers, "image": f"{bwf * 8}x{bhf * 7}",
+                "writers": workers, "image": f"{bwf * 8}x{bhf * pyf}",
                 "steps_only_s": round(bare, 4), "with_frames_s": round(queued, 4),
                 "frames_on_disk_s": round(on_disk, 4),
                 "loop_overhead": round(queued / bare - 1.0, 4)}

@@ -110,15 +110,8 @@ lg_step2_kernel(LgGeom G, const uint64_t* __restrict src, uint64_t* __restrict d
                 const uint32_t* __restrict kcur, const uint32_t* __restrict icur,
                 uint32_t* __restrict knext, uint32_t* __restrict inext, uint32_t cblocks,
                 uint32_t dblocks) {
-  const uint32_t m = min(cblocks, dblocks);
-  uint32_t b = blockIdx.x, draw;
-  if (b < 2 * m) {
-    draw = b & 1u;
-    b >>= 1;
-  } else {
-    draw = cblocks > dblocks ? 0u : 1u;
-    b -= m;
-  }
+  uint32_t b, draw;
+  lg_step2_role(blockIdx.x, cblocks, dblocks, &draw, &b);
   const size_t kstride = 2 * (size_t)G.total, istride = 2 * (size_t)G.iw * G.h;
   const uint32_t u = b * blockDim.x + threadIdx.x;
   if (draw) {

@@ -54,6 +54,10 @@ void* hs_new(uint32_t w, uint32_t h, int rest, uint64_t seed, uint32_t bands, ui
 void hs_free(void* p) { delete static_cast<Sim*>(p); }
 void hs_set_units(void* p, uint32_t units) { static_cast<Sim*>(p)->units = units; }
 void hs_set_passes(void* p, uint32_t passes) { static_cast<Sim*>(p)->passes = passes; }
+// lg_step2_role for blocks 0..cblocks+dblocks-1 into draw[], rb[]
+void hs_step2_roles(uint32_t cblocks, uint32_t dblocks, uint32_t* draw, uint32_t* rb) {
+  for (uint32_t b = 0; b < cblocks + dblocks; ++b) lg_step2_role(b, cblocks, dblocks, &draw[b], &rb[b]);
+}
 void hs_set_draw_units(void* p, uint32_t units) { static_cast<Sim*>(p)->draw_units = units; }
 
 void hs_load(void* p, const uint8_t* cells) {

@@ -30,6 +30,7 @@ def hs():
     L.hs_jump_state.argtypes = [C.c_uint64, C.c_uint64]
     L.hs_mulz_check.restype = C.c_int
     L.hs_mulz_check.argtypes = [C.c_uint64, C.c_int]
+    L.hs_step2_roles.argtypes = [C.c_uint32, C.c_uint32, P(C.c_uint32), P(C.c_uint32)]
     return L
 
 
@@ -37,6 +38,27 @@ def test_table_multiplication_by_z_powers(hs):
     """The byte-table c(z) z^f mod P used for the jump tables equals the
     bit-serial product."""
     assert hs.hs_mulz_check(12345, 200000) == 0
+
+
+@pytest.mark.parametrize("cblocks,dblocks", [(576, 576), (576, 1152), (1152, 576), (1, 1), (0, 5),
+                                             (5, 0), (7, 9), (13, 8), (8, 8), (17, 16), (100, 3)])
+def test_two_pass_block_roles(hs, cblocks, dblocks):
+    """The launch's block -> (role, block of that role) map of the two-pass
+    step covers every collide and draw block once, and where both roles have
+    blocks left every run of eight consecutive blocks (one per XCD) has one
+    role and the next run the other."""
+    n = cblocks + dblocks
+    draw = np.zeros(max(n, 1), dtype=np.uint32)
+    rb = np.zeros(max(n, 1), dtype=np.uint32)
+    hs.hs_step2_roles(cblocks, dblocks, draw.ctypes.data_as(C.POINTER(C.c_uint32)),
+                      rb.ctypes.data_as(C.POINTER(C.c_uint32)))
+    draw, rb = draw[:n], rb[:n]
+    assert sorted(rb[draw == 0]) == list(range(cblocks))
+    assert sorted(rb[draw == 1]) == list(range(dblocks))
+    paired = 2 * (min(cblocks, dblocks) // 8 * 8)
+    for x in range(8):  # blocks of XCD x alternate roles over the paired part
+        roles = draw[x:paired:8]
+        assert (roles[1:] != roles[:-1]).all()
 
 
 def u8p(a):
