@@ -309,12 +309,10 @@ int build_tiles(lg_batch* b, DevBuf<uint2>& buf, uint32_t& n, std::vector<uint2>
   if (b->passes == 2) {
     std::vector<uint2> mixed;
     mixed.reserve(t.size() + draws.size());
-    // Runs of eight tiles of each role in turn: blocks go to the eight XCDs
-    // in turn, so each XCD takes collide and draw tiles alike.
     size_t a = 0, d = 0;
     while (a < t.size() || d < draws.size()) {
-      for (int r = 0; r < 8 && a < t.size(); ++r) mixed.push_back(t[a++]);
-      for (int r = 0; r < 8 && d < draws.size(); ++r) mixed.push_back(draws[d++]);
+      if (a < t.size()) mixed.push_back(t[a++]);
+      if (d < draws.size()) mixed.push_back(draws[d++]);
     }
     t.swap(mixed);
   }
