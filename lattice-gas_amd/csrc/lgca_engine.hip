@@ -1211,46 +1211,87 @@ struct Projector {
   }
 };
 
-// averaged() of src/transport.rs:131-171 over the engine.
+struct Handle {
+  lg_t* g = nullptr;
+  ~Handle() { lg_destroy(g); }
+};
+
+struct BatchHandle {
+  lg_batch_t* b = nullptr;
+  ~BatchHandle() { lg_batch_destroy(b); }
+};
+
+// averaged() of src/transport.rs:131-171 over the engine. decay_window runs
+// realisation 0 alone, as a batch of one (its chunks of steps replay as a
+// graph: the 256 x 256 box is far too small for launch latency not to
+// dominate); the REPS trajectories then run together as one batch, each
+// member exactly as it would run alone. The samples are summed in the
+// reference's order (realisation by realisation), and the advance after the
+// last sample, which no result reads, is left out.
 template <class StartFn>
-int measure_mode(lg_t* g, Projector& P, const StartFn& start, std::vector<double>& times,
+int measure_mode(lg_t* const* hs, Projector& P, const StartFn& start, std::vector<double>& times,
                  std::vector<std::pair<double, double>>& acc) {
   int rc;
-  Realisation r0 = start(0);
-  if ((rc = lg_restart(g, r0.cells.data(), r0.cells.size(), r0.seed))) return rc;
-  if ((rc = lg_advance(g, TR_TRANSIENT))) return rc;
   std::pair<double, double> ab;
-  if ((rc = P.project(&ab))) return rc;
-  double a0 = std::max(std::sqrt(ab.first * ab.first + ab.second * ab.second), 1e-12);
   uint64_t t = 0;
-  while (t < TR_MAX_WINDOW) {
-    if ((rc = lg_advance(g, 25))) return rc;
-    t += 25;
+  {
+    Realisation r0 = start(0);
+    if ((rc = lg_restart(hs[0], r0.cells.data(), r0.cells.size(), r0.seed))) return rc;
+    BatchHandle B;
+    B.b = lg_batch_create(hs, 1);
+    if (!B.b) return LG_EDEVICE;
+    if ((rc = lg_batch_advance(B.b, TR_TRANSIENT))) return rc;
+    P.g = hs[0];
     if ((rc = P.project(&ab))) return rc;
-    if (std::sqrt(ab.first * ab.first + ab.second * ab.second) < a0 / 3.0) break;
+    double a0 = std::max(std::sqrt(ab.first * ab.first + ab.second * ab.second), 1e-12);
+    while (t < TR_MAX_WINDOW) {
+      if ((rc = lg_batch_advance(B.b, 25))) return rc;
+      t += 25;
+      if ((rc = P.project(&ab))) return rc;
+      if (std::sqrt(ab.first * ab.first + ab.second * ab.second) < a0 / 3.0) break;
+    }
   }
   uint64_t window = std::min(std::max(t, (uint64_t)TR_SAMPLES * TR_MIN_INTERVAL), TR_MAX_WINDOW);
   uint64_t interval = std::max(window / (uint64_t)TR_SAMPLES, TR_MIN_INTERVAL);
-  acc.assign(TR_SAMPLES, {0.0, 0.0});
   for (uint64_t rep = 0; rep < TR_REPS; ++rep) {
     Realisation r = start(rep);
-    if ((rc = lg_restart(g, r.cells.data(), r.cells.size(), r.seed))) return rc;
-    if ((rc = lg_advance(g, TR_TRANSIENT))) return rc;
-    for (size_t i = 0; i < TR_SAMPLES; ++i) {
-      if ((rc = P.project(&ab))) return rc;
-      acc[i].first += ab.first / (double)TR_REPS;
-      acc[i].second += ab.second / (double)TR_REPS;
-      if ((rc = lg_advance(g, interval))) return rc;
-    }
+    if ((rc = lg_restart(hs[rep], r.cells.data(), r.cells.size(), r.seed))) return rc;
   }
+  BatchHandle B;
+  B.b = lg_batch_create(hs, TR_REPS);
+  if (!B.b) return LG_EDEVICE;
+  if ((rc = lg_batch_advance(B.b, TR_TRANSIENT))) return rc;
+  std::vector<std::pair<double, double>> traj(TR_REPS * TR_SAMPLES);
+  for (size_t i = 0; i < TR_SAMPLES; ++i) {
+    for (uint64_t rep = 0; rep < TR_REPS; ++rep) {
+      P.g = hs[rep];
+      if ((rc = P.project(&traj[rep * TR_SAMPLES + i]))) return rc;
+    }
+    if (i + 1 < TR_SAMPLES && (rc = lg_batch_advance(B.b, interval))) return rc;
+  }
+  acc.assign(TR_SAMPLES, {0.0, 0.0});
+  for (uint64_t rep = 0; rep < TR_REPS; ++rep)
+    for (size_t i = 0; i < TR_SAMPLES; ++i) {
+      acc[i].first += traj[rep * TR_SAMPLES + i].first / (double)TR_REPS;
+      acc[i].second += traj[rep * TR_SAMPLES + i].second / (double)TR_REPS;
+    }
   times.resize(TR_SAMPLES);
   for (size_t i = 0; i < TR_SAMPLES; ++i) times[i] = (double)(i * interval);
   return LG_OK;
 }
 
-struct Handle {
-  lg_t* g = nullptr;
-  ~Handle() { lg_destroy(g); }
+// REPS handles for one measurement: quiescent periodic boxes (box_of,
+// src/transport.rs:176-179).
+struct Boxes {
+  Handle h[TR_REPS];
+  lg_t* g[TR_REPS] = {};
+  int create(size_t size, int rest, size_t threads) {
+    for (uint64_t i = 0; i < TR_REPS; ++i) {
+      h[i].g = g[i] = lg_create(size, size, rest, 1, threads);
+      if (!g[i]) return LG_EDEVICE;
+    }
+    return LG_OK;
+  }
 };
 
 }  // namespace
@@ -1276,15 +1317,14 @@ int lg_measure_viscosity(float density, int rest, size_t threads, uint64_t seed,
     }
     return r;
   };
-  Handle H;
-  H.g = lg_create(size, size, rest, 1, threads);  // box_of, src/transport.rs:176-179
-  if (!H.g) return LG_EDEVICE;
-  Projector P{H.g, true, k, {}, {}};
-  int rc = P.init();
+  Boxes X;
+  int rc = X.create(size, rest, threads);
   if (rc) return rc;
+  Projector P{X.g[0], true, k, {}, {}};
+  if ((rc = P.init())) return rc;
   std::vector<double> times;
   std::vector<std::pair<double, double>> mode;
-  if ((rc = measure_mode(H.g, P, start, times, mode))) return rc;
+  if ((rc = measure_mode(X.g, P, start, times, mode))) return rc;
   std::vector<std::pair<double, double>> pts;
   for (size_t i = 0; i < times.size(); ++i) {
     double a = mode[i].first, b = mode[i].second;
@@ -1330,15 +1370,14 @@ int lg_measure_advection(float density, int rest, size_t threads, uint64_t seed,
     }
     u_actual = mass == 0 ? 0.0f : (float)px * 0.5f / (float)mass;
   }
-  Handle H;
-  H.g = lg_create(size, size, rest, 1, threads);
-  if (!H.g) return LG_EDEVICE;
-  Projector P{H.g, false, k, {}, {}};
-  int rc = P.init();
+  Boxes X;
+  int rc = X.create(size, rest, threads);
   if (rc) return rc;
+  Projector P{X.g[0], false, k, {}, {}};
+  if ((rc = P.init())) return rc;
   std::vector<double> times;
   std::vector<std::pair<double, double>> mode;
-  if ((rc = measure_mode(H.g, P, start, times, mode))) return rc;
+  if ((rc = measure_mode(X.g, P, start, times, mode))) return rc;
   const double PI = 3.14159265358979323846;
   double unwrapped = 0.0, prev = std::atan2(mode[0].first, mode[0].second);
   std::vector<std::pair<double, double>> pts;
