@@ -27,6 +27,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -535,6 +536,19 @@ int check_field(const float* ux, const float* uy, const float* solid, size_t bw,
   return LG_OK;
 }
 
+// Runs f, turning a C++ exception (out of host memory, a thread that cannot
+// start) into an error return: nothing thrown crosses the C ABI.
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return fail(LG_EINVAL, "out of host memory");
+  } catch (const std::exception& e) {
+    return fail(LG_EINVAL, e.what());
+  }
+}
+
 }  // namespace
 
 // ---- the writer pool (src/main.rs:196-281) ------------------------------------
@@ -584,7 +598,7 @@ struct lg_frames {
         queue.pop_front();
       }
       room.notify_one();
-      if (write_frame(*job) != LG_OK) {
+      if (guarded([&] { return write_frame(*job); }) != LG_OK) {
         // Strip the path write_file put in front: the message names the PNG,
         // as the reference's does, whichever of the two files failed.
         std::string msg = lgca_internal::g_err;
@@ -638,41 +652,48 @@ int lg_field_mean_velocity(const float* ux, const float* uy, const float* solid,
 int lg_vorticity_scale(const float* vort, const float* solid, size_t n, float* wmax) {
   if ((!vort || !solid) && n) return fail(LG_EINVAL, "null array");
   if (!wmax) return fail(LG_EINVAL, "null output");
-  return vorticity_scale(vort, solid, n, wmax);
+  return guarded([&] { return vorticity_scale(vort, solid, n, wmax); });
 }
 
 int lg_png_deflate(const uint8_t* data, size_t n, uint8_t* out, size_t cap, size_t* len) {
   if ((!data && n) || !len) return fail(LG_EINVAL, "null argument");
-  std::vector<uint8_t> z;
-  deflate(data, n, z);
-  *len = z.size();
-  if (!out) return LG_OK;
-  if (cap < z.size()) return fail(LG_EINVAL, "output buffer too small (" + std::to_string(z.size()) + " bytes needed)");
-  std::memcpy(out, z.data(), z.size());
-  return LG_OK;
+  return guarded([&] {
+    std::vector<uint8_t> z;
+    deflate(data, n, z);
+    *len = z.size();
+    if (!out) return LG_OK;
+    if (cap < z.size())
+      return fail(LG_EINVAL, "output buffer too small (" + std::to_string(z.size()) + " bytes needed)");
+    std::memcpy(out, z.data(), z.size());
+    return LG_OK;
+  });
 }
 
 int lg_write_png_rgb(const char* path, size_t w, size_t h, const uint8_t* rgb) {
   if (!path || (!rgb && w && h)) return fail(LG_EINVAL, "null argument");
   if (w > 0x7FFFFFFFu || h > 0x7FFFFFFFu) return fail(LG_EINVAL, "image too large for PNG");
-  const size_t stride = 1 + w * 3;
-  Scratch& S = scratch();
-  S.raw.resize(h * stride);
-  for (size_t y = 0; y < h; ++y) {
-    S.raw[y * stride] = 0;
-    if (w) std::memcpy(S.raw.data() + y * stride + 1, rgb + y * w * 3, w * 3);
-  }
-  png_from_scanlines(w, h, S.raw, S.file);
-  return write_file(path, S.file.data(), S.file.size());
+  return guarded([&] {
+    const size_t stride = 1 + w * 3;
+    Scratch& S = scratch();
+    S.raw.resize(h * stride);
+    for (size_t y = 0; y < h; ++y) {
+      S.raw[y * stride] = 0;
+      if (w) std::memcpy(S.raw.data() + y * stride + 1, rgb + y * w * 3, w * 3);
+    }
+    png_from_scanlines(w, h, S.raw, S.file);
+    return write_file(path, S.file.data(), S.file.size());
+  });
 }
 
 int lg_write_vorticity(const char* path, const float* ux, const float* uy, const float* solid,
                        size_t bw, size_t bh, size_t bx, size_t by, size_t scale, float wmax) {
   if (!path) return fail(LG_EINVAL, "null path");
   if (int rc = check_field(ux, uy, solid, bw, bh, bx, by)) return rc;
-  std::vector<float> vort(bw * bh);
-  vorticity(ux, uy, bw, bh, bx, by, vort.data());
-  return vorticity_png(path, vort.data(), solid, bw, bh, bx, by, scale, wmax);
+  return guarded([&] {
+    std::vector<float> vort(bw * bh);
+    vorticity(ux, uy, bw, bh, bx, by, vort.data());
+    return vorticity_png(path, vort.data(), solid, bw, bh, bx, by, scale, wmax);
+  });
 }
 
 int lg_write_arrows(const char* path, const float* ux, const float* uy, const float* solid,
@@ -680,9 +701,11 @@ int lg_write_arrows(const char* path, const float* ux, const float* uy, const fl
                     float frame_y) {
   if (!path) return fail(LG_EINVAL, "null path");
   if (int rc = check_field(ux, uy, solid, bw, bh, bx, by)) return rc;
-  std::string& s = scratch().svg;
-  arrows_svg(s, ux, uy, solid, bw, bh, bx, by, scale, frame_x, frame_y);
-  return write_file(path, s.data(), s.size());
+  return guarded([&] {
+    std::string& s = scratch().svg;
+    arrows_svg(s, ux, uy, solid, bw, bh, bx, by, scale, frame_x, frame_y);
+    return write_file(path, s.data(), s.size());
+  });
 }
 
 lg_frames_t* lg_frames_create(size_t workers) {
@@ -690,7 +713,11 @@ lg_frames_t* lg_frames_create(size_t workers) {
     fail(LG_EINVAL, "writer count must be 1..256");
     return nullptr;
   }
-  auto* fr = new lg_frames;
+  lg_frames* fr = new (std::nothrow) lg_frames;
+  if (!fr) {
+    fail(LG_EINVAL, "out of host memory");
+    return nullptr;
+  }
   fr->bound = workers;
   try {
     for (size_t i = 0; i < workers; ++i) fr->workers.emplace_back([fr] { fr->run(); });
@@ -713,15 +740,17 @@ int lg_frames_write(lg_frames_t* fr, const float* ux, const float* uy, const flo
                     size_t scale, float arrow_scale, float mean_x, float mean_y) {
   if (!fr || !png || !svg) return fail(LG_EINVAL, "null argument");
   if (int rc = check_field(ux, uy, solid, bw, bh, bx, by)) return rc;
-  auto job = std::make_unique<lg_frames::Job>();
-  const size_t n = bw * bh;
-  job->ux.assign(ux, ux + n);
-  job->uy.assign(uy, uy + n);
-  job->solid.assign(solid, solid + n);
-  job->bw = bw, job->bh = bh, job->bx = bx, job->by = by, job->scale = scale;
-  job->arrow_scale = arrow_scale, job->mean_x = mean_x, job->mean_y = mean_y;
-  job->png = png, job->svg = svg;
-  return fr->submit(std::move(job));
+  return guarded([&] {
+    auto job = std::make_unique<lg_frames::Job>();
+    const size_t n = bw * bh;
+    job->ux.assign(ux, ux + n);
+    job->uy.assign(uy, uy + n);
+    job->solid.assign(solid, solid + n);
+    job->bw = bw, job->bh = bh, job->bx = bx, job->by = by, job->scale = scale;
+    job->arrow_scale = arrow_scale, job->mean_x = mean_x, job->mean_y = mean_y;
+    job->png = png, job->svg = svg;
+    return fr->submit(std::move(job));
+  });
 }
 
 int lg_frames_write_lattice(lg_frames_t* fr, lg_t* g, const char* png, const char* svg,
@@ -729,18 +758,20 @@ int lg_frames_write_lattice(lg_frames_t* fr, lg_t* g, const char* png, const cha
   if (!fr || !g || !png || !svg) return fail(LG_EINVAL, "null argument");
   size_t bw, bh, bx, by;
   if (int rc = lg_field_dims(g, &bw, &bh, &bx, &by)) return rc;
-  auto job = std::make_unique<lg_frames::Job>();
-  const size_t n = bw * bh;
-  job->ux.resize(n), job->uy.resize(n), job->solid.resize(n);
-  std::vector<float> rho(n);
-  if (int rc = lg_read_field(g, job->ux.data(), job->uy.data(), rho.data(), job->solid.data(), n))
-    return rc;
-  lg_field_mean_velocity(job->ux.data(), job->uy.data(), job->solid.data(), n, &job->mean_x,
-                         &job->mean_y);
-  job->bw = bw, job->bh = bh, job->bx = bx, job->by = by, job->scale = scale;
-  job->arrow_scale = arrow_scale;
-  job->png = png, job->svg = svg;
-  return fr->submit(std::move(job));
+  return guarded([&] {
+    auto job = std::make_unique<lg_frames::Job>();
+    const size_t n = bw * bh;
+    job->ux.resize(n), job->uy.resize(n), job->solid.resize(n);
+    std::vector<float> rho(n);
+    if (int rc = lg_read_field(g, job->ux.data(), job->uy.data(), rho.data(), job->solid.data(), n))
+      return rc;
+    lg_field_mean_velocity(job->ux.data(), job->uy.data(), job->solid.data(), n, &job->mean_x,
+                           &job->mean_y);
+    job->bw = bw, job->bh = bh, job->bx = bx, job->by = by, job->scale = scale;
+    job->arrow_scale = arrow_scale;
+    job->png = png, job->svg = svg;
+    return fr->submit(std::move(job));
+  });
 }
 
 int lg_frames_finish(lg_frames_t* fr) {
