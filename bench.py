@@ -277,9 +277,17 @@ def transport_leg():
     t0 = time.perf_counter()
     nu, g = lgca_amd.measure_transport(0.22, True, 12, 0x5EED, 256)
     wall = time.perf_counter() - t0
+    # the same procedure at the shear workload's statistics (density 0.3 with
+    # rest particles, 16 bands, seed 7)
+    t0 = time.perf_counter()
+    nu3, g3 = lgca_amd.measure_transport(0.3, True, 16, 7, 256)
+    wall3 = time.perf_counter() - t0
     return {"nu": round(nu, 6), "g": round(g, 6), "wall_s": round(wall, 3),
             "published": {"nu": 0.2989, "g": 0.4405},
-            "matches_published": f"{nu:.4f}" == "0.2989" and f"{g:.4f}" == "0.4405"}
+            "matches_published": f"{nu:.4f}" == "0.2989" and f"{g:.4f}" == "0.4405",
+            "shear_workload_statistics": {"density": 0.3, "bands": 16, "seed": 7,
+                                          "nu": round(nu3, 6), "g": round(g3, 6),
+                                          "wall_s": round(wall3, 3)}}
 
 
 def frames_leg(bands, passes, units, dunits, ahead, workers=4, frames=10, frame_every=500):
