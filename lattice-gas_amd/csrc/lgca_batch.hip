@@ -82,9 +82,13 @@ __global__ void __launch_bounds__(256) lg_tables_batch_kernel(const LgDesc* __re
 
 // Step k of the chunk for every tile. A tile is TILE consecutive thread
 // indices of one member, mapped to work units exactly as lg_step_kernel_{w,h} map
-// them (the member's static word order, NH units per word).
+// them (the member's static word order, NH units per word). Register target:
+// three waves per SIMD, which leaves the tile loop room to keep everything in
+// registers (134 VGPRs, no scratch); at four waves it spills 32 bytes a lane
+// and the 256-lattice ensemble runs 3% slower. (The single-lattice fused
+// kernel is the other way round: 7% slower at three waves.)
 template <int TILE, int NH>
-__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(4))) LG_FLAT lg_step_batch_kernel(const LgDesc* __restrict D,
+__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(3))) LG_FLAT lg_step_batch_kernel(const LgDesc* __restrict D,
                                                              const uint2* __restrict tiles,
                                                              uint32_t ntiles, uint32_t k) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
