@@ -205,8 +205,9 @@ int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits);
  * lg_batch_snapshot_size(b). */
 int lg_batch_snapshot(lg_batch_t* b, float alpha, float* out, size_t nfloats);
 size_t lg_batch_snapshot_size(lg_batch_t* b);
-/* Launch shape: tile = words per workgroup (128 or 256), grid = step-kernel
- * workgroups (0: one per tile; otherwise a persistent grid walking the
+/* Launch shape: tile = words per workgroup (128 or 256; by default 128 for
+ * the fused step, 256 for the two-pass step), grid = step-kernel workgroups
+ * (0, the default: one per tile; otherwise a persistent grid walking the
  * tiles), chunk = steps per graph replay (even, at most 256). */
 int lg_batch_set_shape(lg_batch_t* b, uint32_t tile, uint32_t grid, uint32_t chunk);
 int lg_batch_get_shape(lg_batch_t* b, uint32_t* tile, uint32_t* grid, uint32_t* chunk);
