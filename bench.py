@@ -25,9 +25,11 @@ CPU-runnable case on the GPU; its CPU rates and check sit under
 rate), `ensemble` (256 uneven lattices, batched vs one at a time),
 `serving` (a Poisson request stream, continuous batching vs the plain loop),
 `transport_measure` (the reference's --measure run on the engine: viscosity and
-advection factor with their wall time) and `frames` (the frame writers on the
+advection factor with their wall time), `frames` (the frame writers on the
 reference's render-benchmark field, and what queueing a frame every 500 steps
-to the writer pool costs the headline run's stepping loop).
+to the writer pool costs the headline run's stepping loop) and
+`reference_suite` (the reference's own GPU benchmark cases, same lattices and
+harness, beside its published figures).
 """
 from __future__ import annotations
 
@@ -202,7 +204,8 @@ def small_leg(bands, passes, units, dunits, ahead):
     g.set_work_unit(units)
     g.set_passes(passes)
     g.set_draw_unit(dunits)
-    g.set_draw_ahead(ahead)
+    if ahead >= 0:
+        g.set_draw_ahead(ahead)
     g.attach_field(cfg["block"], cfg["block"])
     t0 = time.perf_counter()
     g.advance_sampling(cfg["steps"], cfg["steps"], 1.0)
@@ -250,7 +253,8 @@ def shear_leg(steps, bands, passes, units, dunits, ahead):
     g.set_work_unit(units)
     g.set_passes(passes)
     g.set_draw_unit(dunits)
-    g.set_draw_ahead(ahead)
+    if ahead >= 0:
+        g.set_draw_ahead(ahead)
     g.advance(2)  # warm-up (builds the tables, first launches)
     t0 = time.perf_counter()
     dev_ms = g.advance_timed(steps)
@@ -346,7 +350,8 @@ def frames_leg(bands, passes, units, dunits, ahead, workers=4, frames=10, frame_
         g.set_work_unit(units)
         g.set_passes(passes)
         g.set_draw_unit(dunits)
-        g.set_draw_ahead(ahead)
+        if ahead >= 0:
+            g.set_draw_ahead(ahead)
         g.attach_field(cfg["block"], cfg["block"])
         g.advance_sampling(every, every, 1.0)  # warm-up
         t0 = time.perf_counter()
@@ -406,6 +411,101 @@ def cpu_frames(field):
             "cores": 1, "kind": "port", "verified_bit_exact": same}
 
 
+# benches/speed.rs:477-591 and their published M3 Pro (Metal) figures,
+# docs/benchmarks.md:298-306: (case, time per iteration in s, Mcell/s).
+REFERENCE_GPU_SUITE = [
+    ("step/gpu/2048x1280/one-per-submit", 183.83e-6, 14260.5),
+    ("step/gpu/2048x1280/batched", 3.434e-3, 76339.7),
+    ("step/gpu/256x256/batched", 834.60e-6, 7852.4),
+    ("step/gpu/4096x2048/batched", 11.109e-3, 75511.8),
+    ("step/gpu/2048x1280/batched+inlet", 3.591e-3, 73009.7),
+    ("step/gpu/2048x1280/batched+inlet+sample", 3.770e-3, 69532.0),
+    ("field/gpu/sample/2048x1280", 150.69e-6, 17396.3),
+    ("lattice/gpu/store/2048x1280", 998.14e-6, 2626.3),
+    ("lattice/gpu/total-particles/2048x1280", 165.75e-6, 15815.9),
+]
+
+
+def _speed_case(body, warm_s=0.12, sample_s=0.15, samples=7):
+    """The harness of benches/speed.rs:128-185: a warm-up, then `samples`
+    samples of about `sample_s` each; the fastest sample's time per
+    iteration, and the median beside it."""
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < warm_s or n == 0:
+        body()
+        n += 1
+    per = (time.perf_counter() - t0) / n
+    iters = max(1, int(sample_s / max(per, 1e-9)))
+    res = []
+    for _ in range(samples):
+        t = time.perf_counter()
+        for _ in range(iters):
+            body()
+        res.append((time.perf_counter() - t) / iters)
+    res.sort()
+    return res[0], res[len(res) // 2]
+
+
+def reference_suite_leg(bands):
+    """The reference's own GPU benchmark cases (benches/speed.rs:477-591) on
+    the engine, with the same lattices (rest particles, equilibrium at
+    density 0.22 and drift 0.4 from seed 0x5EED, periodic unless the case
+    adds the 8-column inflow; 20 x 20 field blocks, a sample every 5 steps at
+    alpha 0.2; 100 steps per timed call for the batched cases) and the same
+    harness, beside the figures the reference publishes for its Metal path on
+    an M3 Pro. The engine draws the CPU path's exact random streams; the
+    reference's Metal kernels draw cheaper ones of their own."""
+    import numpy as np
+
+    from lgca_amd import GpuLattice
+
+    def lattice(w, h, inlet=False, field=False):
+        g = GpuLattice(w, h, True, 0x5EED, bands)
+        g.init_equilibrium(0.22, 0.4, 0.0)
+        if inlet:
+            g.set_inlet(8, 0.22, (0.4, 0.0))
+        if field:
+            g.attach_field(20, 20)
+        return g
+
+    out = []
+    open_handles = []
+
+    def case(name, g, body, cells):
+        best, med = _speed_case(body)
+        _, ref_t, ref_rate = next(c for c in REFERENCE_GPU_SUITE if c[0] == name)
+        rate = cells / best / 1e6
+        out.append({"case": name, "time_s": best, "median_s": med,
+                    "mcells_per_s": round(rate, 1),
+                    "reference_m3pro": {"time_s": ref_t, "mcells_per_s": ref_rate},
+                    "speedup": round(rate / ref_rate, 2)})
+        g.close()
+
+    bw, bh = 2048, 1280
+    g = lattice(bw, bh)
+    case("step/gpu/2048x1280/one-per-submit", g, lambda: g.advance(1), bw * bh)
+    g = lattice(bw, bh)
+    case("step/gpu/2048x1280/batched", g, lambda: g.advance(100), bw * bh * 100)
+    g = lattice(256, 256)
+    case("step/gpu/256x256/batched", g, lambda: g.advance(100), 256 * 256 * 100)
+    g = lattice(4096, 2048)
+    case("step/gpu/4096x2048/batched", g, lambda: g.advance(100), 4096 * 2048 * 100)
+    g = lattice(bw, bh, inlet=True)
+    case("step/gpu/2048x1280/batched+inlet", g, lambda: g.advance(100), bw * bh * 100)
+    g = lattice(bw, bh, inlet=True, field=True)
+    case("step/gpu/2048x1280/batched+inlet+sample", g,
+         lambda: g.advance_sampling(100, 5, 0.2), bw * bh * 100)
+    g = lattice(bw, bh, field=True)
+    case("field/gpu/sample/2048x1280", g, lambda: g.sample_field(0.2), bw * bh)
+    g = lattice(bw, bh)
+    buf = np.zeros(bw * bh, dtype=np.uint8)
+    case("lattice/gpu/store/2048x1280", g, lambda: g.store(buf), bw * bh)
+    g = lattice(bw, bh)
+    case("lattice/gpu/total-particles/2048x1280", g, g.total_particles, bw * bh)
+    return {"bands": bands, "cases": out}
+
+
 def serving_leg(n, rate, seed=5):
     """Served stream: Poisson arrivals, log-uniform sizes 256x256..4096x2304,
     steps 500-20000, random plate and drift, HBM budget 200 GB, 32x32-block
@@ -431,11 +531,11 @@ def main():
     ap.add_argument("--passes", type=int, default=0, choices=(0, 1, 2),
                     help="kernels per step: draw pass + collide pass (2), one fused kernel (1) "
                          "or chosen by lattice size (0)")
-    ap.add_argument("--draw-cells-per-lane", type=int, default=64, choices=(64, 32),
-                    help="work unit of the two-pass step's draw work")
-    ap.add_argument("--draw-ahead", type=int, default=0, metavar="N",
+    ap.add_argument("--draw-cells-per-lane", type=int, default=0, choices=(0, 64, 32),
+                    help="work unit of the two-pass step's draw work (0: by lattice size)")
+    ap.add_argument("--draw-ahead", type=int, default=-1, metavar="N",
                     help="two-pass step: draws of up to N steps per launch on a second stream "
-                         "(0: draws share the collide launches)")
+                         "(0: draws share the collide launches; -1: by lattice size)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--ensemble", type=int, default=256, metavar="N",
@@ -448,6 +548,8 @@ def main():
                     help="skip the transport-measure leg (the reference's --measure)")
     ap.add_argument("--no-frames", action="store_true",
                     help="skip the frame-output leg")
+    ap.add_argument("--no-reference-suite", action="store_true",
+                    help="skip the reference's own GPU benchmark cases")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 512 x 256 leg (keeps profiles to the headline lattice)")
     ap.add_argument("--shear-steps", type=int, default=400, metavar="N",
@@ -488,7 +590,8 @@ def main():
     g.set_work_unit(args.cells_per_lane)
     g.set_passes(args.passes)
     g.set_draw_unit(args.draw_cells_per_lane)
-    g.set_draw_ahead(args.draw_ahead)
+    if args.draw_ahead >= 0:
+        g.set_draw_ahead(args.draw_ahead)
     g.attach_field(cfg["block"], cfg["block"])
     if args.warmup:
         g.advance_sampling(args.warmup, every, 1.0)
@@ -572,6 +675,11 @@ def main():
                 out["cpu_baseline"]["frames"] = cpu_frames(frame_field)
         except Exception as e:
             out["frames"] = {"error": repr(e)}
+    if not args.no_reference_suite:
+        try:
+            out["reference_suite"] = reference_suite_leg(bands)
+        except Exception as e:
+            out["reference_suite"] = {"error": repr(e)}
     if args.ensemble:
         try:
             out["ensemble"] = ensemble_leg(args.ensemble, args.ensemble_steps, bands)

@@ -127,6 +127,8 @@ int lg_set_work_unit(lg_t* g, uint32_t cells_per_lane);
  * whose one-launch step holds too few waves to keep every CU busy to its end,
  * the fused step for larger ones. The results are identical either way. */
 int lg_set_passes(lg_t* g, uint32_t passes);
+/* (lg_set_passes(g, 0) also returns the draw unit and the draw-ahead below to
+ * their size rule.) */
 
 /* The pipeline the next step will use: kernels per step (1 or 2), cells per
  * lane of the collide (or fused) and draw work, draw-ahead steps. */
@@ -137,11 +139,16 @@ int lg_get_pipeline(lg_t* g, uint32_t* passes, uint32_t* cells_per_lane,
  * launch that collides the step before it; N > 0 (at most 64) runs the draws
  * of up to N steps per launch on a second HIP stream, double-buffered ahead of
  * the one-collide-per-step launches on the handle's stream, so the draws fill
- * the gaps of the serial collide chain. Results are identical either way. */
+ * the gaps of the serial collide chain. Results are identical either way.
+ * Until set, it follows the size rule: 8 for lattices below 2^16 words per
+ * plane that have inflow columns (their inflow words' serial draws then
+ * overlap other steps), else 0. */
 int lg_set_draw_ahead(lg_t* g, uint32_t steps);
 
-/* Cells per lane (64, the default, or 32) of the two-pass step's draw work,
- * as lg_set_work_unit sets them for its collide work. Results are identical. */
+/* Cells per lane (64 or 32) of the two-pass step's draw work, as
+ * lg_set_work_unit sets them for its collide work; 0, the default, follows the
+ * size rule: 32 for lattices below 2^16 words per plane without inflow
+ * columns, else 64. Results are identical. */
 int lg_set_draw_unit(lg_t* g, uint32_t cells_per_lane);
 
 /* New: advance n steps and report the device time of the step kernels in ms:

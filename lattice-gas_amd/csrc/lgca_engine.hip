@@ -581,6 +581,7 @@ int enqueue_ahead_group(lg_t* g, uint32_t s0, uint32_t nd) {
 int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) {
   int trc = ensure_tables(g);
   if (trc) return trc;
+  resolve_pipeline(g);
   uint64_t left = n, since = 0;
   const uint32_t per = g->G.nbands + 1;
   if (timed) g->tev_used = 0;
@@ -683,6 +684,12 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
 uint32_t lgca_internal::step_passes(const lg_t* g) {
   if (g->passes) return g->passes;
   return g->G.total >= LG_AUTO_FUSED_WORDS ? 1u : 2u;
+}
+
+void lgca_internal::resolve_pipeline(lg_t* g) {
+  const bool small = g->G.total < LG_SMALL_WORDS, inflow = g->G.inlet_cols > 0;
+  if (g->auto_draw_units) g->draw_units = small && !inflow ? 2u : 1u;
+  if (g->auto_ahead) g->ahead = small && inflow ? 8u : 0u;
 }
 
 int lgca_internal::ensure_tables(lg_t* g) {
@@ -994,14 +1001,16 @@ int lg_set_draw_ahead(lg_t* g, uint32_t steps) {
   if (steps > 64) return fail(LG_EINVAL, "draw-ahead is at most 64 steps");
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->ahead = steps;
+  g->auto_ahead = false;
   return LG_OK;
 }
 
 int lg_set_draw_unit(lg_t* g, uint32_t cells_per_lane) {
   if (!g) return fail(LG_EINVAL, "null handle");
-  if (cells_per_lane != 64 && cells_per_lane != 32)
-    return fail(LG_EINVAL, "cells per lane must be 64 or 32");
-  g->draw_units = cells_per_lane == 64 ? 1u : 2u;
+  if (cells_per_lane != 64 && cells_per_lane != 32 && cells_per_lane != 0)
+    return fail(LG_EINVAL, "cells per lane must be 64, 32 or 0 (by size)");
+  g->auto_draw_units = cells_per_lane == 0;
+  if (cells_per_lane) g->draw_units = cells_per_lane == 64 ? 1u : 2u;
   return LG_OK;
 }
 
@@ -1009,6 +1018,7 @@ int lg_set_passes(lg_t* g, uint32_t passes) {
   if (!g) return fail(LG_EINVAL, "null handle");
   if (passes > 2) return fail(LG_EINVAL, "passes per step must be 0 (by size), 1 or 2");
   g->passes = passes;
+  if (passes == 0) g->auto_draw_units = g->auto_ahead = true;  // the whole pipeline by size
   return LG_OK;
 }
 
@@ -1016,6 +1026,7 @@ int lg_get_pipeline(lg_t* g, uint32_t* passes, uint32_t* cells_per_lane,
                     uint32_t* draw_cells_per_lane, uint32_t* draw_ahead) {
   if (!g || !passes || !cells_per_lane || !draw_cells_per_lane || !draw_ahead)
     return fail(LG_EINVAL, "null argument");
+  resolve_pipeline(g);
   *passes = step_passes(g);
   *cells_per_lane = g->units == 1 ? 64u : 32u;
   *draw_cells_per_lane = g->draw_units == 1 ? 64u : 32u;

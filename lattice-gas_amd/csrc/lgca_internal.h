@@ -61,6 +61,20 @@ void equilibrium_thresholds(float density, float ux, float uy, bool rest, uint32
 // step below it.
 uint32_t step_passes(const lg_lattice* g);
 
+// Words per plane below which the two-pass step's draw work is split finer by
+// default: into 32-cell draw lanes, or, with inflow columns, into draws run 8
+// steps ahead on the second stream (so that an inflow word's long serial
+// draws overlap other steps). Measured on MI355X with the reference's own
+// benchmark lattices (density 0.22, drift 0.4): 256 x 256, 1,024 words, 5.2
+// -> 5.0 us per step; 2048 x 1280, 40,960 words, 7.9 -> 6.8 us, with inflow
+// 11.1 -> 8.8 us; 1024 x 1024 with inflow 10.7 -> 7.7 us; at 131,072 words
+// (4096 x 2048) 32-cell draw lanes are 17% slower and draw-ahead 10% slower.
+#define LG_SMALL_WORDS (1u << 16)
+
+// Settle the draw unit and draw-ahead that follow the size rule (those the
+// caller has not set): see LG_SMALL_WORDS.
+void resolve_pipeline(lg_lattice* g);
+
 // Rebuild the static jump and order tables of a lattice if its obstacle
 // layout or inflow width changed since they were last built.
 int ensure_tables(lg_lattice* g);
@@ -96,6 +110,8 @@ struct lg_lattice {
   // 0 = chosen by size (step_passes)
   uint32_t passes = 0;
   uint32_t draw_units = 1;  // draw lanes of the two-pass step: 1 = 64 cells, 2 = 32
+  // draw_units and ahead follow the size rule until the caller sets them
+  bool auto_draw_units = true, auto_ahead = true;
   uint32_t kslot = 0;
   // Draw-ahead (ahead > 0): the draw work runs on `dstream` in launches of up
   // to `ahead` steps into two groups of slots; `stream` runs one collide

@@ -235,12 +235,14 @@ class GpuLattice:
         return tuple(x.value for x in v)
 
     def set_draw_unit(self, cells_per_lane: int):
-        """64 or 32 cells per lane for the draw work of the two-pass step."""
+        """64 or 32 cells per lane for the draw work of the two-pass step; 0
+        (the default) by lattice size."""
         _check(load_library().lg_set_draw_unit(self._p, int(cells_per_lane)))
 
     def set_draw_ahead(self, steps: int):
         """0: draws share the collide launches; N: draws of up to N steps per
-        launch on a second stream, ahead of the collides."""
+        launch on a second stream, ahead of the collides. Until set (or after
+        set_passes(0)) it follows the lattice size."""
         _check(load_library().lg_set_draw_ahead(self._p, int(steps)))
 
     def advance_timed(self, n: int, sample_every: int = 0, alpha: float = 0.0) -> float:

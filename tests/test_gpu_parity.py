@@ -84,9 +84,10 @@ def _compare_run(oracle, w, h, rest, bands, inlet, seed, steps, density=0.22, in
     lat.init_equilibrium()
     g = GpuLattice(w, h, rest, seed, bands)
     g.set_work_unit(cells_per_lane)
-    g.set_passes(passes[0])
-    g.set_draw_unit(passes[1])
-    g.set_draw_ahead(passes[2] if len(passes) > 2 else 0)
+    if passes is not None:  # None: the default pipeline, by size
+        g.set_passes(passes[0])
+        g.set_draw_unit(passes[1])
+        g.set_draw_ahead(passes[2] if len(passes) > 2 else 0)
     g.set_inlet(inlet, density, inflow)
     g.load(lat.cells)
     done = 0
@@ -330,3 +331,29 @@ def test_default_pipeline_follows_the_lattice_size():
     big.set_passes(2)
     assert big.pipeline()[:3] == (2, 64, 64)
     big.close()
+
+
+def test_default_draw_work_of_small_lattices(oracle):
+    """Below 2^16 words: 32-cell draw lanes without inflow, draws 8 steps
+    ahead with inflow; explicit settings override, set_passes(0) restores the
+    size rule; the default runs equal the CPU replay."""
+    say("small-lattice draw defaults")
+    g = GpuLattice(2048, 1280, True, 3, 4)
+    assert g.pipeline() == (2, 64, 32, 0)
+    g.set_inlet(8, 0.22, (0.3, 0.0))
+    assert g.pipeline() == (2, 64, 64, 8)
+    g.set_draw_unit(32)
+    g.set_draw_ahead(0)
+    assert g.pipeline() == (2, 64, 32, 0)
+    g.set_passes(0)
+    assert g.pipeline() == (2, 64, 64, 8)
+    g.set_draw_unit(0)
+    assert g.pipeline() == (2, 64, 64, 8)
+    g.close()
+    big = GpuLattice(4096, 2304, True, 3, 4)
+    big.set_inlet(8, 0.22, (0.3, 0.0))
+    assert big.pipeline() == (2, 64, 64, 0)
+    big.close()
+    for inlet in (0, 8):
+        _compare_run(oracle, 320, 96, True, 3, inlet, 17, 40, disc=(100.0, 40.0, 9.0),
+                     passes=None)
