@@ -797,7 +797,11 @@ int lg_device_name(lg_t* g, char* buf, size_t len) {
   if (!g || !buf || len == 0) return fail(LG_EINVAL, "null argument");
   hipDeviceProp_t p;
   HIP_TRY(hipGetDeviceProperties(&p, g->device));
-  std::snprintf(buf, len, "%s (%s)", p.name, p.gcnArchName);
+  // Some drivers leave the marketing name empty; the architecture is always set.
+  if (p.name[0])
+    std::snprintf(buf, len, "%s (%s)", p.name, p.gcnArchName);
+  else
+    std::snprintf(buf, len, "AMD GPU %s, %d CUs", p.gcnArchName, p.multiProcessorCount);
   return LG_OK;
 }
 

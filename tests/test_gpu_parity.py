@@ -357,3 +357,14 @@ def test_default_draw_work_of_small_lattices(oracle):
     for inlet in (0, 8):
         _compare_run(oracle, 320, 96, True, 3, inlet, 17, 40, disc=(100.0, 40.0, 9.0),
                      passes=None)
+
+
+@pytest.mark.parametrize("w,h", [(256, 64), (100, 32), (129, 18)])
+def test_particle_count_at_ragged_sizes(oracle, w, h):
+    """GpuLattice::total_particles at the reference's own test sizes
+    (src/gpu.rs:1035-1053), with an obstacle and inflow, after 37 steps:
+    equal to the CPU count (src/lattice.rs:292-300)."""
+    say(f"particle count {w}x{h}")
+    lat, g = _compare_run(oracle, w, h, True, 2, 8, 11, 37, disc=(w / 3.0, h / 3.0, 4.0),
+                          passes=None, check_every=37)
+    assert g.total_particles() == lat.total_particles()
