@@ -28,6 +28,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -198,6 +199,8 @@ Scratch& scratch() {
 // else a literal; every position that starts three bytes is hashed in.
 // Appends to `out`.
 void deflate(const uint8_t* d, size_t n, std::vector<uint8_t>& out) {
+  if (n > 0x7FFFFFFFu)  // chain positions are 31-bit
+    throw std::length_error("deflate input over 2 GiB");
   const Codes& T = codes();
   Scratch& S = scratch();
   if (S.prev.size() < n) S.prev.resize(n);
