@@ -24,6 +24,7 @@ from __future__ import annotations
 import math
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -147,18 +148,28 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
     every resident run by min(every, steps left) in the same launches, takes
     one batched snapshot of all of them, and retires the runs that are done.
     A run joins at a boundary, so its snapshots fall every `every` of its own
-    steps and at its end, exactly as in a run of its own."""
+    steps and at its end, exactly as in a run of its own.
+
+    The advance runs on a worker thread (the library call releases the
+    interpreter lock), and while it runs the host frees the lattices retired
+    at the last boundary (a device free waits for the device, so this takes
+    the rest of the advance) and then builds the runs that have arrived; those
+    join at the next boundary. Both touch only lattices outside the batch, and
+    a retired lattice's bytes count against the budget until it is freed."""
     pending = deque(sorted(trace, key=lambda r: r.arrival))
     results = {r.rid: Result(r.rid, r.arrival) for r in trace}
     active = []             # dicts: req, g, done, bytes
-    used = 0
+    ready = []              # built, not yet in the batch
+    retired = []            # (lattice, bytes) of finished runs, not yet freed
+    used = 0                # bytes of every built lattice not yet freed
     batch = None
     t0 = time.perf_counter()
     clock = lambda: time.perf_counter() - t0
     quanta = 0
     peak = 0
     spent = {"admit": 0.0, "batch_setup": 0.0, "advance": 0.0, "snapshot": 0.0,
-             "retire": 0.0, "idle": 0.0}     # wall seconds by activity
+             "retire": 0.0, "idle": 0.0,            # wall seconds by activity
+             "free_during_advance": 0.0, "admit_during_advance": 0.0}
 
     def timed(key, fn, *a):
         t = time.perf_counter()
@@ -166,55 +177,74 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
         spent[key] += time.perf_counter() - t
         return r
 
-    while pending or active:
-        now = clock()
-        changed = False
-        while pending and pending[0].arrival <= now:
+    def admit_arrived():
+        nonlocal used
+        while pending and pending[0].arrival <= clock():
             r = pending[0]
             fp = footprint(r, block)
-            if used + fp > hbm_budget and active:
+            if used + fp > hbm_budget and (active or ready or retired):
                 break                                # wait for memory to free up
             pending.popleft()
-            g = timed("admit", build, r, block)
+            g = build(r, block)
             used += fp
             results[r.rid].start = clock()
-            active.append({"req": r, "g": g, "done": 0, "bytes": fp})
-            changed = True
-        if not active:
-            if pending:
-                timed("idle", time.sleep, max(0.0, min(pending[0].arrival - clock(), 0.01)))
-            continue
-        peak = max(peak, len(active))
-        if changed or batch is None:
-            ts = time.perf_counter()
-            if batch is not None:
+            ready.append({"req": r, "g": g, "done": 0, "bytes": fp})
+
+    def free_retired():
+        nonlocal used
+        while retired:
+            g, fp = retired.pop()
+            g.close()
+            used -= fp
+
+    with ThreadPoolExecutor(max_workers=1) as worker:
+        while pending or active or ready:
+            timed("admit", admit_arrived)
+            changed = bool(ready)
+            active.extend(ready)
+            ready.clear()
+            if not active:
+                timed("retire", free_retired)
+                if pending:
+                    timed("idle", time.sleep, max(0.0, min(pending[0].arrival - clock(), 0.01)))
+                continue
+            peak = max(peak, len(active))
+            if changed or batch is None:
+                ts = time.perf_counter()
+                if batch is not None:
+                    batch.close()
+                batch = Batch([a["g"] for a in active])
+                if shape:
+                    batch.set_shape(*shape)
+                spent["batch_setup"] += time.perf_counter() - ts
+            limits = [a["req"].steps - a["done"] for a in active]
+            ta = time.perf_counter()
+            run = worker.submit(batch.advance_limited, every, limits)
+            try:
+                timed("free_during_advance", free_retired)
+                timed("admit_during_advance", admit_arrived)
+            finally:
+                run.result()            # the batch must be idle before anything else
+            spent["advance"] += time.perf_counter() - ta
+            quanta += 1
+            snaps = timed("snapshot", batch.snapshot, 1.0)
+            for a, lim, snap in zip(active, limits, snaps):
+                a["done"] += min(every, lim)
+                results[a["req"].rid].snapshots.append((a["done"],) + snap)
+            finished = [a for a in active if a["done"] >= a["req"].steps]
+            tr = time.perf_counter()
+            if finished:
+                for a in finished:
+                    res = results[a["req"].rid]
+                    if keep_cells:
+                        res.final_cells = a["g"].store()
+                    res.done = clock()
+                    retired.append((a["g"], a["bytes"]))
+                active = [a for a in active if a["done"] < a["req"].steps]
                 batch.close()
-            batch = Batch([a["g"] for a in active])
-            if shape:
-                batch.set_shape(*shape)
-            spent["batch_setup"] += time.perf_counter() - ts
-        limits = [a["req"].steps - a["done"] for a in active]
-        timed("advance", batch.advance_limited, every, limits)
-        quanta += 1
-        snaps = timed("snapshot", batch.snapshot, 1.0)
-        for a, lim, snap in zip(active, limits, snaps):
-            a["done"] += min(every, lim)
-            results[a["req"].rid].snapshots.append((a["done"],) + tuple(np.array(x) for x in snap))
-        finished = [a for a in active if a["done"] >= a["req"].steps]
-        tr = time.perf_counter()
-        if finished:
-            for a in finished:
-                res = results[a["req"].rid]
-                if keep_cells:
-                    res.final_cells = a["g"].store()
-                res.done = clock()
-                used -= a["bytes"]
-            active = [a for a in active if a["done"] < a["req"].steps]
-            batch.close()
-            batch = None
-            for a in finished:
-                a["g"].close()
-        spent["retire"] += time.perf_counter() - tr
+                batch = None
+            spent["retire"] += time.perf_counter() - tr
+    free_retired()
     t_last = max(res.done for res in results.values())
     s = _summary(trace, results, 0.0, t_last, "batched")
     s["quanta"] = quanta

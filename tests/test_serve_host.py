@@ -1,6 +1,7 @@
 """Host-side pieces of the serving loop: the request trace and the HBM
 footprint used for admission (no device calls)."""
 import math
+import time
 
 import numpy as np
 
@@ -29,3 +30,79 @@ def test_footprint_counts_planes_tables_and_field():
     planes = 16 * 64 * 2304 * 8          # 2 x 7 planes + solid + jump table
     assert planes < fp < planes + 64 * 2**20
     assert serve.footprint(r, 32) > serve.footprint(serve.Request(1, 0.0, 256, 256, 1000, 1, 8, 0.2, 50.0, 30), 32)
+
+
+class _Lattice:
+    """Stand-in for a resident lattice: counts its own steps."""
+    live = {}
+
+    def __init__(self, r):
+        self.r, self.steps, self.closed = r, 0, False
+        _Lattice.live[r.rid] = self
+
+    def close(self):
+        assert not self.closed, "a lattice freed twice"
+        time.sleep(0.003)               # a device free waits for the device
+        self.closed = True
+        del _Lattice.live[self.r.rid]
+
+
+class _Batch:
+    """Stand-in for lgca_amd.Batch: the advance sleeps so that the host work
+    the server overlaps with it really runs beside it."""
+    advancing = False
+
+    def __init__(self, members):
+        assert all(not m.closed for m in members)
+        self.members, self.closed = members, False
+
+    def advance_limited(self, n, limits):
+        assert not self.closed and len(limits) == len(self.members)
+        _Batch.advancing = True
+        time.sleep(0.002)
+        for m, lim in zip(self.members, limits):
+            assert not m.closed, "a member freed during the advance"
+            m.steps += min(n, lim)
+        _Batch.advancing = False
+
+    def snapshot(self, alpha):
+        assert not _Batch.advancing
+        return [(np.full(1, m.steps, np.float32),) * 4 for m in self.members]
+
+    def close(self):
+        assert not self.closed and not _Batch.advancing
+        self.closed = True
+
+
+def test_batched_scheduler_snapshots_budget_and_frees(monkeypatch):
+    """The continuous-batching loop with stand-ins for the device objects:
+    every run's snapshots fall at its own multiples of `every` and its last
+    step, every lattice is freed exactly once and never while its batch
+    advances, and the bytes of lattices not yet freed stay within the budget
+    whenever a run is admitted beside others."""
+    rng = np.random.default_rng(5)
+    trace = [serve.Request(i, 0.002 * i, 64, 64, int(rng.integers(1, 9)) * 25 + int(rng.integers(0, 25)),
+                           i + 1, 4, 0.2, 10.0, 4) for i in range(40)]
+    size = {r.rid: int(rng.integers(1, 5)) for r in trace}
+    budget = 6
+    high = [0]
+
+    def build(r, block):
+        _Lattice(r)
+        live = sum(size[k] for k in list(_Lattice.live))
+        if len(_Lattice.live) > 1:
+            high[0] = max(high[0], live)
+        return _Lattice.live[r.rid]
+
+    monkeypatch.setattr(serve, "build", build)
+    monkeypatch.setattr(serve, "Batch", _Batch)
+    monkeypatch.setattr(serve, "footprint", lambda r, block: size[r.rid])
+    s, res = serve.serve_batched(trace, every=25, block=16, hbm_budget=budget)
+    assert not _Lattice.live
+    assert 0 < high[0] <= budget
+    for r in trace:
+        steps = [snap[0] for snap in res[r.rid].snapshots]
+        assert steps == list(range(25, r.steps, 25)) + [r.steps], r
+        assert all(int(snap[1][0]) == snap[0] for snap in res[r.rid].snapshots)
+        assert res[r.rid].done >= res[r.rid].start >= 0.0
+    assert s["requests"] == len(trace) and s["peak_resident"] >= 2
