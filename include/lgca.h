@@ -47,7 +47,8 @@ typedef struct lg_lattice lg_t;
  * error (see lg_last_error). */
 lg_t* lg_create(size_t w, size_t h, int rest, uint64_t seed, size_t bands);
 
-/* Drop for GpuLattice. NULL is accepted. */
+/* Drop for GpuLattice. NULL is accepted. A handle that a batch still holds
+ * is freed when the last such batch is destroyed (see lg_batch_create). */
 void lg_destroy(lg_t* g);
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -198,7 +199,12 @@ int lg_measure_transport(float density, int rest, size_t threads, uint64_t seed,
  * tile table, and the step loop is captured in a hipGraph and replayed. Each
  * member ends exactly where its own lg_advance would have left it. While a
  * batch exists its members may still be used on their own between batch calls
- * (lg_batch_advance re-reads their state each call). */
+ * (lg_batch_advance re-reads their state each call). lg_destroy of a member
+ * that a batch holds is deferred: the handle is freed by the last
+ * lg_batch_destroy of the batches holding it, and until then those batches'
+ * advance, snapshot, sample and autotune calls fail with LG_EINVAL. Creating,
+ * destroying and calling batches that share a member must not overlap in
+ * time (one thread, or the caller's own lock). */
 typedef struct lg_batch lg_batch_t;
 lg_batch_t* lg_batch_create(lg_t* const* members, size_t n);
 void lg_batch_destroy(lg_batch_t* b);

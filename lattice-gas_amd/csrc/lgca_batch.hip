@@ -471,6 +471,14 @@ void commit_members(lg_batch* b, const uint64_t* limits, uint64_t n) {
 
 }  // namespace
 
+// A batch call on a member that lg_destroy has released to the batch would
+// run on a handle its owner has given up: refuse it.
+static int members_alive(const lg_batch* b) {
+  for (const lg_t* g : b->members)
+    if (g->destroyed) return fail(LG_EINVAL, "a member of the batch was destroyed; destroy the batch");
+  return LG_OK;
+}
+
 extern "C" {
 
 lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
@@ -479,17 +487,18 @@ lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
     return nullptr;
   }
   auto* b = new lg_batch();
-  auto bail = [&](const std::string& m) -> lg_batch_t* {
-    fail(LG_EDEVICE, m);
+  auto bail = [&](const std::string& m, int code = LG_EDEVICE) -> lg_batch_t* {
+    fail(code, m);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return nullptr;
   };
   b->members.assign(members, members + n);
   for (size_t i = 0; i < n; ++i) {
-    if (!members[i]) return bail("null member");
+    if (!members[i]) return bail("null member", LG_EINVAL);
+    if (members[i]->destroyed) return bail("a member was destroyed", LG_EINVAL);
     for (size_t k = 0; k < i; ++k)
-      if (members[k] == members[i]) return bail("a lattice appears twice in the batch");
+      if (members[k] == members[i]) return bail("a lattice appears twice in the batch", LG_EINVAL);
   }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
     return bail("hipStreamCreate failed");
@@ -513,6 +522,7 @@ lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
   if (b->desc.alloc(n)) return bail("hipMalloc failed for descriptors");
   if (ensure_tiles(b)) return bail(g_err);
   if (hipStreamSynchronize(b->stream) != hipSuccess) return bail("hipStreamSynchronize failed");
+  for (lg_t* g : b->members) g->batch_refs += 1;
   return b;
 }
 
@@ -524,11 +534,14 @@ void lg_batch_destroy(lg_batch_t* b) {
   b->tiles128.release();
   b->tiles256.release();
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  for (lg_t* g : b->members)
+    if (--g->batch_refs == 0 && g->destroyed) destroy_lattice(g);
   delete b;
 }
 
 int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits) {
   if (!b) return fail(LG_EINVAL, "null batch");
+  if (int rc = members_alive(b)) return rc;
   if (n == 0) return LG_OK;
   for (lg_t* g : b->members) HIP_TRY(hipStreamSynchronize(g->stream));
   int rc = upload_desc(b, limits, n);
@@ -547,6 +560,7 @@ int lg_batch_advance(lg_batch_t* b, uint64_t n) { return lg_batch_advance_limite
 
 int lg_batch_snapshot(lg_batch_t* b, float alpha, float* out, size_t nfloats) {
   if (!b || !out) return fail(LG_EINVAL, "null argument");
+  if (int rc = members_alive(b)) return rc;
   for (lg_t* g : b->members) HIP_TRY(hipStreamSynchronize(g->stream));
   int rc = upload_desc(b, nullptr, 0);
   if (rc) return rc;
@@ -635,6 +649,7 @@ int lg_batch_get_shape(lg_batch_t* b, uint32_t* tile, uint32_t* grid, uint32_t* 
 int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps) {
   if (!b || !used_steps) return fail(LG_EINVAL, "null argument");
   *used_steps = 0;
+  if (int rc = members_alive(b)) return rc;
   for (lg_t* g : b->members) HIP_TRY(hipStreamSynchronize(g->stream));
   int rc = upload_desc(b, nullptr, ~0ull);
   if (rc) return rc;
@@ -705,6 +720,7 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
 // on the members' own streams, and wait.
 int lg_batch_sample(lg_batch_t* b, float alpha) {
   if (!b) return fail(LG_EINVAL, "null batch");
+  if (int rc = members_alive(b)) return rc;
   for (lg_t* g : b->members) {
     int rc = lg_sample_field_async(g, alpha);
     if (rc) return rc;

@@ -424,6 +424,35 @@ void equilibrium_thresholds(float density, float ux, float uy, bool rest, uint32
 }
 
 
+void destroy_lattice(lg_lattice* g) {
+  (void)hipStreamSynchronize(g->stream);
+  if (g->dstream) (void)hipStreamSynchronize(g->dstream);
+  g->a.release();
+  g->b.release();
+  g->solid.release();
+  g->cjump.release();
+  g->ijump.release();
+  g->basis.release();
+  g->row_band.release();
+  g->order.release();
+  g->bytes.release();
+  g->fux.release();
+  g->fuy.release();
+  g->frho.release();
+  g->fsol.release();
+  g->fsums.release();
+  g->counter.release();
+  for (hipEvent_t e : g->tev) (void)hipEventDestroy(e);
+  for (int q = 0; q < 2; ++q) {
+    if (g->ev_drawn[q]) (void)hipEventDestroy(g->ev_drawn[q]);
+    if (g->ev_used[q]) (void)hipEventDestroy(g->ev_used[q]);
+  }
+  if (g->ev_start) (void)hipEventDestroy(g->ev_start);
+  if (g->dstream) (void)hipStreamDestroy(g->dstream);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
 }  // namespace lgca_internal
 
 using namespace lgca_internal;
@@ -765,32 +794,11 @@ lg_t* lg_create(size_t w, size_t h, int rest, uint64_t seed, size_t bands) {
 
 void lg_destroy(lg_t* g) {
   if (!g) return;
-  (void)hipStreamSynchronize(g->stream);
-  if (g->dstream) (void)hipStreamSynchronize(g->dstream);
-  g->a.release();
-  g->b.release();
-  g->solid.release();
-  g->cjump.release();
-  g->ijump.release();
-  g->basis.release();
-  g->row_band.release();
-  g->order.release();
-  g->bytes.release();
-  g->fux.release();
-  g->fuy.release();
-  g->frho.release();
-  g->fsol.release();
-  g->fsums.release();
-  g->counter.release();
-  for (hipEvent_t e : g->tev) (void)hipEventDestroy(e);
-  for (int q = 0; q < 2; ++q) {
-    if (g->ev_drawn[q]) (void)hipEventDestroy(g->ev_drawn[q]);
-    if (g->ev_used[q]) (void)hipEventDestroy(g->ev_used[q]);
+  if (g->batch_refs) {  // still a batch member: the last lg_batch_destroy frees it
+    g->destroyed = true;
+    return;
   }
-  if (g->ev_start) (void)hipEventDestroy(g->ev_start);
-  if (g->dstream) (void)hipStreamDestroy(g->dstream);
-  if (g->stream) (void)hipStreamDestroy(g->stream);
-  delete g;
+  destroy_lattice(g);
 }
 
 int lg_device_name(lg_t* g, char* buf, size_t len) {

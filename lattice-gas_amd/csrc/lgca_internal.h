@@ -75,6 +75,9 @@ uint32_t step_passes(const lg_lattice* g);
 // caller has not set): see LG_SMALL_WORDS.
 void resolve_pipeline(lg_lattice* g);
 
+// Free a handle and everything it owns (after its streams drain).
+void destroy_lattice(lg_lattice* g);
+
 // Rebuild the static jump and order tables of a lattice if its obstacle
 // layout or inflow width changed since they were last built.
 int ensure_tables(lg_lattice* g);
@@ -112,6 +115,11 @@ struct lg_lattice {
   uint32_t draw_units = 1;  // draw lanes of the two-pass step: 1 = 64 cells, 2 = 32
   // draw_units and ahead follow the size rule until the caller sets them
   bool auto_draw_units = true, auto_ahead = true;
+  // Batches that hold this handle. lg_destroy of a held handle only marks it
+  // destroyed; the last lg_batch_destroy frees it, and the batches' calls
+  // fail until then.
+  uint32_t batch_refs = 0;
+  bool destroyed = false;
   uint32_t kslot = 0;
   // Draw-ahead (ahead > 0): the draw work runs on `dstream` in launches of up
   // to `ahead` steps into two groups of slots; `stream` runs one collide

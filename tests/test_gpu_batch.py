@@ -128,3 +128,32 @@ def test_pipeline_changes_between_calls():
     for s_, g in zip(solo, bat):
         assert g.steps == s_.steps
         assert np.array_equal(g.store(), s_.store())
+
+
+def test_destroying_a_member_waits_for_its_batch():
+    """lg_destroy of a member only marks it: the batch's calls fail with a
+    clear error, and the batch frees it when it is destroyed; the other
+    member is untouched and still equals its own run."""
+    import ctypes as C
+    from lgca_amd import LgcaError, load_library
+    L = load_library()
+    a, b = member(256, 128, True, 1, 3, 8, True), member(192, 64, True, 2, 1, 0, False)
+    ref = member(192, 64, True, 2, 1, 0, False)
+    bt = Batch([a, b])
+    bt.advance(6)
+    raw = a._p                               # still allocated: the batch holds it
+    a.close()
+    with pytest.raises(LgcaError, match="destroyed"):
+        bt.advance(6)
+    with pytest.raises(LgcaError, match="destroyed"):
+        bt.autotune(8)
+    assert not L.lg_batch_create((C.c_void_p * 2)(b._p, raw), 2)
+    assert "destroyed" in L.lg_last_error().decode()
+    bt.close()                               # frees a
+    b.advance(4)
+    ref.advance(10)
+    assert np.array_equal(b.store(), ref.store())
+    with pytest.raises(LgcaError, match="twice"):
+        Batch([b, b])
+    for g in (b, ref):
+        g.close()
