@@ -86,6 +86,18 @@ int lg_advance(lg_t* g, uint64_t n);
  * every == 0 skips sampling. */
 int lg_advance_sampling(lg_t* g, uint64_t n, uint64_t every, float alpha);
 
+/* New: lg_advance and lg_advance_sampling without the wait, for serving loops
+ * that overlap host work with the device (the reference's calls always wait,
+ * src/gpu.rs:657-706). They return once the steps are queued on the handle's
+ * stream; lg_steps already counts them. Every later call on the handle queues
+ * behind them, and the calls that return data (lg_store, lg_read_field, ...)
+ * wait for them; lg_poll reports whether the handle's queued work has ended
+ * (1) or not (0), lg_wait waits for it. */
+int lg_advance_async(lg_t* g, uint64_t n);
+int lg_advance_sampling_async(lg_t* g, uint64_t n, uint64_t every, float alpha);
+int lg_poll(lg_t* g);
+int lg_wait(lg_t* g);
+
 /* GpuLattice::sample_field (src/gpu.rs:710-721). No-op without a field. */
 int lg_sample_field(lg_t* g, float alpha);
 
@@ -212,6 +224,16 @@ int lg_batch_advance(lg_batch_t* b, uint64_t n);
 /* Advance member i by min(n, limits[i]) steps in the same launches (members
  * with fewer steps left stop early); limits has one entry per member. */
 int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits);
+/* lg_batch_advance_limited without the wait (limits may be NULL: n for
+ * all). It first waits for work queued earlier for the members (their own
+ * calls, async advances of batches), then queues the steps and returns; the
+ * members' step counters already count them. A member's own calls that touch
+ * the device, and other batches' calls, wait for them first; lg_poll of a
+ * member includes them. lg_batch_poll reports whether the batch's queued work
+ * has ended (1) or not (0), lg_batch_wait waits for it. */
+int lg_batch_advance_async(lg_batch_t* b, uint64_t n, const uint64_t* limits);
+int lg_batch_poll(lg_batch_t* b);
+int lg_batch_wait(lg_batch_t* b);
 /* Sample every member's attached field (lg_sample_field with alpha) in one
  * launch and copy all of them out: for each member with a field, in member
  * order, its ux, uy, rho, solid arrays back to back. nfloats must equal

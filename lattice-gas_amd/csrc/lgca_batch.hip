@@ -202,6 +202,7 @@ __global__ void __launch_bounds__(256) lg_snapshot_kernel(const LgDesc* __restri
 struct lg_batch {
   std::vector<lg_t*> members;
   hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;  // end of the last advance, waited for by the members' streams
   DevBuf<LgDesc> desc;
   DevBuf<uint2> tiles128, tiles256;
   uint32_t ntiles128 = 0, ntiles256 = 0;
@@ -251,6 +252,8 @@ int upload_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
     if (rc) return rc;
     if ((rc = ensure_draw_buffers(g))) return rc;  // the pipeline may change within a call
   }
+  // the last upload from host_desc may still be in flight after an async advance
+  HIP_TRY(hipStreamSynchronize(b->stream));
   std::vector<LgDesc>& h = b->host_desc;
   h.assign(b->members.size(), LgDesc{});
   size_t off = 0;
@@ -479,6 +482,16 @@ static int members_alive(const lg_batch* b) {
   return LG_OK;
 }
 
+// Wait for everything queued for the members: by their own calls and by
+// async batch advances (this batch's or another's).
+static int settle_members(lg_batch* b) {
+  for (lg_t* g : b->members) {
+    if (int rc = settle(g)) return rc;
+    HIP_TRY(hipStreamSynchronize(g->stream));
+  }
+  return LG_OK;
+}
+
 extern "C" {
 
 lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
@@ -489,6 +502,7 @@ lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
   auto* b = new lg_batch();
   auto bail = [&](const std::string& m, int code = LG_EDEVICE) -> lg_batch_t* {
     fail(code, m);
+    if (b->done) (void)hipEventDestroy(b->done);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return nullptr;
@@ -502,6 +516,10 @@ lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
   }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
     return bail("hipStreamCreate failed");
+  if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+    b->done = nullptr;
+    return bail("hipEventCreate failed");
+  }
   hipDeviceProp_t p;
   if (hipGetDeviceProperties(&p, members[0]->device) == hipSuccess) b->ncu = p.multiProcessorCount;
   b->kmax = ~0u;
@@ -533,6 +551,9 @@ void lg_batch_destroy(lg_batch_t* b) {
   b->desc.release();
   b->tiles128.release();
   b->tiles256.release();
+  for (lg_t* g : b->members)
+    if (g->after == b->done) g->after = nullptr;  // ended: the stream was synchronised
+  if (b->done) (void)hipEventDestroy(b->done);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   for (lg_t* g : b->members)
     if (--g->batch_refs == 0 && g->destroyed) destroy_lattice(g);
@@ -543,7 +564,7 @@ int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits) 
   if (!b) return fail(LG_EINVAL, "null batch");
   if (int rc = members_alive(b)) return rc;
   if (n == 0) return LG_OK;
-  for (lg_t* g : b->members) HIP_TRY(hipStreamSynchronize(g->stream));
+  if (int rc = settle_members(b)) return rc;
   int rc = upload_desc(b, limits, n);
   if (rc) return rc;
   // a fresh membership runs its first call eagerly; from the second call on
@@ -558,10 +579,45 @@ int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits) 
 
 int lg_batch_advance(lg_batch_t* b, uint64_t n) { return lg_batch_advance_limited(b, n, nullptr); }
 
+int lg_batch_advance_async(lg_batch_t* b, uint64_t n, const uint64_t* limits) {
+  if (!b) return fail(LG_EINVAL, "null batch");
+  if (int rc = members_alive(b)) return rc;
+  if (n == 0) return LG_OK;
+  // work queued earlier on the members' own streams, or by an earlier async
+  // advance of this or another batch, ends first (the host waits for it)
+  if (int rc = settle_members(b)) return rc;
+  int rc = upload_desc(b, limits, n);
+  if (rc) return rc;
+  rc = run_steps(b, n, b->calls > 0);
+  if (rc) return rc;
+  // a member's own calls, and its next batch, wait for this advance first
+  // (settle): no wait is queued on the members' streams, which would hold up
+  // unrelated work sharing their hardware queues
+  HIP_TRY(hipEventRecord(b->done, b->stream));
+  for (lg_t* g : b->members) g->after = b->done;
+  commit_members(b, limits, n);
+  b->calls += 1;
+  return LG_OK;
+}
+
+int lg_batch_poll(lg_batch_t* b) {
+  if (!b) return fail(LG_EINVAL, "null batch");
+  hipError_t e = hipStreamQuery(b->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(LG_EDEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
+int lg_batch_wait(lg_batch_t* b) {
+  if (!b) return fail(LG_EINVAL, "null batch");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return LG_OK;
+}
+
 int lg_batch_snapshot(lg_batch_t* b, float alpha, float* out, size_t nfloats) {
   if (!b || !out) return fail(LG_EINVAL, "null argument");
   if (int rc = members_alive(b)) return rc;
-  for (lg_t* g : b->members) HIP_TRY(hipStreamSynchronize(g->stream));
+  if (int rc = settle_members(b)) return rc;
   int rc = upload_desc(b, nullptr, 0);
   if (rc) return rc;
   if (nfloats != b->staging_floats)
@@ -650,7 +706,7 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
   if (!b || !used_steps) return fail(LG_EINVAL, "null argument");
   *used_steps = 0;
   if (int rc = members_alive(b)) return rc;
-  for (lg_t* g : b->members) HIP_TRY(hipStreamSynchronize(g->stream));
+  if (int rc = settle_members(b)) return rc;
   int rc = upload_desc(b, nullptr, ~0ull);
   if (rc) return rc;
   struct Cand {

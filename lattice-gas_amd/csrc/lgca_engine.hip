@@ -836,6 +836,7 @@ int lg_set_inlet_flow(lg_t* g, size_t cols, float density, float ux, float uy) {
 
 int lg_attach_field(lg_t* g, size_t bx, size_t by) {
   if (!g) return fail(LG_EINVAL, "null handle");
+  if (int rc = settle(g)) return rc;
   if (bx == 0 || by == 0) return fail(LG_EINVAL, "block size must be positive");
   g->bx = (uint32_t)bx;
   g->by = (uint32_t)by;
@@ -859,6 +860,7 @@ int lg_attach_field(lg_t* g, size_t bx, size_t by) {
 
 int lg_load(lg_t* g, const uint8_t* cells, size_t n) {
   if (!g || !cells) return fail(LG_EINVAL, "null argument");
+  if (int rc = settle(g)) return rc;
   if (n != (size_t)g->G.w * g->G.h) return fail(LG_EINVAL, "wrong number of cells");
   if (g->bytes.n < n) HIP_TRY(g->bytes.alloc(n));
   HIP_TRY(hipMemcpyAsync(g->bytes.p, cells, n, hipMemcpyHostToDevice, g->stream));
@@ -881,6 +883,7 @@ int lg_restart(lg_t* g, const uint8_t* cells, size_t n, uint64_t seed) {
 
 int lg_store(lg_t* g, uint8_t* cells, size_t n) {
   if (!g || !cells) return fail(LG_EINVAL, "null argument");
+  if (int rc = settle(g)) return rc;
   if (n != (size_t)g->G.w * g->G.h) return fail(LG_EINVAL, "wrong number of cells");
   if (g->bytes.n < n) HIP_TRY(g->bytes.alloc(n));
   hipLaunchKernelGGL(lg_unpack_kernel, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream,
@@ -895,14 +898,47 @@ int lg_advance(lg_t* g, uint64_t n) { return lg_advance_sampling(g, n, 0, 0.0f);
 
 int lg_advance_sampling(lg_t* g, uint64_t n, uint64_t every, float alpha) {
   if (!g) return fail(LG_EINVAL, "null handle");
+  if (int rc = settle(g)) return rc;
   int rc = enqueue_steps(g, n, every, alpha, false);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));
   return LG_OK;
 }
 
+int lg_advance_async(lg_t* g, uint64_t n) { return lg_advance_sampling_async(g, n, 0, 0.0f); }
+
+int lg_advance_sampling_async(lg_t* g, uint64_t n, uint64_t every, float alpha) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  if (int rc = settle(g)) return rc;
+  return enqueue_steps(g, n, every, alpha, false);
+}
+
+// Work queued on the handle's stream is all of it: the draw-ahead stream's
+// launches are each waited for by a later launch on the handle's stream.
+int lg_poll(lg_t* g) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  if (g->after) {  // an async batch advance stepped it last
+    hipError_t q = hipEventQuery(g->after);
+    if (q == hipErrorNotReady) return 0;
+    if (q != hipSuccess) return fail(LG_EDEVICE, std::string("hipEventQuery: ") + hipGetErrorString(q));
+    g->after = nullptr;
+  }
+  hipError_t e = hipStreamQuery(g->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(LG_EDEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
+int lg_wait(lg_t* g) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  if (int rc = settle(g)) return rc;
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return LG_OK;
+}
+
 int lg_advance_sampling_timed(lg_t* g, uint64_t n, uint64_t every, float alpha, float* ms) {
   if (!g || !ms) return fail(LG_EINVAL, "null argument");
+  if (int rc = settle(g)) return rc;
   if (n == 0) {
     *ms = 0.0f;
     return LG_OK;
@@ -926,11 +962,13 @@ int lg_advance_timed(lg_t* g, uint64_t n, float* ms) {
 
 int lg_sample_field_async(lg_t* g, float alpha) {
   if (!g) return fail(LG_EINVAL, "null handle");
+  if (int rc = settle(g)) return rc;
   return sample_now(g, alpha);
 }
 
 int lg_sample_field(lg_t* g, float alpha) {
   if (!g) return fail(LG_EINVAL, "null handle");
+  if (int rc = settle(g)) return rc;
   int rc = sample_now(g, alpha);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));
@@ -946,6 +984,7 @@ int lg_field_dims(lg_t* g, size_t* bw, size_t* bh, size_t* bx, size_t* by) {
 
 int lg_read_field(lg_t* g, float* ux, float* uy, float* rho, float* solid, size_t nblocks) {
   if (!g || !ux || !uy || !rho || !solid) return fail(LG_EINVAL, "null argument");
+  if (int rc = settle(g)) return rc;
   if (!g->has_field) return fail(LG_ESTATE, "no field attached to this lattice");
   if (nblocks != (size_t)g->bw * g->bh)
     return fail(LG_EINVAL, "the field on the device has a different block grid");
@@ -961,6 +1000,7 @@ int lg_read_field(lg_t* g, float* ux, float* uy, float* rho, float* solid, size_
 int lg_read_block_sums(lg_t* g, int64_t* mass, int64_t* px2, int64_t* py2, int64_t* nsolid,
                        size_t nblocks) {
   if (!g || !mass || !px2 || !py2 || !nsolid) return fail(LG_EINVAL, "null argument");
+  if (int rc = settle(g)) return rc;
   if (!g->has_field) return fail(LG_ESTATE, "no field attached to this lattice");
   if (nblocks != (size_t)g->bw * g->bh)
     return fail(LG_EINVAL, "the field on the device has a different block grid");
@@ -980,6 +1020,7 @@ int lg_read_block_sums(lg_t* g, int64_t* mass, int64_t* px2, int64_t* py2, int64
 
 int lg_total_particles(lg_t* g, uint64_t* out) {
   if (!g || !out) return fail(LG_EINVAL, "null argument");
+  if (int rc = settle(g)) return rc;
   HIP_TRY(hipMemsetAsync(g->counter.p, 0, 8, g->stream));
   hipLaunchKernelGGL(lg_count_kernel, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream,
                      g->G, g->a.p, g->counter.p);
@@ -1054,11 +1095,13 @@ int lg_set_steps(lg_t* g, uint64_t steps) {
 
 int lg_add_solid_box(lg_t* g, float x0, float x1, float y0, float y1) {
   if (!g) return fail(LG_EINVAL, "null handle");
+  if (int rc = settle(g)) return rc;
   return add_solid(g, 0, x0, x1, y0, y1);
 }
 
 int lg_add_solid_disc(lg_t* g, float cx, float cy, float r) {
   if (!g) return fail(LG_EINVAL, "null handle");
+  if (int rc = settle(g)) return rc;
   return add_solid(g, 1, cx, cy, r, 0.0f);
 }
 
@@ -1098,6 +1141,7 @@ extern "C" {
 
 int lg_init_equilibrium(lg_t* g, float density, float ux, float uy) {
   if (!g) return fail(LG_EINVAL, "null handle");
+  if (int rc = settle(g)) return rc;
   std::vector<uint64_t> cinit;
   uint64_t basis[64];
   int rc = init_stream(g, cinit, basis);
@@ -1119,6 +1163,7 @@ int lg_init_equilibrium(lg_t* g, float density, float ux, float uy) {
 
 int lg_init_profile(lg_t* g, int axis, float density, const float* ux, const float* uy, size_t n) {
   if (!g || !ux || !uy) return fail(LG_EINVAL, "null argument");
+  if (int rc = settle(g)) return rc;
   if (axis != 0 && axis != 1) return fail(LG_EINVAL, "axis must be 0 (columns) or 1 (rows)");
   if (n != (axis == 0 ? g->G.w : g->G.h)) return fail(LG_EINVAL, "profile length differs from the lattice");
   std::vector<uint32_t> thr(n * 7);

@@ -120,6 +120,9 @@ struct lg_lattice {
   // fail until then.
   uint32_t batch_refs = 0;
   bool destroyed = false;
+  // End of the async batch advance that last stepped this handle, until a
+  // call of its own has waited for it (cleared by that batch's destroy).
+  hipEvent_t after = nullptr;
   uint32_t kslot = 0;
   // Draw-ahead (ahead > 0): the draw work runs on `dstream` in launches of up
   // to `ahead` steps into two groups of slots; `stream` runs one collide
@@ -139,3 +142,18 @@ struct lg_lattice {
   DevBuf<int64_t> fsums;
   DevBuf<unsigned long long> counter;
 };
+
+namespace lgca_internal {
+
+// A handle that an async batch advance has stepped: its planes belong to the
+// batch's stream until that advance ends. Calls of the handle's own that
+// touch the device wait for it here (on the host) before they queue work.
+inline int settle(lg_lattice* g) {
+  if (!g->after) return 0;
+  hipError_t e = hipEventSynchronize(g->after);
+  if (e != hipSuccess) return fail(LG_EDEVICE, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+  g->after = nullptr;
+  return 0;
+}
+
+}  // namespace lgca_internal

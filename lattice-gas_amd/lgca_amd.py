@@ -51,6 +51,10 @@ def load_library(path: str | None = None):
         "lg_store": (i32, [vp, P(C.c_uint8), sz]),
         "lg_advance": (i32, [vp, u64]),
         "lg_advance_sampling": (i32, [vp, u64, u64, f32]),
+        "lg_advance_async": (i32, [vp, u64]),
+        "lg_advance_sampling_async": (i32, [vp, u64, u64, f32]),
+        "lg_poll": (i32, [vp]),
+        "lg_wait": (i32, [vp]),
         "lg_sample_field": (i32, [vp, f32]),
         "lg_read_field": (i32, [vp, P(f32), P(f32), P(f32), P(f32), sz]),
         "lg_read_block_sums": (i32, [vp] + [P(C.c_int64)] * 4 + [sz]),
@@ -83,6 +87,9 @@ def load_library(path: str | None = None):
         "lg_batch_autotune": (i32, [vp, u64, P(u64)]),
         "lg_batch_sample": (i32, [vp, f32]),
         "lg_batch_advance_limited": (i32, [vp, u64, P(u64)]),
+        "lg_batch_advance_async": (i32, [vp, u64, P(u64)]),
+        "lg_batch_poll": (i32, [vp]),
+        "lg_batch_wait": (i32, [vp]),
         "lg_batch_snapshot": (i32, [vp, f32, P(f32), sz]),
         "lg_batch_snapshot_size": (sz, [vp]),
         "lg_sample_field_async": (i32, [vp, f32]),
@@ -217,6 +224,19 @@ class GpuLattice:
     def advance_sampling(self, n: int, sample_every: int, alpha: float):
         _check(load_library().lg_advance_sampling(self._p, int(n), int(sample_every), float(alpha)))
 
+    def advance_async(self, n: int, sample_every: int = 0, alpha: float = 0.0):
+        """advance / advance_sampling without the wait: returns once the steps
+        are queued; later calls on the lattice queue behind them."""
+        _check(load_library().lg_advance_sampling_async(self._p, int(n), int(sample_every),
+                                                         float(alpha)))
+
+    def poll(self) -> bool:
+        """True once the lattice's queued work has ended."""
+        return bool(_check(load_library().lg_poll(self._p)))
+
+    def wait(self):
+        _check(load_library().lg_wait(self._p))
+
     def set_work_unit(self, cells_per_lane: int):
         """64 (a lane steps a u64 word) or 32 (half a word, twice the lanes)."""
         _check(load_library().lg_set_work_unit(self._p, int(cells_per_lane)))
@@ -326,11 +346,28 @@ class Batch:
 
     def advance_limited(self, n: int, limits):
         """Member i advances min(n, limits[i]) steps."""
+        _check(load_library().lg_batch_advance_limited(self._p, int(n), self._limits(limits)))
+
+    def _limits(self, limits):
+        if limits is None:
+            return None
         lim = np.ascontiguousarray(limits, dtype=np.uint64)
         if lim.size != len(self.members):
             raise LgcaError("one limit per member")
-        _check(load_library().lg_batch_advance_limited(self._p, int(n),
-                                                       lim.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return lim.ctypes.data_as(C.POINTER(C.c_uint64))
+
+    def advance_async(self, n: int, limits=None):
+        """advance_limited (limits None: n for every member) without the wait:
+        returns once the steps are queued; a member's own later calls wait
+        for them first."""
+        _check(load_library().lg_batch_advance_async(self._p, int(n), self._limits(limits)))
+
+    def poll(self) -> bool:
+        """True once the batch's queued work has ended."""
+        return bool(_check(load_library().lg_batch_poll(self._p)))
+
+    def wait(self):
+        _check(load_library().lg_batch_wait(self._p))
 
     def snapshot(self, alpha: float):
         """Sample every member's attached field in one launch; returns, per

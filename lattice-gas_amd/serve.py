@@ -24,7 +24,6 @@ from __future__ import annotations
 import math
 import time
 from collections import deque
-from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -150,12 +149,12 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
     A run joins at a boundary, so its snapshots fall every `every` of its own
     steps and at its end, exactly as in a run of its own.
 
-    The advance runs on a worker thread (the library call releases the
-    interpreter lock), and while it runs the host frees the lattices retired
-    at the last boundary (a device free waits for the device, so this takes
-    the rest of the advance) and then builds the runs that have arrived; those
-    join at the next boundary. Both touch only lattices outside the batch, and
-    a retired lattice's bytes count against the budget until it is freed."""
+    The advance is queued without waiting (Batch.advance_async), and while
+    the device runs it the host frees the lattices retired at the last
+    boundary (a device free waits for the device, so this takes the rest of
+    the advance) and then builds the runs that have arrived; those join at the
+    next boundary. Both touch only lattices outside the batch, and a retired
+    lattice's bytes count against the budget until it is freed."""
     pending = deque(sorted(trace, key=lambda r: r.arrival))
     results = {r.rid: Result(r.rid, r.arrival) for r in trace}
     active = []             # dicts: req, g, done, bytes
@@ -197,53 +196,52 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
             g.close()
             used -= fp
 
-    with ThreadPoolExecutor(max_workers=1) as worker:
-        while pending or active or ready:
-            timed("admit", admit_arrived)
-            changed = bool(ready)
-            active.extend(ready)
-            ready.clear()
-            if not active:
-                timed("retire", free_retired)
-                if pending:
-                    timed("idle", time.sleep, max(0.0, min(pending[0].arrival - clock(), 0.01)))
-                continue
-            peak = max(peak, len(active))
-            if changed or batch is None:
-                ts = time.perf_counter()
-                if batch is not None:
-                    batch.close()
-                batch = Batch([a["g"] for a in active])
-                if shape:
-                    batch.set_shape(*shape)
-                spent["batch_setup"] += time.perf_counter() - ts
-            limits = [a["req"].steps - a["done"] for a in active]
-            ta = time.perf_counter()
-            run = worker.submit(batch.advance_limited, every, limits)
-            try:
-                timed("free_during_advance", free_retired)
-                timed("admit_during_advance", admit_arrived)
-            finally:
-                run.result()            # the batch must be idle before anything else
-            spent["advance"] += time.perf_counter() - ta
-            quanta += 1
-            snaps = timed("snapshot", batch.snapshot, 1.0)
-            for a, lim, snap in zip(active, limits, snaps):
-                a["done"] += min(every, lim)
-                results[a["req"].rid].snapshots.append((a["done"],) + snap)
-            finished = [a for a in active if a["done"] >= a["req"].steps]
-            tr = time.perf_counter()
-            if finished:
-                for a in finished:
-                    res = results[a["req"].rid]
-                    if keep_cells:
-                        res.final_cells = a["g"].store()
-                    res.done = clock()
-                    retired.append((a["g"], a["bytes"]))
-                active = [a for a in active if a["done"] < a["req"].steps]
+    while pending or active or ready:
+        timed("admit", admit_arrived)
+        changed = bool(ready)
+        active.extend(ready)
+        ready.clear()
+        if not active:
+            timed("retire", free_retired)
+            if pending:
+                timed("idle", time.sleep, max(0.0, min(pending[0].arrival - clock(), 0.01)))
+            continue
+        peak = max(peak, len(active))
+        if changed or batch is None:
+            ts = time.perf_counter()
+            if batch is not None:
                 batch.close()
-                batch = None
-            spent["retire"] += time.perf_counter() - tr
+            batch = Batch([a["g"] for a in active])
+            if shape:
+                batch.set_shape(*shape)
+            spent["batch_setup"] += time.perf_counter() - ts
+        limits = [a["req"].steps - a["done"] for a in active]
+        ta = time.perf_counter()
+        batch.advance_async(every, limits)
+        try:
+            timed("free_during_advance", free_retired)
+            timed("admit_during_advance", admit_arrived)
+        finally:
+            batch.wait()            # the batch must be idle before anything else
+        spent["advance"] += time.perf_counter() - ta
+        quanta += 1
+        snaps = timed("snapshot", batch.snapshot, 1.0)
+        for a, lim, snap in zip(active, limits, snaps):
+            a["done"] += min(every, lim)
+            results[a["req"].rid].snapshots.append((a["done"],) + snap)
+        finished = [a for a in active if a["done"] >= a["req"].steps]
+        tr = time.perf_counter()
+        if finished:
+            for a in finished:
+                res = results[a["req"].rid]
+                if keep_cells:
+                    res.final_cells = a["g"].store()
+                res.done = clock()
+                retired.append((a["g"], a["bytes"]))
+            active = [a for a in active if a["done"] < a["req"].steps]
+            batch.close()
+            batch = None
+        spent["retire"] += time.perf_counter() - tr
     free_retired()
     t_last = max(res.done for res in results.values())
     s = _summary(trace, results, 0.0, t_last, "batched")

@@ -1,0 +1,80 @@
+"""Non-blocking advance (lg_advance_async, lg_batch_advance_async): the same
+cells and fields as the blocking calls, step counters that count the queued
+steps, and later calls queued behind the work (gpu)."""
+import numpy as np
+import pytest
+
+from lgca_amd import Batch, GpuLattice
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def lattice(w, h, seed, bands, inlet=8, plate=True):
+    g = GpuLattice(w, h, True, seed, bands)
+    g.set_inlet(inlet, 0.25, (0.2, 0.0))
+    if plate:
+        g.add_solid_box(*W.plate_box(w, h, w / 5.0, max(4, h // 6)))
+    g.init_equilibrium(0.25, 0.0, 0.0)
+    return g
+
+
+def same_field(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a.read_field(), b.read_field()))
+
+
+# (kernels per step, draw-ahead steps): two-pass, fused, two-pass drawn ahead
+@pytest.mark.parametrize("passes,ahead", [(2, 0), (1, 0), (2, 4)])
+def test_async_advance_equals_advance(passes, ahead):
+    a, b = lattice(512, 256, 3, 5), lattice(512, 256, 3, 5)
+    for g in (a, b):
+        g.set_passes(passes)
+        g.set_draw_ahead(ahead)
+        g.attach_field(16, 16)
+    a.advance_sampling(300, 100, 0.5)
+    b.advance_async(100, 100, 0.5)           # two calls queued back to back
+    b.advance_async(200, 100, 0.5)
+    assert b.steps == 300                    # counted when queued
+    cells = b.store()                        # waits for the queued work
+    assert b.poll()
+    assert np.array_equal(cells, a.store())
+    assert same_field(a, b)
+    b.advance_async(7)
+    b.wait()
+    assert b.poll()
+    a.advance(7)
+    assert np.array_equal(b.store(), a.store()) and b.total_particles() == a.total_particles()
+    for g in (a, b):
+        g.close()
+
+
+def test_batch_async_advance_orders_member_calls():
+    specs = [(256, 128, 1, 3), (192, 64, 2, 1), (100, 36, 3, 5)]
+    solo = [lattice(*s) for s in specs]
+    mem = [lattice(*s) for s in specs]
+    for g in solo + mem:
+        g.attach_field(4, 4)
+    bt = Batch(mem)
+    limits = [40, 25, 40]
+    bt.advance_async(40, limits)
+    assert [g.steps for g in mem] == limits  # counted when queued
+    # a member's own call queues behind the batch's steps
+    mem[1].advance_async(5)
+    for g, lim in zip(solo, limits):
+        g.advance(lim)
+    solo[1].advance(5)
+    bt.advance_async(10)                     # waits for the members' own work first
+    bt.wait()
+    assert bt.poll() and all(g.poll() for g in mem)
+    for g in solo:
+        g.advance(10)
+    for s, m in zip(solo, mem):
+        assert s.steps == m.steps
+        assert np.array_equal(s.store(), m.store())
+    snaps = bt.snapshot(1.0)
+    for s, snap in zip(solo, snaps):
+        s.sample_field(1.0)
+        assert all(np.array_equal(x, y) for x, y in zip(s.read_field(), snap))
+    bt.close()
+    for g in solo + mem:
+        g.close()

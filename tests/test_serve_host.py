@@ -1,6 +1,7 @@
 """Host-side pieces of the serving loop: the request trace and the HBM
 footprint used for admission (no device calls)."""
 import math
+import threading
 import time
 
 import numpy as np
@@ -48,29 +49,37 @@ class _Lattice:
 
 
 class _Batch:
-    """Stand-in for lgca_amd.Batch: the advance sleeps so that the host work
-    the server overlaps with it really runs beside it."""
-    advancing = False
+    """Stand-in for lgca_amd.Batch: an async advance runs on a thread of its
+    own and sleeps, so that the host work the server overlaps with it really
+    runs beside it."""
 
     def __init__(self, members):
         assert all(not m.closed for m in members)
-        self.members, self.closed = members, False
+        self.members, self.closed, self.run, self.errors = members, False, None, []
 
-    def advance_limited(self, n, limits):
-        assert not self.closed and len(limits) == len(self.members)
-        _Batch.advancing = True
+    def _advance(self, n, limits):
         time.sleep(0.002)
         for m, lim in zip(self.members, limits):
-            assert not m.closed, "a member freed during the advance"
+            if m.closed:
+                self.errors.append("a member freed during the advance")
             m.steps += min(n, lim)
-        _Batch.advancing = False
+
+    def advance_async(self, n, limits):
+        assert not self.closed and self.run is None and len(limits) == len(self.members)
+        self.run = threading.Thread(target=self._advance, args=(n, list(limits)))
+        self.run.start()
+
+    def wait(self):
+        self.run.join()
+        self.run = None
+        assert not self.errors, self.errors
 
     def snapshot(self, alpha):
-        assert not _Batch.advancing
+        assert self.run is None
         return [(np.full(1, m.steps, np.float32),) * 4 for m in self.members]
 
     def close(self):
-        assert not self.closed and not _Batch.advancing
+        assert not self.closed and self.run is None
         self.closed = True
 
 
