@@ -74,11 +74,12 @@ def cpu_baseline(cfg, total_steps, gpu_cells, budget_s):
     per_step = (time.perf_counter() - t0) / 2
     replay = (total_steps - 2) * per_step <= budget_s
     n_left = (total_steps - 2) if replay else max(7, int(budget_s / max(per_step, 1e-6)))
-    chunk = max(1, n_left // 7)
     rates = []
     done = 0
-    while done < n_left:
-        k = min(chunk, n_left - done)
+    for i in range(7):                  # seven chunks of (nearly) equal length
+        k = n_left * (i + 1) // 7 - done
+        if k == 0:
+            continue
         t = time.perf_counter()
         lat.advance(k)
         rates.append(cells * k / (time.perf_counter() - t))
