@@ -216,7 +216,8 @@ int lg_measure_transport(float density, int rest, size_t threads, uint64_t seed,
  * lg_batch_destroy of the batches holding it, and until then those batches'
  * advance, snapshot, sample and autotune calls fail with LG_EINVAL. Creating,
  * destroying and calling batches that share a member must not overlap in
- * time (one thread, or the caller's own lock). */
+ * time (one thread, or the caller's own lock). Members must have fewer than
+ * 2^30 words per plane (68.7G cells; one such lattice fills the device). */
 typedef struct lg_batch lg_batch_t;
 lg_batch_t* lg_batch_create(lg_t* const* members, size_t n);
 void lg_batch_destroy(lg_batch_t* b);

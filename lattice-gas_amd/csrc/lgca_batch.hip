@@ -511,6 +511,10 @@ lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
   for (size_t i = 0; i < n; ++i) {
     if (!members[i]) return bail("null member", LG_EINVAL);
     if (members[i]->destroyed) return bail("a member was destroyed", LG_EINVAL);
+    // a tile's unit index (word x cells-per-lane split) must stay below bit 31,
+    // which marks draw tiles, and below the no-op tile's 0x7FFFFFFF
+    if (members[i]->G.total >= (1u << 30))
+      return bail("a batch member must have fewer than 2^30 words per plane", LG_EINVAL);
     for (size_t k = 0; k < i; ++k)
       if (members[k] == members[i]) return bail("a lattice appears twice in the batch", LG_EINVAL);
   }
