@@ -1,6 +1,7 @@
 // C ABI over the CPU oracle, for the Python tests and the CPU baseline.
 //
-// TEST INFRASTRUCTURE ONLY: the checker, never the thing measured or shipped.
+// TEST INFRASTRUCTURE ONLY: what results are compared with, never the thing
+// measured or shipped.
 // Each entry point forwards to the restatement in lgca_oracle.hpp (the
 // lattice), lgca_transport.hpp (transport measurement) or lgca_render.hpp
 // (frame output), which cite the reference file:line they follow.
