@@ -235,6 +235,14 @@ int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits);
 int lg_batch_advance_async(lg_batch_t* b, uint64_t n, const uint64_t* limits);
 int lg_batch_poll(lg_batch_t* b);
 int lg_batch_wait(lg_batch_t* b);
+/* lg_batch_advance_async followed, in the same queue, by lg_batch_snapshot's
+ * sample of every member's field (n = 0: the sample alone); the result stays
+ * on the device until lg_batch_read_snapshot, which waits for the batch and
+ * copies it out in lg_batch_snapshot's layout (nfloats as there). Any other
+ * batch call in between discards it (LG_ESTATE). */
+int lg_batch_advance_sampling_async(lg_batch_t* b, uint64_t n, const uint64_t* limits,
+                                    float alpha);
+int lg_batch_read_snapshot(lg_batch_t* b, float* out, size_t nfloats);
 /* Sample every member's attached field (lg_sample_field with alpha) in one
  * launch and copy all of them out: for each member with a field, in member
  * order, its ux, uy, rho, solid arrays back to back. nfloats must equal

@@ -226,6 +226,7 @@ struct lg_batch {
   std::vector<uint64_t> advanced;  // per member, steps taken by the last call
   DevBuf<float> staging;           // snapshots of every member's field
   size_t staging_floats = 0;
+  bool staged = false;             // staging holds the snapshot of the last sampling advance
   uint32_t max_field_blocks = 0;
   uint32_t calls = 0;              // advance calls on this membership
   std::vector<uint2> host_tiles128, host_tiles256;
@@ -566,6 +567,7 @@ void lg_batch_destroy(lg_batch_t* b) {
 
 int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits) {
   if (!b) return fail(LG_EINVAL, "null batch");
+  b->staged = false;
   if (int rc = members_alive(b)) return rc;
   if (n == 0) return LG_OK;
   if (int rc = settle_members(b)) return rc;
@@ -583,24 +585,62 @@ int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits) 
 
 int lg_batch_advance(lg_batch_t* b, uint64_t n) { return lg_batch_advance_limited(b, n, nullptr); }
 
-int lg_batch_advance_async(lg_batch_t* b, uint64_t n, const uint64_t* limits) {
-  if (!b) return fail(LG_EINVAL, "null batch");
+// Queue an advance, and with `sample` one sample of every member's field at
+// its end into the staging buffer (the descriptors the advance leaves on the
+// device name each member's current planes).
+static int advance_async(lg_batch* b, uint64_t n, const uint64_t* limits, bool sample,
+                         float alpha) {
   if (int rc = members_alive(b)) return rc;
-  if (n == 0) return LG_OK;
+  b->staged = false;
+  if (n == 0 && !sample) return LG_OK;
   // work queued earlier on the members' own streams, or by an earlier async
   // advance of this or another batch, ends first (the host waits for it)
   if (int rc = settle_members(b)) return rc;
   int rc = upload_desc(b, limits, n);
   if (rc) return rc;
-  rc = run_steps(b, n, b->calls > 0);
-  if (rc) return rc;
+  if (n) {
+    rc = run_steps(b, n, b->calls > 0);
+    if (rc) return rc;
+  }
+  if (sample && b->staging_floats) {
+    if (b->staging.n < b->staging_floats) HIP_TRY(b->staging.alloc(b->staging_floats));
+    uint32_t maxnb = 0;
+    for (lg_t* g : b->members)
+      if (g->has_field) maxnb = std::max(maxnb, g->bw * g->bh);
+    dim3 grid(blocks_for(maxnb, 4), (uint32_t)b->members.size());
+    hipLaunchKernelGGL(lg_snapshot_kernel, grid, dim3(256), 0, b->stream, b->desc.p, alpha,
+                       b->staging.p);
+    HIP_TRY(hipGetLastError());
+  }
   // a member's own calls, and its next batch, wait for this advance first
   // (settle): no wait is queued on the members' streams, which would hold up
   // unrelated work sharing their hardware queues
   HIP_TRY(hipEventRecord(b->done, b->stream));
   for (lg_t* g : b->members) g->after = b->done;
   commit_members(b, limits, n);
-  b->calls += 1;
+  if (n) b->calls += 1;
+  b->staged = sample;
+  return LG_OK;
+}
+
+int lg_batch_advance_async(lg_batch_t* b, uint64_t n, const uint64_t* limits) {
+  if (!b) return fail(LG_EINVAL, "null batch");
+  return advance_async(b, n, limits, false, 0.0f);
+}
+
+int lg_batch_advance_sampling_async(lg_batch_t* b, uint64_t n, const uint64_t* limits,
+                                    float alpha) {
+  if (!b) return fail(LG_EINVAL, "null batch");
+  return advance_async(b, n, limits, true, alpha);
+}
+
+int lg_batch_read_snapshot(lg_batch_t* b, float* out, size_t nfloats) {
+  if (!b || (!out && nfloats)) return fail(LG_EINVAL, "null argument");
+  if (!b->staged) return fail(LG_ESTATE, "no sampling advance since the last batch call");
+  if (nfloats != b->staging_floats)
+    return fail(LG_EINVAL, "output size differs from the members' fields (4 x blocks each)");
+  if (nfloats) HIP_TRY(hipMemcpyAsync(out, b->staging.p, nfloats * 4, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
   return LG_OK;
 }
 
@@ -620,6 +660,7 @@ int lg_batch_wait(lg_batch_t* b) {
 
 int lg_batch_snapshot(lg_batch_t* b, float alpha, float* out, size_t nfloats) {
   if (!b || !out) return fail(LG_EINVAL, "null argument");
+  b->staged = false;
   if (int rc = members_alive(b)) return rc;
   if (int rc = settle_members(b)) return rc;
   int rc = upload_desc(b, nullptr, 0);
@@ -709,6 +750,7 @@ int lg_batch_get_shape(lg_batch_t* b, uint32_t* tile, uint32_t* grid, uint32_t* 
 int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps) {
   if (!b || !used_steps) return fail(LG_EINVAL, "null argument");
   *used_steps = 0;
+  b->staged = false;
   if (int rc = members_alive(b)) return rc;
   if (int rc = settle_members(b)) return rc;
   int rc = upload_desc(b, nullptr, ~0ull);
@@ -780,6 +822,7 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
 // on the members' own streams, and wait.
 int lg_batch_sample(lg_batch_t* b, float alpha) {
   if (!b) return fail(LG_EINVAL, "null batch");
+  b->staged = false;
   if (int rc = members_alive(b)) return rc;
   for (lg_t* g : b->members) {
     int rc = lg_sample_field_async(g, alpha);

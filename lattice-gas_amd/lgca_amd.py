@@ -90,6 +90,8 @@ def load_library(path: str | None = None):
         "lg_batch_advance_async": (i32, [vp, u64, P(u64)]),
         "lg_batch_poll": (i32, [vp]),
         "lg_batch_wait": (i32, [vp]),
+        "lg_batch_advance_sampling_async": (i32, [vp, u64, P(u64), f32]),
+        "lg_batch_read_snapshot": (i32, [vp, P(f32), sz]),
         "lg_batch_snapshot": (i32, [vp, f32, P(f32), sz]),
         "lg_batch_snapshot_size": (sz, [vp]),
         "lg_sample_field_async": (i32, [vp, f32]),
@@ -376,6 +378,24 @@ class Batch:
         n = int(L.lg_batch_snapshot_size(self._p))
         buf = np.empty(n, dtype=np.float32)
         _check(L.lg_batch_snapshot(self._p, float(alpha), buf.ctypes.data_as(C.POINTER(C.c_float)), n))
+        return self._split(buf)
+
+    def advance_sampling_async(self, n: int, limits=None, alpha: float = 1.0):
+        """advance_async, then the sample of snapshot() queued behind it; fetch
+        the result with read_snapshot()."""
+        _check(load_library().lg_batch_advance_sampling_async(self._p, int(n), self._limits(limits),
+                                                              float(alpha)))
+
+    def read_snapshot(self):
+        """Wait for the batch and return the snapshot its last
+        advance_sampling_async took, per member as snapshot() does."""
+        L = load_library()
+        n = int(L.lg_batch_snapshot_size(self._p))
+        buf = np.empty(n, dtype=np.float32)
+        _check(L.lg_batch_read_snapshot(self._p, buf.ctypes.data_as(C.POINTER(C.c_float)), n))
+        return self._split(buf)
+
+    def _split(self, buf):
         out, off = [], 0
         for m in self.members:
             if m._field is None:

@@ -149,8 +149,9 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
     A run joins at a boundary, so its snapshots fall every `every` of its own
     steps and at its end, exactly as in a run of its own.
 
-    The advance is queued without waiting (Batch.advance_async), and while
-    the device runs it the host frees the lattices retired at the last
+    The advance and the snapshot behind it are queued without waiting
+    (Batch.advance_sampling_async), and while the device runs them the host
+    frees the lattices retired at the last
     boundary (a device free waits for the device, so this takes the rest of
     the advance) and then builds the runs that have arrived; those join at the
     next boundary. Both touch only lattices outside the batch, and a retired
@@ -217,7 +218,7 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
             spent["batch_setup"] += time.perf_counter() - ts
         limits = [a["req"].steps - a["done"] for a in active]
         ta = time.perf_counter()
-        batch.advance_async(every, limits)
+        batch.advance_sampling_async(every, limits, 1.0)
         try:
             timed("free_during_advance", free_retired)
             timed("admit_during_advance", admit_arrived)
@@ -225,7 +226,7 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
             batch.wait()            # the batch must be idle before anything else
         spent["advance"] += time.perf_counter() - ta
         quanta += 1
-        snaps = timed("snapshot", batch.snapshot, 1.0)
+        snaps = timed("snapshot", batch.read_snapshot)
         for a, lim, snap in zip(active, limits, snaps):
             a["done"] += min(every, lim)
             results[a["req"].rid].snapshots.append((a["done"],) + snap)

@@ -78,3 +78,35 @@ def test_batch_async_advance_orders_member_calls():
     bt.close()
     for g in solo + mem:
         g.close()
+
+
+def test_batch_sampling_advance_equals_advance_then_snapshot():
+    from lgca_amd import LgcaError
+    specs = [(256, 128, 1, 3), (192, 64, 2, 1), (100, 36, 3, 5), (640, 512, 6, 7)]
+    a, b = [lattice(*s) for s in specs], [lattice(*s) for s in specs]
+    for g in a[:3] + b[:3]:                  # the last member has no field
+        g.attach_field(8, 8)
+    ba, bb = Batch(a), Batch(b)
+    limits = [30, 12, 30, 30]                # member 1 stops early in every call
+    for _ in range(3):
+        ba.advance_limited(30, limits)
+        sa = ba.snapshot(0.5)
+        bb.advance_sampling_async(30, limits, 0.5)
+        sb = bb.read_snapshot()
+        assert sa[3] is None and sb[3] is None
+        for x, y in zip(sa[:3], sb[:3]):
+            assert all(np.array_equal(u, v) for u, v in zip(x, y))
+    for x, y in zip(a, b):
+        assert x.steps == y.steps and np.array_equal(x.store(), y.store())
+    bb.advance_sampling_async(0, None, 1.0)  # the sample alone
+    sb, sa = bb.read_snapshot(), ba.snapshot(1.0)
+    for x, y in zip(sa[:3], sb[:3]):
+        assert all(np.array_equal(u, v) for u, v in zip(x, y))
+    bb.advance_sampling_async(2, None, 1.0)
+    bb.advance(1)                            # another batch call discards the sample
+    with pytest.raises(LgcaError, match="sampling advance"):
+        bb.read_snapshot()
+    for bt in (ba, bb):
+        bt.close()
+    for g in a + b:
+        g.close()

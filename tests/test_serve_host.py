@@ -64,7 +64,7 @@ class _Batch:
                 self.errors.append("a member freed during the advance")
             m.steps += min(n, lim)
 
-    def advance_async(self, n, limits):
+    def advance_sampling_async(self, n, limits, alpha):
         assert not self.closed and self.run is None and len(limits) == len(self.members)
         self.run = threading.Thread(target=self._advance, args=(n, list(limits)))
         self.run.start()
@@ -74,7 +74,7 @@ class _Batch:
         self.run = None
         assert not self.errors, self.errors
 
-    def snapshot(self, alpha):
+    def read_snapshot(self):
         assert self.run is None
         return [(np.full(1, m.steps, np.float32),) * 4 for m in self.members]
 
