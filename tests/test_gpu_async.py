@@ -110,3 +110,22 @@ def test_batch_sampling_advance_equals_advance_then_snapshot():
         bt.close()
     for g in a + b:
         g.close()
+
+
+def test_destroy_with_async_work_queued():
+    """Destroying a batch or a lattice right after an async advance (no wait)
+    lets the queued steps finish first: the members hold the advanced state."""
+    specs = [(256, 128, 1, 3), (100, 36, 3, 5)]
+    solo, mem = [lattice(*s) for s in specs], [lattice(*s) for s in specs]
+    bt = Batch(mem)
+    bt.advance_async(20)
+    bt.close()                               # no wait
+    for s, m in zip(solo, mem):
+        s.advance(20)
+        assert m.poll() and m.steps == 20
+        assert np.array_equal(s.store(), m.store())
+    g = lattice(512, 256, 9, 4)
+    g.advance_async(50)
+    g.close()                                # queued steps end before the free
+    for x in solo + mem:
+        x.close()
