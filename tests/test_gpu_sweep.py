@@ -88,3 +88,51 @@ def test_random_configuration(oracle, i):
     for a, b in zip(g.read_field(), f.read()):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     g.close()
+
+
+@pytest.mark.parametrize("j", range(8))
+def test_random_batch(oracle, j):
+    """Batches of members drawn from the sweep's configurations (their own
+    sizes, bands, inflow and obstacles), a random pipeline and tile shape, and
+    two advances with random per-member limits: every member against its own
+    oracle replay."""
+    from lgca_amd import Batch
+    r = np.random.default_rng(0xBA7C0000 + j)
+    idx = [int(x) for x in r.choice(48, size=int(r.integers(2, 7)), replace=False)]
+    passes, units, draw = int(r.integers(1, 3)), int(r.choice([64, 32])), int(r.choice([64, 32]))
+    tile, chunk = int(r.choice([128, 256])), int(r.choice([2, 4, 8]))
+    print(f"batch {j}: members {idx} passes={passes} units={units} draw={draw} tile={tile} "
+          f"chunk={chunk}", flush=True)
+    sys.stdout.flush()
+    lats, gs = [], []
+    for i in idx:
+        c = case(i)
+        lat = oracle.Lattice(c["w"], c["h"], c["density"], (c["drift"], 0.0), c["inlet"],
+                             c["rest"], c["bands"], c["seed"])
+        if c["obstacle"]:
+            kind, args = c["obstacle"]
+            (lat.add_solid_box if kind == "box" else lat.add_solid_disc)(*args)
+        lat.init_equilibrium()
+        g = GpuLattice(c["w"], c["h"], c["rest"], c["seed"], c["bands"])
+        g.set_inlet(c["inlet"], c["density"], (c["drift"], 0.0))
+        g.load(lat.cells)
+        lats.append(lat)
+        gs.append(g)
+    b = Batch(gs)
+    b.set_passes(passes)
+    b.set_units(units)
+    b.set_draw_unit(draw)
+    b.set_shape(tile, 0, chunk)
+    for _ in range(2):
+        n = int(r.integers(5, 40))
+        limits = [int(x) for x in r.integers(0, n + 5, size=len(gs))]
+        for lat, lim in zip(lats, limits):
+            if min(n, lim):
+                lat.advance(min(n, lim))
+        b.advance_limited(n, limits)
+    for i, lat, g in zip(idx, lats, gs):
+        assert g.steps == lat.steps, i
+        assert np.array_equal(g.store(), lat.cells), f"member {i}"
+    b.close()
+    for g in gs:
+        g.close()
