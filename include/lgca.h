@@ -10,7 +10,9 @@
  *
  * Conventions
  *   - Every call is synchronous (returns after the device work finished), as
- *     the reference's methods are (src/gpu.rs:705,720,752).
+ *     the reference's methods are (src/gpu.rs:705,720,752), except the
+ *     *_async calls, which return once their work is queued (lg_poll and
+ *     lg_wait, lg_batch_poll and lg_batch_wait follow it).
  *   - Functions returning int give 0 on success and a negative code on error;
  *     the message is then available from lg_last_error() (thread-local). The
  *     library never aborts on bad input; where the reference asserts
