@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(256) lg_snapshot_kernel(const LgDesc* __restri
 struct lg_batch {
   std::vector<lg_t*> members;
   hipStream_t stream = nullptr;
-  hipEvent_t done = nullptr;  // end of the last advance, waited for by the members' streams
+  hipEvent_t done = nullptr;  // end of the last async advance; members' own calls wait for it
   DevBuf<LgDesc> desc;
   DevBuf<uint2> tiles128, tiles256;
   uint32_t ntiles128 = 0, ntiles256 = 0;
