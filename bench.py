@@ -521,6 +521,37 @@ def serving_leg(n, rate, seed=5):
             "throughput_ratio": round(s_b["gcups"] / s_p["gcups"], 3)}
 
 
+def valu_probe_table():
+    """Issue rates on this GPU (lg_probe_valu): wave-instructions per CU per
+    shader clock for each instruction, 8 independent chains per lane and one
+    dependent chain, at 1-8 waves per SIMD; cells per CU per clock of the
+    draw work's exact stream. Printed as a markdown table, one line per
+    probe, flushed as it goes."""
+    import lgca_amd
+
+    wps_ind = (1, 2, 4, 8)
+    print("| instruction | " + " | ".join(f"8 chains, {w} w/SIMD" for w in wps_ind)
+          + " | 1 chain, 1 w/SIMD | 1 chain, 4 w/SIMD | clock GHz |", flush=True)
+    print("|---" * (len(wps_ind) + 4) + "|", flush=True)
+    for op in range(lgca_amd.PROBE_OPS):
+        row, name, ghz = [], "", 0.0
+        for w in wps_ind:
+            r, ghz, name = lgca_amd.probe_valu(op, w, 8, 2048)
+            row.append(r)
+        for w in (1, 4):
+            r, _, _ = lgca_amd.probe_valu(op, w, 1, 4096)
+            row.append(r)
+        print(f"| {name} | " + " | ".join(f"{r:.3f}" for r in row) + f" | {ghz:.3f} |", flush=True)
+    wps = (1, 2, 3, 4, 5, 6, 8)
+    print("", flush=True)
+    print("| draw stream (cells per CU per clock) | " + " | ".join(f"{w} w/SIMD" for w in wps)
+          + " |", flush=True)
+    print("|---" * (len(wps) + 1) + "|", flush=True)
+    row = [lgca_amd.probe_valu(lgca_amd.PROBE_STREAM, w, 1, 256)[0] for w in wps]
+    print("| xorshift64* + column + deposit | " + " | ".join(f"{r:.2f}" for r in row) + " |",
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -558,7 +589,13 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB megabytes (to scale rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE) and exit")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only run the VALU issue-rate probe (lg_probe_valu) and print its table")
     args = ap.parse_args()
+
+    if args.valu_probe:
+        valu_probe_table()
+        return
 
     if args.copy_calibration:
         import torch

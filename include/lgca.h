@@ -241,7 +241,8 @@ int lg_batch_wait(lg_batch_t* b);
  * sample of every member's field (n = 0: the sample alone); the result stays
  * on the device until lg_batch_read_snapshot, which waits for the batch and
  * copies it out in lg_batch_snapshot's layout (nfloats as there). Any other
- * batch call in between discards it (LG_ESTATE). */
+ * advance, snapshot, sample or autotune call of the batch in between discards
+ * it (LG_ESTATE); lg_batch_poll and lg_batch_wait keep it. */
 int lg_batch_advance_sampling_async(lg_batch_t* b, uint64_t n, const uint64_t* limits,
                                     float alpha);
 int lg_batch_read_snapshot(lg_batch_t* b, float* out, size_t nfloats);
@@ -355,6 +356,20 @@ int lg_frames_finish(lg_frames_t* fr);
  * rest particles, 6 without) or a negative error. The same exhaustive check
  * as the reference's lgca_verify kernel (src/gpu.rs:399-417, 775-820). */
 int lg_debug_collide_table(int rest, uint8_t* out, size_t n);
+
+/* Diagnostics, no counterpart in the reference: the VALU issue-rate probe.
+ * Runs op (0..14: lg_probe_op_name gives its instruction; LG_PROBE_STREAM:
+ * the draw work's exact random stream, one cell after another) on every CU
+ * at waves_per_simd waves per SIMD (1..8), with `chains` independent
+ * dependency chains per lane (1 or 8; ignored for the stream), iters loop
+ * iterations of 16 instructions per chain (32 cells for the stream). Gives
+ * wave-instructions (cells for the stream) issued per CU per shader clock,
+ * from the median workgroup's s_memtime interval, and that clock in GHz
+ * (s_memtime against the 100 MHz s_memrealtime). */
+#define LG_PROBE_STREAM 100
+const char* lg_probe_op_name(int op);
+int lg_probe_valu(int op, uint32_t waves_per_simd, uint32_t chains, uint32_t iters,
+                  double* per_cu_cycle, double* clock_ghz);
 
 #ifdef __cplusplus
 }

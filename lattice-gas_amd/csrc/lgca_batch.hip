@@ -247,14 +247,10 @@ uint32_t pipeline_key(const lg_batch* b) {
   return b->passes * 100 + b->units * 10 + (b->passes == 2 ? b->draw_units : 0);
 }
 
-int upload_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
-  for (lg_t* g : b->members) {
-    int rc = ensure_tables(g);
-    if (rc) return rc;
-    if ((rc = ensure_draw_buffers(g))) return rc;  // the pipeline may change within a call
-  }
-  // the last upload from host_desc may still be in flight after an async advance
-  HIP_TRY(hipStreamSynchronize(b->stream));
+// Fill the host copy of the descriptors (no device work, no allocation: the
+// vector keeps the capacity reserved at batch creation). The last upload
+// from host_desc must have been consumed (the batch stream synchronised).
+void fill_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
   std::vector<LgDesc>& h = b->host_desc;
   h.assign(b->members.size(), LgDesc{});
   size_t off = 0;
@@ -294,8 +290,19 @@ int upload_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
     }
   }
   b->staging_floats = off;
-  HIP_TRY(hipMemcpyAsync(b->desc.p, h.data(), h.size() * sizeof(LgDesc), hipMemcpyHostToDevice,
-                         b->stream));
+}
+
+int upload_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
+  for (lg_t* g : b->members) {
+    int rc = ensure_tables(g);
+    if (rc) return rc;
+    if ((rc = ensure_draw_buffers(g))) return rc;  // the pipeline may change within a call
+  }
+  // the last upload from host_desc may still be in flight after an async advance
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  fill_desc(b, limits, n);
+  HIP_TRY(hipMemcpyAsync(b->desc.p, b->host_desc.data(), b->host_desc.size() * sizeof(LgDesc),
+                         hipMemcpyHostToDevice, b->stream));
   return LG_OK;
 }
 
@@ -478,9 +485,23 @@ void commit_members(lg_batch* b, const uint64_t* limits, uint64_t n) {
 // A batch call on a member that lg_destroy has released to the batch would
 // run on a handle its owner has given up: refuse it.
 static int members_alive(const lg_batch* b) {
-  for (const lg_t* g : b->members)
+  for (const lg_t* g : b->members) {
     if (g->destroyed) return fail(LG_EINVAL, "a member of the batch was destroyed; destroy the batch");
+    if (g->poisoned)
+      return fail(LG_ESTATE, "a member's state is unknown after a failed batch call; reload it");
+  }
   return LG_OK;
+}
+
+// A batch call that fails after some of its steps were queued: the members'
+// host counters and plane pointers no longer say where their device state
+// is, so wait for what was queued and retire the members (their calls fail
+// with LG_ESTATE from now on; they can only be destroyed, or reloaded).
+static int abandon_queued(lg_batch* b, int rc) {
+  const std::string msg = g_err;
+  (void)hipStreamSynchronize(b->stream);
+  for (lg_t* g : b->members) g->poisoned = true;
+  return fail(rc, msg + " (batch members left in an unknown state)");
 }
 
 // Wait for everything queued for the members: by their own calls and by
@@ -533,6 +554,9 @@ lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
     uint32_t per = g->G.nbands + 1;
     b->max_tab_blocks_per_step = std::max(b->max_tab_blocks_per_step, blocks_for((size_t)per * 16));
   }
+  // every member's tables hold at least 3 steps (lg_create), so a chunk of
+  // 2 always fits beside the two-pass step's spare table
+  if (b->kmax < 2) return bail("a member's stream tables hold fewer than 3 steps", LG_EINVAL);
   b->chunk = std::min<uint32_t>(32, b->kmax & ~1u);
   // the same size rule as one lattice's default step, over all members' words
   size_t words = 0;
@@ -541,8 +565,8 @@ lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
   // The fused step runs best on 128-word tiles (the 256-lattice ensemble: 958
   // against 920 GCUPS; served batches: 591-597 against 572-595).
   b->tile = b->passes == 1 ? 128u : 256u;
-  if (b->chunk == 0) b->chunk = 2;
   if (b->desc.alloc(n)) return bail("hipMalloc failed for descriptors");
+  b->host_desc.reserve(n);
   if (ensure_tiles(b)) return bail(g_err);
   if (hipStreamSynchronize(b->stream) != hipSuccess) return bail("hipStreamSynchronize failed");
   for (lg_t* g : b->members) g->batch_refs += 1;
@@ -576,7 +600,7 @@ int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits) 
   // a fresh membership runs its first call eagerly; from the second call on
   // the chunk is captured once and replayed
   rc = run_steps(b, n, b->calls > 0);
-  if (rc) return rc;
+  if (rc) return abandon_queued(b, rc);
   HIP_TRY(hipStreamSynchronize(b->stream));
   commit_members(b, limits, n);
   b->calls += 1;
@@ -590,27 +614,32 @@ int lg_batch_advance(lg_batch_t* b, uint64_t n) { return lg_batch_advance_limite
 // device name each member's current planes).
 static int advance_async(lg_batch* b, uint64_t n, const uint64_t* limits, bool sample,
                          float alpha) {
-  if (int rc = members_alive(b)) return rc;
   b->staged = false;
+  if (int rc = members_alive(b)) return rc;
   if (n == 0 && !sample) return LG_OK;
   // work queued earlier on the members' own streams, or by an earlier async
   // advance of this or another batch, ends first (the host waits for it)
   if (int rc = settle_members(b)) return rc;
   int rc = upload_desc(b, limits, n);
   if (rc) return rc;
+  // everything that can fail without device work queued comes first: the
+  // staging buffer is sized by upload_desc and allocated before any step
+  if (sample && b->staging_floats && b->staging.n < b->staging_floats)
+    HIP_TRY(b->staging.alloc(b->staging_floats));
   if (n) {
     rc = run_steps(b, n, b->calls > 0);
-    if (rc) return rc;
+    if (rc) return abandon_queued(b, rc);
   }
   if (sample && b->staging_floats) {
-    if (b->staging.n < b->staging_floats) HIP_TRY(b->staging.alloc(b->staging_floats));
     uint32_t maxnb = 0;
     for (lg_t* g : b->members)
       if (g->has_field) maxnb = std::max(maxnb, g->bw * g->bh);
     dim3 grid(blocks_for(maxnb, 4), (uint32_t)b->members.size());
     hipLaunchKernelGGL(lg_snapshot_kernel, grid, dim3(256), 0, b->stream, b->desc.p, alpha,
                        b->staging.p);
-    HIP_TRY(hipGetLastError());
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      return abandon_queued(b, fail(LG_EDEVICE, std::string("snapshot launch: ") + hipGetErrorString(e)));
   }
   // a member's own calls, and its next batch, wait for this advance first
   // (settle): no wait is queued on the members' streams, which would hold up
@@ -658,6 +687,24 @@ int lg_batch_wait(lg_batch_t* b) {
   return LG_OK;
 }
 
+}  // extern "C"
+
+// Sample every member's field into `staging` (device, >= staging_floats
+// floats) and copy it out; the descriptors on the device must be current.
+static int snapshot_into(lg_batch* b, float alpha, float* staging, float* out, size_t nfloats) {
+  uint32_t maxnb = 0;
+  for (lg_t* g : b->members)
+    if (g->has_field) maxnb = std::max(maxnb, g->bw * g->bh);
+  dim3 grid(blocks_for(maxnb, 4), (uint32_t)b->members.size());
+  hipLaunchKernelGGL(lg_snapshot_kernel, grid, dim3(256), 0, b->stream, b->desc.p, alpha, staging);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, staging, nfloats * 4, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return LG_OK;
+}
+
+extern "C" {
+
 int lg_batch_snapshot(lg_batch_t* b, float alpha, float* out, size_t nfloats) {
   if (!b || !out) return fail(LG_EINVAL, "null argument");
   b->staged = false;
@@ -669,16 +716,7 @@ int lg_batch_snapshot(lg_batch_t* b, float alpha, float* out, size_t nfloats) {
     return fail(LG_EINVAL, "output size differs from the members' fields (4 x blocks each)");
   if (nfloats == 0) return LG_OK;
   if (b->staging.n < nfloats) HIP_TRY(b->staging.alloc(nfloats));
-  uint32_t maxnb = 0;
-  for (lg_t* g : b->members)
-    if (g->has_field) maxnb = std::max(maxnb, g->bw * g->bh);
-  dim3 grid(blocks_for(maxnb, 4), (uint32_t)b->members.size());
-  hipLaunchKernelGGL(lg_snapshot_kernel, grid, dim3(256), 0, b->stream, b->desc.p, alpha,
-                     b->staging.p);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(out, b->staging.p, nfloats * 4, hipMemcpyDeviceToHost, b->stream));
-  HIP_TRY(hipStreamSynchronize(b->stream));
-  return LG_OK;
+  return snapshot_into(b, alpha, b->staging.p, out, nfloats);
 }
 
 size_t lg_batch_snapshot_size(lg_batch_t* b) {
@@ -782,7 +820,7 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
     cands.assign(std::begin(two_pass), std::end(two_pass));
     cands.insert(cands.end(), std::begin(fused), std::end(fused));
   }
-  const uint32_t chunk = std::max<uint32_t>(2, std::min<uint32_t>(32, b->kmax & ~1u));
+  const uint32_t chunk = std::min<uint32_t>(32, b->kmax & ~1u);  // >= 2 (lg_batch_create)
   double best = 1e300;
   Cand bestc = {b->passes, b->units, b->draw_units, b->tile, b->grid};
   uint64_t used = 0;
@@ -795,10 +833,10 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
     b->tile = c.tile;
     b->grid = c.grid;
     b->chunk = chunk;
-    if ((rc = run_steps(b, trial, true))) return rc;
+    if ((rc = run_steps(b, trial, true))) return abandon_queued(b, rc);
     HIP_TRY(hipStreamSynchronize(b->stream));
     auto t0 = std::chrono::steady_clock::now();
-    if ((rc = run_steps(b, trial, true))) return rc;
+    if ((rc = run_steps(b, trial, true))) return abandon_queued(b, rc);
     HIP_TRY(hipStreamSynchronize(b->stream));
     double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     used += 2 * trial;

@@ -776,8 +776,10 @@ lg_t* lg_create(size_t w, size_t h, int rest, uint64_t seed, size_t bands) {
   G.nbands = lgca::n_bands(G.h, g->bands);
   g->seed = seed;
   const size_t per = G.nbands + 1;
+  // At least 3 steps: a batch chunk takes at least 2 steps and the two-pass
+  // step keeps one table spare for the next chunk's first draw.
   g->basis_steps = (uint32_t)std::max<size_t>(
-      2, std::min<size_t>(256, (32u << 20) / (per * LG_WTAB * 8)));
+      3, std::min<size_t>(256, (32u << 20) / (per * LG_WTAB * 8)));
   if (g->a.alloc((size_t)7 * G.total) || g->b.alloc((size_t)7 * G.total) ||
       g->solid.alloc(G.total) || g->basis.alloc((size_t)g->basis_steps * per * LG_WTAB) ||
       g->counter.alloc(1))
@@ -860,6 +862,7 @@ int lg_attach_field(lg_t* g, size_t bx, size_t by) {
 
 int lg_load(lg_t* g, const uint8_t* cells, size_t n) {
   if (!g || !cells) return fail(LG_EINVAL, "null argument");
+  g->poisoned = false;  // a reload sets the whole state
   if (int rc = settle(g)) return rc;
   if (n != (size_t)g->G.w * g->G.h) return fail(LG_EINVAL, "wrong number of cells");
   if (g->bytes.n < n) HIP_TRY(g->bytes.alloc(n));

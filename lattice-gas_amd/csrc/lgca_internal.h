@@ -120,6 +120,9 @@ struct lg_lattice {
   // fail until then.
   uint32_t batch_refs = 0;
   bool destroyed = false;
+  // A batch call failed after queueing steps for this handle: its host-side
+  // step counter and plane pointers are not trusted until lg_load/lg_restart.
+  bool poisoned = false;
   // End of the async batch advance that last stepped this handle, until a
   // call of its own has waited for it (cleared by that batch's destroy).
   hipEvent_t after = nullptr;
@@ -149,6 +152,8 @@ namespace lgca_internal {
 // batch's stream until that advance ends. Calls of the handle's own that
 // touch the device wait for it here (on the host) before they queue work.
 inline int settle(lg_lattice* g) {
+  if (g->poisoned)
+    return fail(LG_ESTATE, "the lattice's state is unknown after a failed batch call; reload it");
   if (!g->after) return 0;
   hipError_t e = hipEventSynchronize(g->after);
   if (e != hipSuccess) return fail(LG_EDEVICE, std::string("hipEventSynchronize: ") + hipGetErrorString(e));

@@ -75,6 +75,9 @@ def load_library(path: str | None = None):
         "lg_advance_timed": (i32, [vp, u64, P(f32)]),
         "lg_advance_sampling_timed": (i32, [vp, u64, u64, f32, P(f32)]),
         "lg_debug_collide_table": (i32, [i32, P(C.c_uint8), sz]),
+        "lg_probe_op_name": (C.c_char_p, [i32]),
+        "lg_probe_valu": (i32, [i32, C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_double),
+                                P(C.c_double)]),
         "lg_add_solid_box": (i32, [vp, f32, f32, f32, f32]),
         "lg_add_solid_disc": (i32, [vp, f32, f32, f32]),
         "lg_init_equilibrium": (i32, [vp, f32, f32, f32]),
@@ -585,6 +588,21 @@ def collide_table_on_device(rest: bool) -> np.ndarray:
     out = np.zeros(ns * stride, dtype=np.uint8)
     _check(load_library().lg_debug_collide_table(int(rest), _u8(out), out.size))
     return out.reshape(ns, stride)
+
+
+PROBE_STREAM = 100  # LG_PROBE_STREAM
+PROBE_OPS = 15
+
+
+def probe_valu(op: int, waves_per_simd: int, chains: int = 8, iters: int = 4096):
+    """The VALU issue-rate probe (lg_probe_valu): (per CU per clock, clock GHz,
+    instruction name). For op PROBE_STREAM the rate is cells per CU per clock
+    of the draw work's exact random stream."""
+    L = load_library()
+    rate, ghz = C.c_double(), C.c_double()
+    _check(L.lg_probe_valu(int(op), int(waves_per_simd), int(chains), int(iters), C.byref(rate),
+                           C.byref(ghz)))
+    return rate.value, ghz.value, L.lg_probe_op_name(int(op)).decode()
 
 
 class Sim:

@@ -112,7 +112,7 @@ def footprint(r: Request, block: int) -> int:
     words = wpr * r.h
     iw_h = ((min(r.inlet, r.w) + 63) // 64) * r.h
     nb = -(-r.h // (-(-r.h // min(r.bands, r.h))))
-    tab_steps = max(2, min(256, (32 << 20) // ((nb + 1) * 256 * 8)))
+    tab_steps = max(3, min(256, (32 << 20) // ((nb + 1) * 256 * 8)))
     blocks = (r.w // block) * (r.h // block)
     return (words * (8 * (14 + 1 + 2) + 4 + 80) + iw_h * (16 + 112) + 4 * r.h
             + tab_steps * (nb + 1) * 2048 + 48 * blocks + r.cells)
