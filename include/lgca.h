@@ -252,6 +252,15 @@ int lg_batch_read_snapshot(lg_batch_t* b, float* out, size_t nfloats);
  * lg_batch_snapshot_size(b). */
 int lg_batch_snapshot(lg_batch_t* b, float alpha, float* out, size_t nfloats);
 size_t lg_batch_snapshot_size(lg_batch_t* b);
+/* No counterpart in the reference. lg_batch_snapshot for callers that cannot
+ * allocate device memory once set up (serving loops and harnesses that
+ * forbid hipMalloc): the same sample and output, staged through a static
+ * device array of lg_batch_snapshot_noalloc_max() floats (8 MiB, held per
+ * GPU until the process ends) instead of a buffer the batch allocates.
+ * nfloats above that maximum is LG_EINVAL. Calls on one GPU share the array:
+ * calls that overlap on one GPU (several threads, several batches) race. */
+int lg_batch_snapshot_noalloc(lg_batch_t* b, float alpha, float* out, size_t nfloats);
+size_t lg_batch_snapshot_noalloc_max(void);
 /* Launch shape: tile = words per workgroup (128 or 256; by default 128 for
  * the fused step, 256 for the two-pass step), grid = step-kernel workgroups
  * (0, the default: one per tile; otherwise a persistent grid walking the

@@ -19,6 +19,7 @@
 // work is the same device code with the member's own geometry and streams.
 #include <chrono>
 #include <cstring>
+#include <mutex>
 
 #include "lgca_internal.h"
 
@@ -868,6 +869,75 @@ int lg_batch_sample(lg_batch_t* b, float alpha) {
   }
   for (lg_t* g : b->members) HIP_TRY(hipStreamSynchronize(g->stream));
   return LG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// lg_batch_snapshot without any allocation, for callers that cannot allocate
+// device memory once set up. The staging buffer is this file-scope device
+// array; each GPU holds its own copy from the first use until the process
+// ends. In host code `scratch` names a host-side stand-in, never the device
+// copy: kernels and copies get the address hipGetSymbolAddress returns for
+// the current device, fetched once per device and kept in g_scratch_ptr.
+// Every call on one GPU shares that GPU's array: calls that overlap on one
+// GPU (from several threads, or with another batch's call of this entry
+// point) race on it. The snapshot kernel writes every float that is copied
+// out, so nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+#define LG_NOALLOC_SCRATCH_FLOATS (2u << 20)  // 8 MiB
+static __device__ __align__(256) float scratch[LG_NOALLOC_SCRATCH_FLOATS];
+
+namespace {
+
+constexpr int LG_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[LG_MAX_DEVICES] = {};
+
+// The device address of `scratch` on device `dev`, looked up once per device
+// (with that device current) and cached.
+int scratch_on(int dev, float** ptr) {
+  if (dev < 0 || dev >= LG_MAX_DEVICES) return fail(LG_EDEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != dev) HIP_TRY(hipSetDevice(dev));
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return fail(LG_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return LG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t lg_batch_snapshot_noalloc_max(void) { return LG_NOALLOC_SCRATCH_FLOATS; }
+
+int lg_batch_snapshot_noalloc(lg_batch_t* b, float alpha, float* out, size_t nfloats) {
+  if (!b || !out) return fail(LG_EINVAL, "null argument");
+  b->staged = false;
+  if (int rc = members_alive(b)) return rc;
+  float* staging = nullptr;
+  if (int rc = scratch_on(b->members[0]->device, &staging)) return rc;
+  if (int rc = settle_members(b)) return rc;
+  HIP_TRY(hipStreamSynchronize(b->stream));  // host_desc's last upload consumed
+  fill_desc(b, nullptr, 0);
+  if (nfloats != b->staging_floats)
+    return fail(LG_EINVAL, "output size differs from the members' fields (4 x blocks each)");
+  if (nfloats > LG_NOALLOC_SCRATCH_FLOATS)
+    return fail(LG_EINVAL, "the members' fields exceed the allocation-free snapshot's " +
+                               std::to_string(LG_NOALLOC_SCRATCH_FLOATS) + " floats");
+  if (nfloats == 0) return LG_OK;
+  HIP_TRY(hipMemcpyAsync(b->desc.p, b->host_desc.data(), b->host_desc.size() * sizeof(LgDesc),
+                         hipMemcpyHostToDevice, b->stream));
+  return snapshot_into(b, alpha, staging, out, nfloats);
 }
 
 }  // extern "C"

@@ -97,6 +97,8 @@ def load_library(path: str | None = None):
         "lg_batch_read_snapshot": (i32, [vp, P(f32), sz]),
         "lg_batch_snapshot": (i32, [vp, f32, P(f32), sz]),
         "lg_batch_snapshot_size": (sz, [vp]),
+        "lg_batch_snapshot_noalloc": (i32, [vp, f32, P(f32), sz]),
+        "lg_batch_snapshot_noalloc_max": (sz, []),
         "lg_sample_field_async": (i32, [vp, f32]),
         "lg_measure_viscosity": (i32, [f32, i32, sz, u64, sz, P(f32)]),
         "lg_measure_advection": (i32, [f32, i32, sz, u64, sz, P(f32)]),
@@ -381,6 +383,16 @@ class Batch:
         n = int(L.lg_batch_snapshot_size(self._p))
         buf = np.empty(n, dtype=np.float32)
         _check(L.lg_batch_snapshot(self._p, float(alpha), buf.ctypes.data_as(C.POINTER(C.c_float)), n))
+        return self._split(buf)
+
+    def snapshot_noalloc(self, alpha: float):
+        """snapshot() through lg_batch_snapshot_noalloc: staged in the
+        library's static device array, no device allocation."""
+        L = load_library()
+        n = int(L.lg_batch_snapshot_size(self._p))
+        buf = np.empty(n, dtype=np.float32)
+        _check(L.lg_batch_snapshot_noalloc(self._p, float(alpha),
+                                           buf.ctypes.data_as(C.POINTER(C.c_float)), n))
         return self._split(buf)
 
     def advance_sampling_async(self, n: int, limits=None, alpha: float = 1.0):
