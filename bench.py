@@ -568,6 +568,11 @@ def main():
     ap.add_argument("--draw-ahead", type=int, default=-1, metavar="N",
                     help="two-pass step: draws of up to N steps per launch on a second stream "
                          "(0: draws share the collide launches; -1: by lattice size)")
+    ap.add_argument("--temporal-depth", type=int, default=0, metavar="K",
+                    help="collide K steps per launch on bands of rows held in registers "
+                         "(lg_set_temporal_depth; 0: the engine's default)")
+    ap.add_argument("--tile-rows", type=int, default=0, metavar="R",
+                    help="valid rows per band of the temporally blocked collide (0: by size)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--ensemble", type=int, default=256, metavar="N",
@@ -630,6 +635,8 @@ def main():
     g.set_draw_unit(args.draw_cells_per_lane)
     if args.draw_ahead >= 0:
         g.set_draw_ahead(args.draw_ahead)
+    if args.temporal_depth:
+        g.set_temporal_depth(args.temporal_depth, args.tile_rows)
     g.attach_field(cfg["block"], cfg["block"])
     if args.warmup:
         g.advance_sampling(args.warmup, every, 1.0)

@@ -166,6 +166,32 @@ int lg_set_draw_ahead(lg_t* g, uint32_t steps);
  * columns, else 64. Results are identical. */
 int lg_set_draw_unit(lg_t* g, uint32_t cells_per_lane);
 
+/* New. Temporal blocking of the collide pass: each collide launch steps a
+ * band of rows `steps` steps (1..8) in registers, its rows' neighbours
+ * exchanged across lanes and through LDS, and stores once; the band's
+ * 2 x steps edge rows repeat the collision, never the draws (the digits come
+ * from the draw pass, which runs `steps` or more steps ahead on the second
+ * stream: lg_set_draw_ahead is raised to `steps` if lower). rows_per_tile =
+ * valid rows per band (0: about one band per CU; at most 32 - 2 x steps).
+ * Rows of at most 64 words (4096 cells) only; steps = 1 turns it off.
+ * Results are identical for every choice. */
+int lg_set_temporal_depth(lg_t* g, uint32_t steps, uint32_t rows_per_tile);
+
+/* New. Resident bands (on = 1): each call's steps run in one cooperative
+ * launch per chunk (a chunk ends at a sample or at 256 steps), in which one
+ * workgroup per band of rows keeps its rows in registers for the whole
+ * chunk, draws its own random digits one step ahead and exchanges only its
+ * edge rows with the neighbouring bands through global memory and per-band
+ * step counters. For rows of at most 64 words (4096 cells) and at most 12
+ * rows per band at one band per CU; other lattices keep the other
+ * pipelines. Results are identical. lg_get_resident returns 1 (and the band
+ * height and count) when the next advance runs resident, else 0. A wait
+ * between bands that runs out (it cannot hang the device: every wait is
+ * bounded) makes the call fail with LG_EDEVICE and the lattice must be
+ * reloaded. */
+int lg_set_resident(lg_t* g, uint32_t on);
+int lg_get_resident(lg_t* g, uint32_t* rows_per_band, uint32_t* bands);
+
 /* New: advance n steps and report the device time of the step kernels in ms:
  * the sum over HIP event pairs recorded on the handle's stream around each
  * run of back-to-back step launches (the random-table and sampling kernels

@@ -30,7 +30,7 @@ def build(verbose: bool = False) -> str:
                     "--out", os.path.join(CSRC, "lgca_collide.gen.h")], check=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB + ".tmp"
-    srcs = [os.path.join(CSRC, f) for f in ("lgca_engine.hip", "lgca_batch.hip", "lgca_probe.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("lgca_engine.hip", "lgca_batch.hip", "lgca_band.hip", "lgca_probe.hip")]
     # The frame writers are host-only C++: built by the host compiler into an
     # object that hipcc links into the same library.
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)

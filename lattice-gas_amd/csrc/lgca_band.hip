@@ -1,0 +1,392 @@
+// Resident-band pipeline: one persistent workgroup per band of rows steps
+// its rows through a whole chunk of steps without leaving registers.
+//
+// Rows of at most 64 words only: lane j holds word j of its row, wave w row
+// y0 + w of the band, for the whole launch. Per step each row gathers its six
+// neighbour words -- across lanes for x (DPP rotation, or ds_bpermute when
+// the row is shorter than the wave), through LDS from the neighbouring waves
+// for y, and, for a band's first and last row, from the neighbouring bands'
+// halo rows published in global memory -- collides with the digits it drew
+// for this step, publishes its new halo rows, draws the next step's digits
+// (the expensive, state-independent work, which hides the neighbours'
+// latency) and meets the other waves at one barrier.
+//
+// Bands meet through per-band step counters ("flags") and halo words, all
+// stored and loaded as agent-scope atomics (coherent across the GPU's XCDs;
+// see lg_band_wait for the ordering). Every poll is bounded: a wait that runs out records the launch's
+// epoch in an error word and every later wait of that launch returns at
+// once, so the kernel always ends and the host reports the failure. The
+// launch is cooperative, so every band's workgroup is resident at once.
+//
+// Flags count steps from a per-launch epoch that only grows (2^21 apart),
+// so a launch never reads an older launch's counts as its own; halo rows sit
+// in two slots by step parity, which the counters keep apart: a band writes
+// slot (s + 1) & 1 only after both neighbours have read slot (s - 1) & 1.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "lgca_internal.h"
+
+namespace {
+
+// Spin until *f >= v. On running out, record `epoch` in *err; return false
+// at once if *err already holds this launch's epoch.
+//
+// No L2 write-back or invalidate is needed around the counters: everything a
+// band hands to another -- halo words and counters -- moves as agent-scope
+// atomics (written through to, and read from, the device's coherence point,
+// past the per-XCD L2s), a publisher waits for its halo stores to complete
+// before it stores the counter (lg_band_publish), and a reader issues its
+// halo loads only after the counter load has returned the awaited value.
+// The compiler fences keep the compiler from moving the halo accesses across
+// the counter accesses; the agent-scope acquire/release fences the compiler
+// would emit instead (buffer_inv / buffer_wbl2 sc1) cost about 4.5 us per
+// exchange on MI355X and protect only plain loads and stores, which the
+// exchange does not use.
+__device__ __forceinline__ bool lg_band_wait(uint64_t* f, uint64_t v, uint64_t* err,
+                                             uint64_t* err_host, uint64_t epoch) {
+  for (uint32_t it = 0; it < LG_BAND_SPIN_LIMIT; ++it) {
+    if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= v) {
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      return true;
+    }
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __hip_atomic_store(err, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(err_host, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  return false;
+}
+
+// Store a band's step counter once its halo stores have completed.
+__device__ __forceinline__ void lg_band_publish(uint64_t* f, uint64_t v) {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);  // vmcnt(0) expcnt(0) lgkmcnt(0): the halo stores are done
+  __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t lg_ld_agent(const uint64_t* p) {
+  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lg_st_agent(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Word of the lane to the left / right in the row (periodic); see
+// lg_collide_k_kernel for the two forms.
+template <bool WAVEROW>
+__device__ __forceinline__ uint64_t lg_band_left(uint64_t x, int addr) {
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  if (WAVEROW) {
+    lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x13C, 0xF, 0xF, false);  // wave_ror:1
+    hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x13C, 0xF, 0xF, false);
+  } else {
+    lo = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)lo);
+    hi = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)hi);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+template <bool WAVEROW>
+__device__ __forceinline__ uint64_t lg_band_right(uint64_t x, int addr) {
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  if (WAVEROW) {
+    lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x134, 0xF, 0xF, false);  // wave_rol:1
+    hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x134, 0xF, 0xF, false);
+  } else {
+    lo = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)lo);
+    hi = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)hi);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// The collision digits of word (y, j) for one step (bt: its row band's
+// stream table of that step) for fluid mask fm: both halves, the low half's
+// digits in the low 32 bits of K[b] -- lg_draw_unit<1>'s digits, kept in
+// registers. A wave whose lanes all draw every cell of a half takes the
+// plain chain for it; otherwise the whole wave takes the branch-free masked
+// chain, which continues through the masked cells, so the upper half starts
+// where the lower one ended without a second jump.
+template <int NK>
+__device__ __forceinline__ void lg_band_digits(uint64_t fm, uint64_t cj0,
+                                               const uint64_t* __restrict bt, uint32_t stride,
+                                               uint64_t K[LG_KDIGITS]) {
+  uint32_t Kl[NK], Kh[NK];
+  uint64_t st = lg_jump_tab(cj0, bt);
+  const uint32_t f0 = (uint32_t)fm, f1 = (uint32_t)(fm >> 32);
+  if (__ballot(f0 != ~0u))
+    st = lg_draw_half_sel<NK>(st, f0, stride, Kl);
+  else
+    st = lg_draw_half<NK>(st, stride, Kl);
+  if (__ballot(f1 != ~0u))
+    lg_draw_half_sel<NK>(st, f1, stride, Kh);
+  else
+    lg_draw_half<NK>(st, stride, Kh);
+  LG_UNROLL
+  for (int b = 0; b < LG_KDIGITS; ++b) K[b] = b < NK ? ((uint64_t)Kh[b] << 32) | Kl[b] : 0ull;
+}
+
+}  // namespace
+
+// One band per workgroup (blockIdx.x), one row per wave, one word per lane;
+// A.n steps from A.src into A.dst. LDS: two exchange slots of the waves'
+// planes 1, 2, 4, 5 (blockDim / 64 x 4 KB).
+template <bool WAVEROW>
+__global__ void __launch_bounds__(64 * LG_BAND_MAX_ROWS) lg_band_kernel(lgca_internal::LgBandArgs A) {
+  extern __shared__ uint64_t xch[];  // [2 slots][NW waves][4 planes][64 lanes]
+  const LgGeom& G = A.G;
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, NW = blockDim.x >> 6;
+  const uint32_t b = blockIdx.x, y0 = b * A.rows;
+  const uint32_t Bb = min(A.rows, G.h - y0);  // rows of this band
+  const bool row_live = w < Bb;
+  const uint32_t y = y0 + (row_live ? w : 0u);
+  const bool live = lane < G.wpr;
+  const uint32_t j = live ? lane : 0u;
+  const uint32_t cl = (j == 0) ? G.lastbit : 63u, cr = (j == G.lastword) ? G.lastbit : 63u;
+  const uint64_t tail = (j == G.lastword) ? G.tailmask : ~0ull;
+  const int addr_l = (int)(4u * (live ? (j == 0 ? G.wpr - 1 : j - 1) : lane));
+  const int addr_r = (int)(4u * (live ? (j + 1 == G.wpr ? 0u : j + 1) : lane));
+  const uint32_t bu = (b + A.nb - 1) % A.nb, bd = (b + 1) % A.nb;
+  const bool top = w == 0, bot = w + 1 == Bb;
+  const size_t T = G.total, word = (size_t)y * G.wpr + j;
+  const uint32_t per = G.nbands + 1;
+  const size_t hstride = (size_t)A.nb * 128;  // one halo kind of one slot: nb x 2 planes x 64
+
+  uint64_t P[7];
+  LG_UNROLL
+  for (int d = 0; d < 7; ++d) P[d] = A.src[(size_t)d * T + word];
+  const uint64_t sm = A.solid[word];
+  const uint64_t fm = live ? ~sm & tail : 0ull;
+  const uint64_t cj0 = A.cjump[2 * word];
+  const uint32_t stream = A.row_band[y];
+  // inflow: for widths of at most 64 cells lane x samples cell x of the
+  // row's first word from its own jump (icell); wider inflow runs the
+  // serial per-word chain (ijump) in the lanes of the inflow words
+  const bool lane_inflow = G.inlet_cols <= 64;
+  const uint64_t sm0 = A.solid[(size_t)y * G.wpr];
+  const bool icell_live = lane_inflow && lane < G.inlet_cols && !((sm0 >> lane) & 1ull);
+  const uint64_t ic = icell_live ? A.icell[(size_t)y * G.inlet_cols + lane] : 0ull;
+  const uint64_t ij = (!lane_inflow && j < G.iw) ? A.ijump[2 * ((size_t)y * G.iw + j)] : 0ull;
+  uint64_t K[LG_KDIGITS], I[7];
+  LG_UNROLL
+  for (int q = 0; q < LG_KDIGITS; ++q) K[q] = 0;
+  LG_UNROLL
+  for (int d = 0; d < 7; ++d) I[d] = 0;
+  if (row_live) {
+    uint64_t* s0 = xch + (size_t)w * 256;
+    s0[0 * 64 + lane] = P[1];
+    s0[1 * 64 + lane] = P[2];
+    s0[2 * 64 + lane] = P[4];
+    s0[3 * 64 + lane] = P[5];
+  }
+  __syncthreads();
+
+  // Pass i collides step i - 1 (i > 0) and draws step i (i < n): one copy
+  // of each in the code.
+  for (uint32_t i = 0; i <= A.n; ++i) {
+    const uint32_t s = i - 1;  // the step collided in this pass
+    if (row_live && i > 0) {
+      const uint64_t* slot = xch + (size_t)(s & 1u) * NW * 256;
+      uint64_t a1, a2, a4, a5;
+      // planes 1, 2 of the row above
+      if (!top) {
+        a1 = slot[((w - 1) * 4 + 0) * 64 + lane];
+        a2 = slot[((w - 1) * 4 + 1) * 64 + lane];
+      } else if (s == 0) {
+        const size_t up = (size_t)((y0 + G.h - 1) % G.h) * G.wpr + j;
+        a1 = A.src[1 * T + up];
+        a2 = A.src[2 * T + up];
+      } else {
+        lg_band_wait(&A.flags[A.nb + bu], A.epoch + s, A.err, A.err_host, A.epoch);
+        const uint64_t* hb = A.halo + (size_t)(s & 1u) * 2 * hstride + hstride + (size_t)bu * 128;
+        a1 = lg_ld_agent(hb + lane);
+        a2 = lg_ld_agent(hb + 64 + lane);
+      }
+      // planes 4, 5 of the row below
+      if (!bot) {
+        a4 = slot[((w + 1) * 4 + 2) * 64 + lane];
+        a5 = slot[((w + 1) * 4 + 3) * 64 + lane];
+      } else if (s == 0) {
+        const size_t dn = (size_t)((y0 + Bb) % G.h) * G.wpr + j;
+        a4 = A.src[4 * T + dn];
+        a5 = A.src[5 * T + dn];
+      } else {
+        lg_band_wait(&A.flags[bd], A.epoch + s, A.err, A.err_host, A.epoch);
+        const uint64_t* ht = A.halo + (size_t)(s & 1u) * 2 * hstride + (size_t)bd * 128;
+        a4 = lg_ld_agent(ht + lane);
+        a5 = lg_ld_agent(ht + 64 + lane);
+      }
+      const uint32_t odd = y & 1u;
+      const uint64_t a0l = lg_band_left<WAVEROW>(P[0], addr_l);
+      const uint64_t a3r = lg_band_right<WAVEROW>(P[3], addr_r);
+      uint64_t a1l = 0, a5l = 0, a2r = 0, a4r = 0;
+      if (odd) {
+        a2r = lg_band_right<WAVEROW>(a2, addr_r);
+        a4r = lg_band_right<WAVEROW>(a4, addr_r);
+      } else {
+        a1l = lg_band_left<WAVEROW>(a1, addr_l);
+        a5l = lg_band_left<WAVEROW>(a5, addr_l);
+      }
+      uint64_t n[7];
+      lg_gather(odd, cl, cr, P[0], a0l, a1, a1l, a2, a2r, P[3], a3r, a4, a4r, a5, a5l, P[6], n);
+      uint32_t Kl[LG_KDIGITS], Kh[LG_KDIGITS];
+      LG_UNROLL
+      for (int q = 0; q < LG_KDIGITS; ++q) {
+        Kl[q] = (uint32_t)K[q];
+        Kh[q] = (uint32_t)(K[q] >> 32);
+      }
+      uint64_t res[7];
+      lg_collide_word(G, n, sm, Kl, Kh, res);
+      if (j < G.iw) lg_inflow_apply(G, j, sm, tail, I, res);
+      LG_UNROLL
+      for (int d = 0; d < 7; ++d) P[d] = res[d] & tail;
+
+      // publish the new rows: to the neighbouring waves, and the band's edge
+      // rows to the neighbouring bands
+      const uint32_t ns = (s + 1) & 1u;
+      uint64_t* out = xch + ((size_t)ns * NW + w) * 256;
+      out[0 * 64 + lane] = P[1];
+      out[1 * 64 + lane] = P[2];
+      out[2 * 64 + lane] = P[4];
+      out[3 * 64 + lane] = P[5];
+      if (top) {  // planes 4, 5 of the band's first row, for the band above
+        uint64_t* ht = A.halo + (size_t)ns * 2 * hstride + (size_t)b * 128;
+        lg_st_agent(ht + lane, P[4]);
+        lg_st_agent(ht + 64 + lane, P[5]);
+        lg_band_publish(&A.flags[b], A.epoch + s + 1);
+      }
+      if (bot) {  // planes 1, 2 of the band's last row, for the band below
+        uint64_t* hb = A.halo + (size_t)ns * 2 * hstride + hstride + (size_t)b * 128;
+        lg_st_agent(hb + lane, P[1]);
+        lg_st_agent(hb + 64 + lane, P[2]);
+        lg_band_publish(&A.flags[A.nb + b], A.epoch + s + 1);
+      }
+    }
+    // step i's draws
+    if (row_live && i < A.n) {
+      const uint64_t* tab = A.tabs + (size_t)i * per * LG_WTAB;
+      const uint64_t* itab = tab + (size_t)G.nbands * LG_WTAB;
+      uint64_t c0 = cj0, ci = ic, cw = ij;
+#if defined(__HIP_DEVICE_COMPILE__)
+      // the jumps' window offsets are the same every step; left to itself
+      // the compiler hoists them out of the loop and spills them
+      asm volatile("" : "+v"(c0), "+v"(ci), "+v"(cw));
+#endif
+      if (live) {
+        if (G.rest)
+          lg_band_digits<LG_COLLIDE_REST_KBITS>(fm, c0, tab + (size_t)stream * LG_WTAB,
+                                                LG_COLLIDE_REST_STRIDE, K);
+        else
+          lg_band_digits<LG_COLLIDE_NOREST_KBITS>(fm, c0, tab + (size_t)stream * LG_WTAB,
+                                                  LG_COLLIDE_NOREST_STRIDE, K);
+      }
+      if (lane_inflow) {
+        if (G.inlet_cols > 0) {  // cell `lane` of word 0, gathered across the wave
+          uint32_t bits = 0;
+          if (icell_live) {
+            uint64_t st = lg_jump_tab(ci, itab);
+            bits = lg_eq_cell(G, st);
+          }
+          LG_UNROLL
+          for (int d = 0; d < 7; ++d) I[d] = __ballot((bits >> d) & 1u);
+        }
+      } else if (live && j < G.iw) {
+        const uint64_t apply = lg_inlet_apply(G, j, fm);
+        if (apply) lg_inlet_bits(G, lg_jump_tab(cw, itab), apply, I);
+      }
+    }
+    __syncthreads();
+  }
+  if (row_live && live) {
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) A.dst[(size_t)d * T + word] = P[d];
+  }
+}
+
+namespace lgca_internal {
+
+bool band_eligible(const lg_t* g, uint32_t* rows, uint32_t* nb) {
+  const uint32_t ncu = (uint32_t)std::max(g->ncu, 1);
+  if (g->G.wpr > 64) return false;
+  const uint32_t r = (g->G.h + ncu - 1) / ncu;
+  if (r > LG_BAND_MAX_ROWS) return false;
+  *rows = std::max<uint32_t>(r, 1);
+  *nb = (g->G.h + *rows - 1) / *rows;
+  return true;
+}
+
+int ensure_band_buffers(lg_t* g, uint32_t nb) {
+  if (g->band_nb >= nb && g->band_halo.p) return LG_OK;
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  HIP_TRY(g->band_halo.alloc((size_t)2 * 2 * nb * 128));
+  HIP_TRY(g->band_flags.alloc((size_t)2 * nb));
+  HIP_TRY(hipMemsetAsync(g->band_flags.p, 0, (size_t)2 * nb * 8, g->stream));
+  if (!g->band_err_dev.p) {
+    HIP_TRY(g->band_err_dev.alloc(1));
+    HIP_TRY(hipMemsetAsync(g->band_err_dev.p, 0, 8, g->stream));
+  }
+  if (!g->band_err_host) {
+    HIP_TRY(hipHostMalloc((void**)&g->band_err_host, 8, hipHostMallocDefault));
+    *g->band_err_host = 0;
+  }
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->band_nb = nb;  // the epoch keeps growing: cleared flags lie below any target
+  return LG_OK;
+}
+
+// Launch n steps of the resident-band kernel from the handle's tables
+// (tabs: the chunk's first step) on its stream, planes a -> b.
+int launch_band(lg_t* g, uint32_t n, const uint64_t* tabs) {
+  uint32_t rows = 0, nb = 0;
+  if (!band_eligible(g, &rows, &nb)) return fail(LG_EINVAL, "lattice not eligible for resident bands");
+  if (int rc = ensure_band_buffers(g, nb)) return rc;
+  LgBandArgs A;
+  A.G = g->G;
+  A.src = g->a.p;
+  A.dst = g->b.p;
+  A.solid = g->solid.p;
+  A.cjump = g->cjump.p;
+  A.row_band = g->row_band.p;
+  A.ijump = g->ijump.p;
+  A.icell = g->icell.p;
+  A.tabs = tabs;
+  A.halo = g->band_halo.p;
+  A.flags = g->band_flags.p;
+  A.err = g->band_err_dev.p;
+  A.err_host = g->band_err_host;
+  g->band_epoch += 1ull << 21;
+  A.epoch = g->band_epoch;
+  A.n = n;
+  A.rows = rows;
+  A.nb = nb;
+  void* args[] = {&A};
+  const dim3 grid(nb), block(64 * rows);
+  const size_t lds = (size_t)rows * 4096;
+  hipError_t e;
+  if (g->G.wpr == 64)
+    e = hipLaunchCooperativeKernel((const void*)lg_band_kernel<true>, grid, block, args, lds, g->stream);
+  else
+    e = hipLaunchCooperativeKernel((const void*)lg_band_kernel<false>, grid, block, args, lds, g->stream);
+  if (e != hipSuccess)
+    return fail(LG_EDEVICE, std::string("resident-band launch: ") + hipGetErrorString(e));
+  g->band_check = true;
+  return LG_OK;
+}
+
+// After the handle's stream has drained: did a launch's wait run out?
+int check_band(lg_t* g) {
+  if (!g->band_check) return LG_OK;
+  g->band_check = false;
+  // epochs only grow: a failure since the last check carries a larger one
+  const uint64_t e = __atomic_load_n(g->band_err_host, __ATOMIC_ACQUIRE);
+  const bool failed = e > g->band_epoch_checked;
+  g->band_epoch_checked = g->band_epoch;
+  if (failed) {
+    g->poisoned = true;
+    return fail(LG_EDEVICE, "a resident-band wait timed out: the lattice's state is unknown");
+  }
+  return LG_OK;
+}
+
+}  // namespace lgca_internal

@@ -265,6 +265,44 @@ LG_HD inline uint64_t lg_draw_half(uint64_t st, uint32_t stride, uint32_t P[NK])
   return st;
 }
 
+// The same 32 cells when only the cells in fh draw (solid sites and cells
+// past the row's end take no draw, src/lattice.rs:358-362), without
+// branches: every cell advances a candidate state and keeps it only if it
+// draws, and a non-drawing cell's digits are zero (what lg_draw_masked32
+// gives). It returns the stream state after the drawing cells, so the next
+// half can continue the chain. About a fifth more work than lg_draw_half,
+// the same for every lane: a wave where some lane's half is partial runs it
+// for all its lanes instead of both paths one after the other.
+template <int NK>
+LG_HD inline uint64_t lg_draw_half_sel(uint64_t st, uint32_t fh, uint32_t stride, uint32_t P[NK]) {
+  uint32_t R[8];
+  LG_UNROLL
+  for (int m = 0; m < 8; ++m) R[m] = 0;
+  LG_UNROLL
+  for (uint32_t x = 0; x < 32; ++x) {
+    const uint64_t nx = lg_xs(st);
+    const bool take = (fh >> x) & 1u;
+    st = take ? nx : st;
+    uint32_t k = lg_kcol(lg_out32(st), stride);
+    k = take ? k : 0u;
+    if (x + NK <= 32)
+      R[x & 7] |= k << x;
+    else
+      R[x & 7] |= (k << x) | (k >> (32 - x));
+  }
+  LG_UNROLL
+  for (int b = 0; b < NK; ++b) {
+    uint32_t acc = 0;
+    LG_UNROLL
+    for (int m = 0; m < 8; ++m) {
+      const uint32_t sel = 0x01010101u << m;
+      acc = lg_op3<0xEA>(R[m], b ? ((sel << b) | (sel >> (32 - b))) : sel, acc);
+    }
+    P[b] = b ? ((acc >> b) | (acc << (32 - b))) : acc;
+  }
+  return st;
+}
+
 // General path: only the cells in `fm` draw, in order (solid sites and the
 // cells past the row's end take no draw, src/lattice.rs:358-362).
 template <int NK>
@@ -310,23 +348,30 @@ LG_HD inline void lg_draw_word(uint64_t st, uint64_t fm, uint32_t stride, uint32
 // Inflow re-seed / equilibrium sampling for the cells `apply` of one word
 // (src/lattice.rs:102-115, 197-208, 270-285): each applied cell takes 4 draws
 // (3 without rest) in order; bits[d] receives the seven sampled planes.
+// Equilibrium::sample of one cell (src/lattice.rs:102-115) from the stream
+// state st (advanced past its 4 draws, 3 without rest): bits 0-6.
+LG_HD inline uint32_t lg_eq_cell(const LgGeom& G, uint64_t& st) {
+  uint32_t s = 0;
+  for (int d = 0; d < 6; d += 2) {
+    st = lg_xs(st);
+    uint64_t r = st * LG_RNG_MUL;
+    s |= (uint32_t)((uint32_t)(r >> 40) < G.thresh[d]) << d;
+    s |= (uint32_t)(((uint32_t)(r >> 16) & 0xFFFFFFu) < G.thresh[d + 1]) << (d + 1);
+  }
+  if (G.rest) {
+    st = lg_xs(st);
+    uint64_t r = st * LG_RNG_MUL;
+    s |= (uint32_t)((uint32_t)(r >> 40) < G.thresh[6]) << 6;
+  }
+  return s;
+}
+
 LG_HD inline void lg_inlet_bits(const LgGeom& G, uint64_t st, uint64_t apply, uint64_t bits[7]) {
   for (int d = 0; d < 7; ++d) bits[d] = 0;
   for (uint32_t x = 0; x < 64; ++x) {
     if (!((apply >> x) & 1ull)) continue;
-    uint64_t s = 0;
-    for (int d = 0; d < 6; d += 2) {
-      st = lg_xs(st);
-      uint64_t r = st * LG_RNG_MUL;
-      s |= (uint64_t)((uint32_t)(r >> 40) < G.thresh[d]) << d;
-      s |= (uint64_t)(((uint32_t)(r >> 16) & 0xFFFFFFu) < G.thresh[d + 1]) << (d + 1);
-    }
-    if (G.rest) {
-      st = lg_xs(st);
-      uint64_t r = st * LG_RNG_MUL;
-      s |= (uint64_t)((uint32_t)(r >> 40) < G.thresh[6]) << 6;
-    }
-    for (int d = 0; d < 7; ++d) bits[d] |= ((s >> d) & 1ull) << x;
+    const uint32_t s = lg_eq_cell(G, st);
+    for (int d = 0; d < 7; ++d) bits[d] |= (uint64_t)((s >> d) & 1u) << x;
   }
 }
 LG_HD inline void lg_inlet_word(const LgGeom& G, uint64_t st, uint64_t apply, uint64_t res[7]) {
@@ -626,6 +671,53 @@ LG_HD inline void lg_draw_unit(const LgGeom& G, const uint64_t* __restrict solid
   }
 }
 
+// Gather of destination word (y, j) from its source words (parity `odd` of
+// row y): aD the plane-D word straight above/below/beside it as the table
+// of src/lattice.rs:333-356 says (planes 1, 2 from row y - 1; 4, 5 from row
+// y + 1; 0, 3, 6 from row y), aDl / aDr the same plane's word to the left /
+// right; cl = the bit taken from a left word (63, or lastbit for j = 0), cr =
+// where a right word's bit 0 lands (63, or lastbit for the last word).
+LG_HD inline void lg_gather(uint32_t odd, uint32_t cl, uint32_t cr, uint64_t a0, uint64_t a0l,
+                            uint64_t a1, uint64_t a1l, uint64_t a2, uint64_t a2r, uint64_t a3,
+                            uint64_t a3r, uint64_t a4, uint64_t a4r, uint64_t a5, uint64_t a5l,
+                            uint64_t a6, uint64_t n[7]) {
+  n[0] = lg_shl_in(a0, a0l, cl);
+  n[1] = odd ? a1 : lg_shl_in(a1, a1l, cl);
+  n[2] = odd ? lg_shr_in(a2, a2r, cr) : a2;
+  n[3] = lg_shr_in(a3, a3r, cr);
+  n[4] = odd ? lg_shr_in(a4, a4r, cr) : a4;
+  n[5] = odd ? a5 : lg_shl_in(a5, a5l, cl);
+  n[6] = a6;
+}
+
+// Collision of a whole gathered word with the digits of its two halves,
+// then bounce-back on the solid sites sm (src/hex.rs:45-48,
+// src/lattice.rs:358-359): res = the word's new planes before inflow and tail.
+LG_HD inline void lg_collide_word(const LgGeom& G, const uint64_t n[7], uint64_t sm,
+                                  const uint32_t Kl[], const uint32_t Kh[], uint64_t res[7]) {
+  uint64_t o[7];
+  uint32_t nh[7], oh[7];
+  LG_UNROLL
+  for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)n[d];
+  if (G.rest)
+    lg_collide_rest<uint32_t>(nh, Kl, oh);
+  else
+    lg_collide_norest<uint32_t>(nh, Kl, oh);
+  LG_UNROLL
+  for (int d = 0; d < 7; ++d) o[d] = oh[d];
+  LG_UNROLL
+  for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)(n[d] >> 32);
+  if (G.rest)
+    lg_collide_rest<uint32_t>(nh, Kh, oh);
+  else
+    lg_collide_norest<uint32_t>(nh, Kh, oh);
+  LG_UNROLL
+  for (int d = 0; d < 7; ++d) o[d] |= (uint64_t)oh[d] << 32;
+  LG_UNROLL
+  for (int d = 0; d < 6; ++d) res[d] = (sm & n[(d + 3) % 6]) | (~sm & o[d]);
+  res[6] = (sm & n[6]) | (~sm & o[6]);
+}
+
 // Collide pass for one unit of word (y, j): the whole word (NH = 1) or its
 // half h (NH = 2); kplanes/iplanes are the step's digit and inflow planes.
 template <int NH>
@@ -665,28 +757,15 @@ LG_HD inline void lg_collide_unit(const LgGeom& G, const uint64_t* __restrict sr
   const uint64_t a6 = src[6 * T + word];
 
   uint64_t n[7];
-  n[0] = lg_shl_in(a0, a0l, cl);
-  n[1] = odd ? a1 : lg_shl_in(a1, a1l, cl);
-  n[2] = odd ? lg_shr_in(a2, a2r, cr) : a2;
-  n[3] = lg_shr_in(a3, a3r, cr);
-  n[4] = odd ? lg_shr_in(a4, a4r, cr) : a4;
-  n[5] = odd ? a5 : lg_shl_in(a5, a5l, cl);
-  n[6] = a6;
+  lg_gather(odd, cl, cr, a0, a0l, a1, a1l, a2, a2r, a3, a3r, a4, a4r, a5, a5l, a6, n);
 
-  uint64_t o[7];
-  {
+  uint64_t res[7];
+  if (NH == 1) {
+    lg_collide_word(G, n, sm, Kl, Kh, res);
+  } else {
+    uint64_t o[7];
     uint32_t nh[7], oh[7];
-    if (NH == 1) {
-      LG_UNROLL
-      for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)n[d];
-      if (G.rest)
-        lg_collide_rest<uint32_t>(nh, Kl, oh);
-      else
-        lg_collide_norest<uint32_t>(nh, Kl, oh);
-      LG_UNROLL
-      for (int d = 0; d < 7; ++d) o[d] = oh[d];
-    }
-    const uint32_t sh = NH == 1 ? 32u : 32u * h;
+    const uint32_t sh = 32u * h;
     LG_UNROLL
     for (int d = 0; d < 7; ++d) nh[d] = (uint32_t)(n[d] >> sh);
     if (G.rest)
@@ -694,14 +773,12 @@ LG_HD inline void lg_collide_unit(const LgGeom& G, const uint64_t* __restrict sr
     else
       lg_collide_norest<uint32_t>(nh, Kh, oh);
     LG_UNROLL
-    for (int d = 0; d < 7; ++d) o[d] = (NH == 1 ? o[d] : 0ull) | ((uint64_t)oh[d] << sh);
+    for (int d = 0; d < 7; ++d) o[d] = (uint64_t)oh[d] << sh;
+    // walls: bounce back, no collision (src/hex.rs:45-48, src/lattice.rs:358-359)
+    LG_UNROLL
+    for (int d = 0; d < 6; ++d) res[d] = (sm & n[(d + 3) % 6]) | (~sm & o[d]);
+    res[6] = (sm & n[6]) | (~sm & o[6]);
   }
-
-  // walls: bounce back, no collision (src/hex.rs:45-48, src/lattice.rs:358-359)
-  uint64_t res[7];
-  LG_UNROLL
-  for (int d = 0; d < 6; ++d) res[d] = (sm & n[(d + 3) % 6]) | (~sm & o[d]);
-  res[6] = (sm & n[6]) | (~sm & o[6]);
 
   if (j < G.iw) {
     const uint64_t mine = NH == 1 ? ~0ull : (0xFFFFFFFFull << (32 * h));
@@ -726,3 +803,31 @@ LG_HD inline void lg_collide_unit(const LgGeom& G, const uint64_t* __restrict sr
       dst32[2 * ((size_t)d * T + word) + h] = (uint32_t)((res[d] & tail) >> (32 * h));
   }
 }
+
+// ---------------------------------------------------------------------------
+// Temporal blocking of the collide pass. A tile is a band of L rows, full
+// width, stepped `ks` steps in registers before one store: tile t holds the
+// rows t R - ks .. t R + R + ks - 1 (mod h), R = L - 2 ks valid rows. A row
+// next to the band's edge sees zero planes beyond it and goes wrong after the
+// first step, its neighbour after the second: after ks steps the middle R
+// rows are exact, and only those are stored. The random digits of every step
+// come from the draw pass (they depend on the step, not on the state), so
+// the edge rows repeat only the gather and the circuit.
+// ---------------------------------------------------------------------------
+
+// Global row of tile-local row i of tile t (R valid rows, ks halo rows).
+LG_HD inline uint32_t lg_tile_row(uint32_t t, uint32_t i, uint32_t R, uint32_t ks, uint32_t h) {
+  const uint32_t base = (uint32_t)(((uint64_t)t * R + i) % h);
+  const uint32_t back = ks % h;
+  return base >= back ? base - back : base + h - back;
+}
+
+// Inflow re-seed of word (y, j < iw) from its sampled planes bits[7]: the
+// fluid cells of the inflow columns take them (src/lattice.rs:270-285).
+LG_HD inline void lg_inflow_apply(const LgGeom& G, uint32_t j, uint64_t sm, uint64_t tail,
+                                  const uint64_t bits[7], uint64_t res[7]) {
+  const uint64_t apply = lg_inlet_apply(G, j, ~sm & tail);
+  LG_UNROLL
+  for (int d = 0; d < 7; ++d) res[d] = (res[d] & ~apply) | (bits[d] & apply);
+}
+

@@ -160,6 +160,157 @@ lg_collide_kernel(LgGeom G, const uint64_t* __restrict src, uint64_t* __restrict
                       word - y * G.wpr, NH == 1 ? 0u : (u & 1u));
 }
 
+// Word of the lane to the left (j - 1) or right (j + 1) of this one in the
+// row, periodic in x. WAVEROW: the row is exactly the 64 lanes of the wave
+// (64 words), so a whole-wave DPP rotation by one lane is the neighbour
+// (wave_ror:1: lane n reads lane n - 1, lane 0 reads lane 63; wave_rol:1 the
+// other way). Otherwise ds_bpermute from the byte address of the neighbour
+// lane (addr), which wraps at the row's own length.
+template <bool WAVEROW>
+__device__ __forceinline__ uint64_t lg_lane_left(uint64_t x, int addr) {
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  if (WAVEROW) {
+    lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x13C, 0xF, 0xF, false);
+    hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x13C, 0xF, 0xF, false);
+  } else {
+    lo = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)lo);
+    hi = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)hi);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+template <bool WAVEROW>
+__device__ __forceinline__ uint64_t lg_lane_right(uint64_t x, int addr) {
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  if (WAVEROW) {
+    lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x134, 0xF, 0xF, false);
+    hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x134, 0xF, 0xF, false);
+  } else {
+    lo = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)lo);
+    hi = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)hi);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Temporally blocked collide pass (rows <= 64 words): workgroup t steps tile
+// t of the lattice (see lg_tile_row) ks steps with the digit and inflow
+// planes of those steps (kb0 / ib0: the first step's slot, the next steps'
+// slots following), then stores its R valid rows. Lane j holds word j of its
+// rows (lanes past the row's end idle along), wave w holds tile rows w M ..
+// w M + M - 1 in registers: rows inside a wave meet in the lane, the wave's
+// first and last rows meet the neighbouring waves' through LDS (one barrier
+// per step, two exchange slots alternating), words meet their left and right
+// neighbours across lanes. Every global access is to a row y < h (mod h) and
+// word j < wpr; the launch has ceil(h / R) workgroups of L / M waves and
+// L / M * 4 KB of dynamic LDS.
+template <int M, bool WAVEROW>
+__global__ void __launch_bounds__(1024)
+lg_collide_k_kernel(LgGeom G, const uint64_t* __restrict src, uint64_t* __restrict dst,
+                    const uint64_t* __restrict solid, const uint32_t* __restrict kb0,
+                    const uint32_t* __restrict ib0, uint32_t ks, uint32_t R) {
+  extern __shared__ uint64_t xch[];  // [2 slots][NW waves][4 planes][64 lanes]
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, NW = blockDim.x >> 6;
+  const uint32_t L = NW * M;
+  const size_t T = G.total, kstride = 2 * T, istride = 2 * (size_t)G.iw * G.h;
+  const bool live = lane < G.wpr;
+  const uint32_t j = live ? lane : 0u;
+  const uint32_t cl = (j == 0) ? G.lastbit : 63u, cr = (j == G.lastword) ? G.lastbit : 63u;
+  const uint64_t tail = (j == G.lastword) ? G.tailmask : ~0ull;
+  const int addr_l = (int)(4u * (live ? (j == 0 ? G.wpr - 1 : j - 1) : lane));
+  const int addr_r = (int)(4u * (live ? (j + 1 == G.wpr ? 0u : j + 1) : lane));
+  const int nk = G.rest ? LG_COLLIDE_REST_KBITS : LG_COLLIDE_NOREST_KBITS;
+
+  uint64_t P[M][7], S[M];
+  uint32_t Y[M];
+  LG_UNROLL
+  for (int m = 0; m < M; ++m) {
+    Y[m] = lg_tile_row(blockIdx.x, w * M + m, R, ks, G.h);
+    const size_t word = (size_t)Y[m] * G.wpr + j;
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) P[m][d] = src[(size_t)d * T + word];
+    S[m] = solid[word];
+  }
+  for (uint32_t s = 0; s < ks; ++s) {
+    const uint32_t* __restrict kb = kb0 + (size_t)s * LG_KDIGITS * kstride;
+    const uint32_t* __restrict ib = ib0 + (size_t)s * 7 * istride;
+    // this step's digits of every row (in flight across the exchange)
+    uint64_t K[M][LG_KDIGITS];
+    LG_UNROLL
+    for (int m = 0; m < M; ++m) {
+      const size_t word = (size_t)Y[m] * G.wpr + j;
+      LG_UNROLL
+      for (int b = 0; b < LG_KDIGITS; ++b)
+        K[m][b] = b < nk ? reinterpret_cast<const uint64_t*>(kb + (size_t)b * kstride)[word] : 0ull;
+    }
+    uint64_t* slot = xch + (size_t)(s & 1u) * NW * 256;
+    slot[(w * 4 + 0) * 64 + lane] = P[0][4];
+    slot[(w * 4 + 1) * 64 + lane] = P[0][5];
+    slot[(w * 4 + 2) * 64 + lane] = P[M - 1][1];
+    slot[(w * 4 + 3) * 64 + lane] = P[M - 1][2];
+    __syncthreads();
+    // planes 1, 2 of the row above this wave's first row, 4, 5 of the row
+    // below its last (zero beyond the tile's edges)
+    uint64_t a1 = 0, a2 = 0, dn4 = 0, dn5 = 0;
+    if (w > 0) {
+      a1 = slot[((w - 1) * 4 + 2) * 64 + lane];
+      a2 = slot[((w - 1) * 4 + 3) * 64 + lane];
+    }
+    if (w + 1 < NW) {
+      dn4 = slot[((w + 1) * 4 + 0) * 64 + lane];
+      dn5 = slot[((w + 1) * 4 + 1) * 64 + lane];
+    }
+    LG_UNROLL
+    for (int m = 0; m < M; ++m) {
+      const uint64_t a4 = (m + 1 < M) ? P[m + 1][4] : dn4;
+      const uint64_t a5 = (m + 1 < M) ? P[m + 1][5] : dn5;
+      const uint32_t odd = Y[m] & 1u;  // the same for the whole wave
+      const uint64_t a0l = lg_lane_left<WAVEROW>(P[m][0], addr_l);
+      const uint64_t a3r = lg_lane_right<WAVEROW>(P[m][3], addr_r);
+      uint64_t a1l = 0, a5l = 0, a2r = 0, a4r = 0;
+      if (odd) {
+        a2r = lg_lane_right<WAVEROW>(a2, addr_r);
+        a4r = lg_lane_right<WAVEROW>(a4, addr_r);
+      } else {
+        a1l = lg_lane_left<WAVEROW>(a1, addr_l);
+        a5l = lg_lane_left<WAVEROW>(a5, addr_l);
+      }
+      uint64_t n[7];
+      lg_gather(odd, cl, cr, P[m][0], a0l, a1, a1l, a2, a2r, P[m][3], a3r, a4, a4r, a5, a5l,
+                P[m][6], n);
+      const uint64_t o1 = P[m][1], o2 = P[m][2];  // row m's old planes 1, 2, for row m + 1
+      uint32_t Kl[LG_KDIGITS], Kh[LG_KDIGITS];
+      LG_UNROLL
+      for (int b = 0; b < LG_KDIGITS; ++b) {
+        Kl[b] = (uint32_t)K[m][b];
+        Kh[b] = (uint32_t)(K[m][b] >> 32);
+      }
+      uint64_t res[7];
+      lg_collide_word(G, n, S[m], Kl, Kh, res);
+      if (j < G.iw) {
+        const uint64_t* __restrict iin =
+            reinterpret_cast<const uint64_t*>(ib) + (size_t)Y[m] * G.iw + j;
+        uint64_t bits[7];
+        LG_UNROLL
+        for (int d = 0; d < 7; ++d) bits[d] = iin[(size_t)d * (istride / 2)];
+        lg_inflow_apply(G, j, S[m], tail, bits, res);
+      }
+      LG_UNROLL
+      for (int d = 0; d < 7; ++d) P[m][d] = res[d] & tail;
+      a1 = o1;
+      a2 = o2;
+    }
+  }
+  if (!live) return;
+  LG_UNROLL
+  for (int m = 0; m < M; ++m) {
+    const uint32_t i = w * M + m;
+    // valid rows only, and each global row once (the last tile may run past h)
+    if (i < ks || i >= L - ks || (uint64_t)blockIdx.x * R + (i - ks) >= G.h) continue;
+    const size_t word = (size_t)Y[m] * G.wpr + j;
+    LG_UNROLL
+    for (int d = 0; d < 7; ++d) dst[(size_t)d * T + word] = P[m][d];
+  }
+}
+
 // Bytes -> planes (GpuLattice::load, src/gpu.rs:572-610), one thread per word.
 __global__ void __launch_bounds__(256) lg_pack_kernel(LgGeom G, const uint8_t* __restrict cells,
                                                       uint64_t* __restrict planes,
@@ -432,10 +583,16 @@ void destroy_lattice(lg_lattice* g) {
   g->solid.release();
   g->cjump.release();
   g->ijump.release();
+  g->icell.release();
   g->basis.release();
   g->row_band.release();
   g->order.release();
   g->bytes.release();
+  g->band_halo.release();
+  g->band_flags.release();
+  g->band_err_dev.release();
+  if (g->band_err_host) (void)hipHostFree(g->band_err_host);
+  g->band_err_host = nullptr;
   g->fux.release();
   g->fuy.release();
   g->frho.release();
@@ -476,6 +633,12 @@ int rebuild_tables(lg_t* g, const std::vector<uint64_t>& solid_host) {
   HIP_TRY(hipMemcpyAsync(g->order.p, ord.data(), ord.size() * 4, hipMemcpyHostToDevice, g->stream));
   if (!ij.empty())
     HIP_TRY(hipMemcpyAsync(g->ijump.p, ij.data(), ij.size() * 8, hipMemcpyHostToDevice, g->stream));
+  std::vector<uint64_t> ic;
+  if (g->G.inlet_cols > 0 && g->G.inlet_cols <= 64) {  // resident bands' per-cell inflow jumps
+    lgca::build_inflow_cells(g->G.w, g->G.h, g->G.inlet_cols, g->G.rest != 0, solid_host, ic);
+    HIP_TRY(g->icell.alloc(ic.size()));
+    HIP_TRY(hipMemcpyAsync(g->icell.p, ic.data(), ic.size() * 8, hipMemcpyHostToDevice, g->stream));
+  }
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->tables_dirty = false;
   return LG_OK;
@@ -555,6 +718,47 @@ void launch_step2(lg_t* g, uint32_t cur, bool collide, bool draw, const uint64_t
                      kn, in, cblocks, dblocks);
 }
 
+// Steps per collide launch: the temporal depth where it applies (draw-ahead
+// pipeline, rows of at most 64 words), else 1.
+uint32_t collide_depth(const lg_t* g) {
+  if (g->tdepth < 2 || g->ahead == 0 || g->G.wpr > 64) return 1;
+  return g->tdepth;
+}
+
+// Tile shape of a temporally blocked collide of ks steps: rows per wave M,
+// waves per workgroup NW (<= 16), valid rows per tile R. By default about one
+// tile per CU (R = ceil(h / CUs)), grown to fill whole waves.
+struct TileShape {
+  uint32_t M, NW, R, ntiles;
+};
+TileShape tile_shape(const lg_t* g, uint32_t ks) {
+  TileShape t;
+  t.M = 2;
+  uint32_t R = g->trows ? g->trows : (g->G.h + (uint32_t)g->ncu - 1) / (uint32_t)g->ncu;
+  R = std::max<uint32_t>(R, 1);
+  uint32_t L = R + 2 * ks;
+  t.NW = (L + t.M - 1) / t.M;
+  if (t.NW > 16) t.NW = 16;
+  L = t.NW * t.M;
+  t.R = L - 2 * ks;  // >= 1: ks <= LG_MAX_TDEPTH keeps 2 ks < 32
+  t.ntiles = (g->G.h + t.R - 1) / t.R;
+  return t;
+}
+
+// One temporally blocked collide launch of ks steps from the digit and
+// inflow slots kc / ic (steps following each other) on the handle's stream.
+void launch_collide_k(lg_t* g, const uint32_t* kc, const uint32_t* ic, uint32_t ks) {
+  const TileShape t = tile_shape(g, ks);
+  const dim3 grid(t.ntiles), block(64 * t.NW);
+  const size_t lds = (size_t)t.NW * 4096;
+  if (g->G.wpr == 64)
+    hipLaunchKernelGGL((lg_collide_k_kernel<2, true>), grid, block, lds, g->stream, g->G, g->a.p,
+                       g->b.p, g->solid.p, kc, ic, ks, t.R);
+  else
+    hipLaunchKernelGGL((lg_collide_k_kernel<2, false>), grid, block, lds, g->stream, g->G, g->a.p,
+                       g->b.p, g->solid.p, kc, ic, ks, t.R);
+}
+
 // The second stream and the events of the draw-ahead pipeline (once).
 int ensure_ahead_stream(lg_t* g) {
   if (g->dstream) return LG_OK;
@@ -589,9 +793,18 @@ int enqueue_ahead_group(lg_t* g, uint32_t s0, uint32_t nd) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(g->ev_drawn[q], g->dstream));
   HIP_TRY(hipStreamWaitEvent(g->stream, g->ev_drawn[q], 0));
-  for (uint32_t k = 0; k < nd; ++k) {
+  const uint32_t kd = collide_depth(g);
+  for (uint32_t k = 0; k < nd;) {
     const uint32_t* kc = kb + (size_t)k * LG_KDIGITS * nhalf;
     const uint32_t* ic = ib + (size_t)k * 7 * nih;
+    const uint32_t ks = std::min(kd, nd - k);
+    if (ks >= 2) {
+      launch_collide_k(g, kc, ic, ks);
+      std::swap(g->a.p, g->b.p);
+      g->steps += ks;
+      k += ks;
+      continue;
+    }
     if (g->units == 2)
       hipLaunchKernelGGL(lg_collide_kernel<2>, dim3(blocks_for(nhalf)), dim3(256), 0, g->stream,
                          g->G, g->a.p, g->b.p, g->solid.p, kc, ic);
@@ -600,6 +813,7 @@ int enqueue_ahead_group(lg_t* g, uint32_t s0, uint32_t nd) {
                          g->stream, g->G, g->a.p, g->b.p, g->solid.p, kc, ic);
     std::swap(g->a.p, g->b.p);
     g->steps += 1;
+    k += 1;
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(g->ev_used[q], g->stream));
@@ -623,6 +837,37 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
     // the draw stream starts with this call's work on the main stream
     HIP_TRY(hipEventRecord(g->ev_start, g->stream));
     HIP_TRY(hipStreamWaitEvent(g->dstream, g->ev_start, 0));
+  }
+  uint32_t brows = 0, bnb = 0;
+  if (g->resident && band_eligible(g, &brows, &bnb)) {
+    // resident bands: per chunk, the chunk's stream tables, then one launch
+    while (left > 0) {
+      uint64_t chunk = std::min<uint64_t>(left, g->basis_steps);
+      if (every > 0) chunk = std::min<uint64_t>(chunk, every - since);
+      hipLaunchKernelGGL(lg_tables_kernel, dim3(blocks_for((size_t)chunk * per * 16)), dim3(256), 0,
+                         g->stream, g->seed, g->steps, (uint32_t)chunk, g->G.nbands, g->basis.p);
+      HIP_TRY(hipGetLastError());
+      if (timed) {
+        while (g->tev.size() < 2 * (g->tev_used + 1)) {
+          hipEvent_t e;
+          HIP_TRY(hipEventCreate(&e));
+          g->tev.push_back(e);
+        }
+        HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used], g->stream));
+      }
+      if (int rc = launch_band(g, (uint32_t)chunk, g->basis.p)) return rc;
+      if (timed) HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used++ + 1], g->stream));
+      std::swap(g->a.p, g->b.p);
+      g->steps += chunk;
+      left -= chunk;
+      since += chunk;
+      if (every > 0 && (since >= every || left == 0)) {
+        since = 0;
+        int rc = sample_now(g, alpha);
+        if (rc) return rc;
+      }
+    }
+    return LG_OK;
   }
   while (left > 0) {
     const uint32_t extra = (passes == 2 && !ahead) ? 1u : 0u;  // see the two-pass branch
@@ -712,6 +957,7 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
 
 uint32_t lgca_internal::step_passes(const lg_t* g) {
   if (g->passes) return g->passes;
+  if (g->tdepth >= 2 && g->G.wpr <= 64) return 2u;  // temporal blocking is a two-pass pipeline
   return g->G.total >= LG_AUTO_FUSED_WORDS ? 1u : 2u;
 }
 
@@ -759,6 +1005,10 @@ lg_t* lg_create(size_t w, size_t h, int rest, uint64_t seed, size_t bands) {
     return nullptr;
   };
   if (hipGetDevice(&g->device) != hipSuccess) return bail("hipGetDevice failed");
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, g->device) == hipSuccess) g->ncu = prop.multiProcessorCount;
+  }
   if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
     return bail("hipStreamCreate failed");
   LgGeom& G = g->G;
@@ -905,7 +1155,7 @@ int lg_advance_sampling(lg_t* g, uint64_t n, uint64_t every, float alpha) {
   int rc = enqueue_steps(g, n, every, alpha, false);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));
-  return LG_OK;
+  return check_band(g);
 }
 
 int lg_advance_async(lg_t* g, uint64_t n) { return lg_advance_sampling_async(g, n, 0, 0.0f); }
@@ -927,7 +1177,10 @@ int lg_poll(lg_t* g) {
     g->after = nullptr;
   }
   hipError_t e = hipStreamQuery(g->stream);
-  if (e == hipSuccess) return 1;
+  if (e == hipSuccess) {
+    if (int rc = check_band(g)) return rc;
+    return 1;
+  }
   if (e == hipErrorNotReady) return 0;
   return fail(LG_EDEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
 }
@@ -936,7 +1189,7 @@ int lg_wait(lg_t* g) {
   if (!g) return fail(LG_EINVAL, "null handle");
   if (int rc = settle(g)) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));
-  return LG_OK;
+  return check_band(g);
 }
 
 int lg_advance_sampling_timed(lg_t* g, uint64_t n, uint64_t every, float alpha, float* ms) {
@@ -949,6 +1202,7 @@ int lg_advance_sampling_timed(lg_t* g, uint64_t n, uint64_t every, float alpha, 
   int rc = enqueue_steps(g, n, every, alpha, true);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));
+  if ((rc = check_band(g))) return rc;
   float total = 0.0f;
   for (size_t i = 0; i < g->tev_used; ++i) {
     float t = 0.0f;
@@ -1059,6 +1313,42 @@ int lg_set_draw_ahead(lg_t* g, uint32_t steps) {
   g->ahead = steps;
   g->auto_ahead = false;
   return LG_OK;
+}
+
+int lg_set_temporal_depth(lg_t* g, uint32_t steps, uint32_t rows_per_tile) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  if (steps < 1 || steps > LG_MAX_TDEPTH)
+    return fail(LG_EINVAL, "temporal depth must be 1.." + std::to_string(LG_MAX_TDEPTH));
+  if (rows_per_tile > 32 - 2 * steps)
+    return fail(LG_EINVAL, "a tile holds at most 32 rows with its halo (2 x depth rows)");
+  if (steps >= 2 && g->G.wpr > 64)
+    return fail(LG_EINVAL, "temporal blocking needs rows of at most 64 words (4096 cells)");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  if (g->dstream) HIP_TRY(hipStreamSynchronize(g->dstream));
+  g->tdepth = steps;
+  g->trows = rows_per_tile;
+  if (steps >= 2 && g->ahead < steps) {  // blocks of draws at least as deep
+    g->ahead = steps;
+    g->auto_ahead = false;
+  }
+  return LG_OK;
+}
+
+int lg_set_resident(lg_t* g, uint32_t on) {
+  if (!g) return fail(LG_EINVAL, "null handle");
+  if (on > 1) return fail(LG_EINVAL, "resident must be 0 or 1");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->resident = on;
+  return LG_OK;
+}
+
+int lg_get_resident(lg_t* g, uint32_t* rows_per_band, uint32_t* bands) {
+  if (!g || !rows_per_band || !bands) return fail(LG_EINVAL, "null argument");
+  uint32_t r = 0, n = 0;
+  const bool ok = g->resident && band_eligible(g, &r, &n);
+  *rows_per_band = ok ? r : 0;
+  *bands = ok ? n : 0;
+  return ok ? 1 : 0;
 }
 
 int lg_set_draw_unit(lg_t* g, uint32_t cells_per_lane) {

@@ -71,9 +71,37 @@ uint32_t step_passes(const lg_lattice* g);
 // (4096 x 2048) 32-cell draw lanes are 17% slower and draw-ahead 10% slower.
 #define LG_SMALL_WORDS (1u << 16)
 
+// Largest temporal depth of the collide pass (steps per collide launch): a
+// tile of at most 32 rows keeps 2 x depth halo rows and at least 16 valid.
+#define LG_MAX_TDEPTH 8u
+
 // Settle the draw unit and draw-ahead that follow the size rule (those the
 // caller has not set): see LG_SMALL_WORDS.
 void resolve_pipeline(lg_lattice* g);
+
+// Resident bands: rows per band at most (one wave per row), and the polls a
+// neighbour wait makes before it gives up (each sleeps ~64 clocks: ~0.1 s).
+#define LG_BAND_MAX_ROWS 12u
+#define LG_BAND_SPIN_LIMIT (1u << 20)
+
+// Arguments of one resident-band launch (lgca_band.hip).
+struct LgBandArgs {
+  LgGeom G;
+  const uint64_t* src;
+  uint64_t* dst;
+  const uint64_t* solid;
+  const uint64_t* cjump;
+  const uint32_t* row_band;
+  const uint64_t* ijump;
+  const uint64_t* icell;  // per-cell inflow jumps (inflow widths <= 64)
+  const uint64_t* tabs;  // stream tables of the launch's steps
+  uint64_t* halo;        // [2 slots][2 kinds: top, bottom][nb][2 planes][64]
+  uint64_t* flags;       // [2 kinds][nb]: steps published, from epoch
+  uint64_t* err;         // epoch of a launch whose wait ran out (device)
+  uint64_t* err_host;    // the same, in page-locked host memory
+  uint64_t epoch;
+  uint32_t n, rows, nb;
+};
 
 // Free a handle and everything it owns (after its streams drain).
 void destroy_lattice(lg_lattice* g);
@@ -84,6 +112,15 @@ int ensure_tables(lg_lattice* g);
 
 // Allocate the two digit/inflow slots of the two-pass step if needed.
 int ensure_draw_buffers(lg_lattice* g);
+
+// Resident bands (lgca_band.hip): whether the lattice can run them (rows of
+// at most 64 words, at most LG_BAND_MAX_ROWS rows per band with one band per
+// CU) and with which band height and count; one launch of n steps from the
+// tables at tabs, planes a -> b; and, once the stream has drained, whether a
+// launch since the last check ran out of a wait.
+bool band_eligible(const lg_lattice* g, uint32_t* rows, uint32_t* nb);
+int launch_band(lg_lattice* g, uint32_t n, const uint64_t* tabs);
+int check_band(lg_lattice* g);
 
 }  // namespace lgca_internal
 
@@ -105,6 +142,7 @@ struct lg_lattice {
   bool tables_dirty = true;  // cjump/ijump/row_band/order out of date
   // planes: a (current), b (next); solid; static tables
   DevBuf<uint64_t> a, b, solid, cjump, ijump, basis;
+  DevBuf<uint64_t> icell;  // per-cell inflow jumps, inflow widths <= 64 (resident bands)
   DevBuf<uint32_t> row_band, order;
   uint32_t units = 1;  // 1: a lane steps a 64-cell word; 2: a 32-cell half
   // passes per step: 1 = one fused kernel per step; 2 = one launch per step
@@ -131,6 +169,18 @@ struct lg_lattice {
   // to `ahead` steps into two groups of slots; `stream` runs one collide
   // launch per step. ev_drawn[q] / ev_used[q]: group q drawn / collided.
   uint32_t ahead = 0;
+  // Temporal blocking of the collide pass (draw-ahead only): tdepth steps per
+  // collide launch (1: one launch per step), tiles of trows valid rows (0:
+  // chosen by lattice height and CU count). Needs rows of at most 64 words.
+  uint32_t tdepth = 1, trows = 0;
+  int ncu = 256;
+  // Resident bands (lgca_band.hip): 0 = off, 1 = on where eligible.
+  uint32_t resident = 0;
+  DevBuf<uint64_t> band_halo, band_flags, band_err_dev;
+  uint64_t* band_err_host = nullptr;  // page-locked
+  uint32_t band_nb = 0;
+  uint64_t band_epoch = 0, band_epoch_checked = 0;
+  bool band_check = false;
   hipStream_t dstream = nullptr;
   hipEvent_t ev_drawn[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
   hipEvent_t ev_start = nullptr;

@@ -225,4 +225,25 @@ inline void build_tables(uint32_t w, uint32_t h, uint32_t bands, uint32_t inlet_
       if (!(solid[(size_t)y * wpr + j] || (ragged && j == wpr - 1))) order.push_back((y << 16) | j);
 }
 
+// Per-cell inflow jumps for inflow widths of at most 64 cells (one word):
+// icell[y * cols + x] = z^n mod P, n = draws of the inflow stream before
+// cell (y, x) (4 per fluid inflow cell, 3 without rest, row-major), 0 for a
+// solid cell, which takes no draw (src/lattice.rs:270-285). The same
+// positions ijump gives per word, one cell at a time.
+inline void build_inflow_cells(uint32_t w, uint32_t h, uint32_t inlet_cols, bool rest,
+                               const std::vector<uint64_t>& solid, std::vector<uint64_t>& icell) {
+  const uint64_t p = charpoly_low();
+  const uint32_t wpr = (w + 63) / 64;
+  const uint32_t cols = inlet_cols < w ? inlet_cols : w;
+  const uint32_t per = rest ? 4 : 3;
+  icell.assign((size_t)cols * h, 0);
+  uint64_t c = 1;
+  for (uint32_t y = 0; y < h; ++y)
+    for (uint32_t x = 0; x < cols; ++x) {
+      if ((solid[(size_t)y * wpr + x / 64] >> (x % 64)) & 1u) continue;
+      icell[(size_t)y * cols + x] = c;
+      c = poly_mulz(c, per, p);
+    }
+}
+
 }  // namespace lgca

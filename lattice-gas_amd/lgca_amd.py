@@ -66,6 +66,9 @@ def load_library(path: str | None = None):
         "lg_set_passes": (i32, [vp, C.c_uint32]),
         "lg_set_draw_unit": (i32, [vp, C.c_uint32]),
         "lg_set_draw_ahead": (i32, [vp, C.c_uint32]),
+        "lg_set_temporal_depth": (i32, [vp, C.c_uint32, C.c_uint32]),
+        "lg_set_resident": (i32, [vp, C.c_uint32]),
+        "lg_get_resident": (i32, [vp, P(C.c_uint32), P(C.c_uint32)]),
         "lg_get_pipeline": (i32, [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "lg_batch_set_units": (i32, [vp, C.c_uint32]),
         "lg_batch_get_units": (i32, [vp, P(C.c_uint32)]),
@@ -271,6 +274,21 @@ class GpuLattice:
         launch on a second stream, ahead of the collides. Until set (or after
         set_passes(0)) it follows the lattice size."""
         _check(load_library().lg_set_draw_ahead(self._p, int(steps)))
+
+    def set_temporal_depth(self, steps: int, rows_per_tile: int = 0):
+        """Collide `steps` steps per launch on bands of rows held in registers
+        (lg_set_temporal_depth); 1 turns it off."""
+        _check(load_library().lg_set_temporal_depth(self._p, int(steps), int(rows_per_tile)))
+
+    def set_resident(self, on: bool):
+        """Resident bands (lg_set_resident): one persistent launch per chunk."""
+        _check(load_library().lg_set_resident(self._p, int(bool(on))))
+
+    def resident(self):
+        """(rows per band, bands) if the next advance runs resident, else None."""
+        r, n = C.c_uint32(), C.c_uint32()
+        on = _check(load_library().lg_get_resident(self._p, C.byref(r), C.byref(n)))
+        return (r.value, n.value) if on else None
 
     def advance_timed(self, n: int, sample_every: int = 0, alpha: float = 0.0) -> float:
         """Advance and return the device milliseconds of the step kernels,

@@ -149,6 +149,96 @@ void hs_advance(void* p, uint64_t n) {
   }
 }
 
+// The temporally blocked collide pass (lg_collide_k_kernel) emulated on the
+// host: steps are drawn as the draw pass draws them, then every tile of L =
+// NW x M rows (R = L - 2 ks valid) is stepped ks steps with zero rows beyond
+// its edges and its words' neighbours taken across the row (periodic), and
+// only its valid rows are stored. Groups of ks steps until n are done.
+void hs_advance_tiled(void* p, uint64_t n, uint32_t ks, uint32_t L) {
+  auto* S = static_cast<Sim*>(p);
+  LgGeom& G = S->G;
+  const size_t nhalf = (size_t)2 * G.total, nih = (size_t)2 * G.iw * G.h;
+  const size_t T = G.total;
+  uint64_t done = 0;
+  while (done < n) {
+    const uint32_t k = (uint32_t)std::min<uint64_t>(ks, n - done);
+    const uint32_t R = L - 2 * k;
+    // draws of the group's k steps
+    std::vector<uint32_t> kb((size_t)k * LG_KDIGITS * nhalf, 0xDEADBEEFu);
+    std::vector<uint32_t> ib(std::max<size_t>((size_t)k * 7 * nih, 1), 0xDEADBEEFu);
+    for (uint32_t s = 0; s < k; ++s) {
+      const uint64_t t = S->steps + s;
+      for (uint32_t q = 0; q <= G.nbands; ++q) {
+        uint64_t st = lg_rng_init(q < G.nbands ? lg_band_seed(S->seed, t, q)
+                                               : lg_inlet_seed(S->seed, t + 1));
+        lg_build_tab(st, S->basis.data() + (size_t)q * LG_WTAB);
+      }
+      for (uint32_t e : S->order)
+        lg_draw_unit<1>(G, S->solid.data(), S->cjump.data(), S->row_band.data(), S->ijump.data(),
+                        S->basis.data(), e >> 16, e & 0xFFFFu, 0,
+                        kb.data() + (size_t)s * LG_KDIGITS * nhalf, nhalf,
+                        ib.data() + (size_t)s * 7 * nih, nih);
+    }
+    const uint32_t ntiles = (G.h + R - 1) / R;
+    std::vector<uint64_t> P((size_t)L * 7 * G.wpr), Q(P.size());
+    std::vector<uint32_t> Y(L);
+    for (uint32_t t = 0; t < ntiles; ++t) {
+      for (uint32_t i = 0; i < L; ++i) {
+        Y[i] = lg_tile_row(t, i, R, k, G.h);
+        for (int d = 0; d < 7; ++d)
+          for (uint32_t j = 0; j < G.wpr; ++j)
+            P[((size_t)i * 7 + d) * G.wpr + j] = S->a[(size_t)d * T + (size_t)Y[i] * G.wpr + j];
+      }
+      auto at = [&](uint32_t i, int d, uint32_t j) -> uint64_t {  // zero beyond the tile
+        return i < L ? P[((size_t)i * 7 + d) * G.wpr + j] : 0ull;
+      };
+      for (uint32_t s = 0; s < k; ++s) {
+        const uint32_t* kbs = kb.data() + (size_t)s * LG_KDIGITS * nhalf;
+        const uint32_t* ibs = ib.data() + (size_t)s * 7 * nih;
+        for (uint32_t i = 0; i < L; ++i) {
+          const uint32_t y = Y[i], odd = y & 1u, up = i - 1, dn = i + 1;  // up wraps past L at i = 0
+          for (uint32_t j = 0; j < G.wpr; ++j) {
+            const uint32_t jl = (j == 0) ? G.lastword : j - 1, jr = (j == G.lastword) ? 0 : j + 1;
+            const uint32_t cl = (j == 0) ? G.lastbit : 63u, cr = (j == G.lastword) ? G.lastbit : 63u;
+            const uint64_t tail = (j == G.lastword) ? G.tailmask : ~0ull;
+            const size_t word = (size_t)y * G.wpr + j;
+            uint64_t nn[7];
+            lg_gather(odd, cl, cr, at(i, 0, j), at(i, 0, jl), at(up, 1, j), at(up, 1, jl), at(up, 2, j),
+                      at(up, 2, jr), at(i, 3, j), at(i, 3, jr), at(dn, 4, j), at(dn, 4, jr),
+                      at(dn, 5, j), at(dn, 5, jl), at(i, 6, j), nn);
+            uint32_t Kl[LG_KDIGITS] = {0, 0, 0, 0, 0}, Kh[LG_KDIGITS] = {0, 0, 0, 0, 0};
+            const int nk = G.rest ? LG_COLLIDE_REST_KBITS : LG_COLLIDE_NOREST_KBITS;
+            for (int b = 0; b < nk; ++b) {
+              Kl[b] = kbs[(size_t)b * nhalf + 2 * word];
+              Kh[b] = kbs[(size_t)b * nhalf + 2 * word + 1];
+            }
+            uint64_t res[7];
+            lg_collide_word(G, nn, S->solid[word], Kl, Kh, res);
+            if (j < G.iw) {
+              const size_t iw2 = 2 * ((size_t)y * G.iw + j);
+              uint64_t bits[7];
+              for (int d = 0; d < 7; ++d)
+                bits[d] = ((uint64_t)ibs[(size_t)d * nih + iw2 + 1] << 32) | ibs[(size_t)d * nih + iw2];
+              lg_inflow_apply(G, j, S->solid[word], tail, bits, res);
+            }
+            for (int d = 0; d < 7; ++d) Q[((size_t)i * 7 + d) * G.wpr + j] = res[d] & tail;
+          }
+        }
+        std::swap(P, Q);
+      }
+      for (uint32_t i = k; i < L - k; ++i) {
+        if ((uint64_t)t * R + (i - k) >= G.h) continue;
+        for (int d = 0; d < 7; ++d)
+          for (uint32_t j = 0; j < G.wpr; ++j)
+            S->b[(size_t)d * T + (size_t)Y[i] * G.wpr + j] = P[((size_t)i * 7 + d) * G.wpr + j];
+      }
+    }
+    std::swap(S->a, S->b);
+    S->steps += k;
+    done += k;
+  }
+}
+
 // Exhaustive check of the generated circuit: for every state and every k,
 // the collided state (stride 30 with rest, 6 without) into out[s * stride + k].
 void hs_collide_table(int rest, uint8_t* out) {
@@ -175,6 +265,89 @@ void hs_collide_table(int rest, uint8_t* out) {
       }
       out[s * stride + k] = bad ? 0xFF : r;
     }
+}
+
+// The branch-free masked half (lg_draw_half_sel) against the branching one
+// (lg_draw_masked32) and the plain one for full masks, on n pseudo-random
+// (state, mask) pairs: digits, and the state after the drawing cells.
+// Returns the number of mismatches.
+int hs_draw_sel_check(uint64_t seed, int n) {
+  uint64_t x = seed | 1u;
+  int bad = 0;
+  for (int i = 0; i < n; ++i) {
+    x = lg_xs(x);
+    const uint64_t st = lg_rng_init(x);
+    x = lg_xs(x);
+    uint32_t fh = (uint32_t)(x >> 17);
+    if (i % 4 == 0) fh = ~0u;
+    if (i % 4 == 1) fh &= (uint32_t)(x >> 40);  // sparser
+    uint32_t A[5], B[5], C[5];
+    const uint64_t end = lg_draw_half_sel<5>(st, fh, 30, A);
+    lg_draw_masked32<5>(st, fh, 30, B);
+    uint64_t want = st;
+    for (int c = 0; c < __builtin_popcount(fh); ++c) want = lg_xs(want);
+    for (int b = 0; b < 5; ++b) bad += A[b] != B[b];
+    bad += end != want;
+    if (fh == ~0u) {
+      const uint64_t e2 = lg_draw_half<5>(st, 30, C);
+      for (int b = 0; b < 5; ++b) bad += A[b] != C[b];
+      bad += e2 != end;
+    }
+    uint32_t A3[3], B3[3];
+    lg_draw_half_sel<3>(st, fh, 6, A3);
+    lg_draw_masked32<3>(st, fh, 6, B3);
+    for (int b = 0; b < 3; ++b) bad += A3[b] != B3[b];
+  }
+  return bad;
+}
+
+// Per-cell inflow jumps (build_inflow_cells) against the per-word ones
+// (build_tables' ijump): for a random solid layout and inflow width <= 64,
+// every fluid inflow cell sampled from its own jump gives the bits the
+// serial per-word chain gives. Returns the number of mismatching words.
+int hs_inflow_cells_check(uint32_t w, uint32_t h, uint32_t cols, int rest, uint64_t seed) {
+  const uint32_t wpr = (w + 63) / 64;
+  std::vector<uint64_t> solid((size_t)wpr * h, 0), cj, ij, ic;
+  std::vector<uint32_t> rb, ord;
+  uint64_t x = seed | 1u;
+  for (auto& v : solid) {
+    x = lg_xs(x);
+    v = x & lg_xs(x) & lg_xs(lg_xs(x));  // about one cell in eight
+  }
+  for (uint32_t y = 0; y < h; ++y)
+    if (w % 64) solid[(size_t)y * wpr + wpr - 1] &= (1ull << (w % 64)) - 1;
+  lgca::build_tables(w, h, 1, cols, rest != 0, solid, cj, rb, ij, ord);
+  lgca::build_inflow_cells(w, h, cols, rest != 0, solid, ic);
+  LgGeom G;
+  std::memset(&G, 0, sizeof(G));
+  G.w = w;
+  G.rest = rest ? 1u : 0u;
+  G.inlet_cols = cols < w ? cols : w;
+  G.iw = (G.inlet_cols + 63) / 64;
+  const uint32_t th[7] = {5000000, 9000000, 3000000, 12000000, 7000000, 1000000, 8000000};
+  for (int d = 0; d < 7; ++d) G.thresh[d] = th[d];
+  std::vector<uint64_t> tab(LG_WTAB);
+  lg_build_tab(lg_rng_init(seed ^ 0x77), tab.data());
+  int bad = 0;
+  for (uint32_t y = 0; y < h; ++y) {
+    const uint64_t sm = solid[(size_t)y * wpr];
+    const uint64_t apply = lg_inlet_apply(G, 0, ~sm);
+    uint64_t want[7];
+    lg_inlet_bits(G, lg_jump_tab(ij[2 * (size_t)y * G.iw], tab.data()), apply, want);
+    uint64_t got[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t c = 0; c < G.inlet_cols; ++c) {
+      if ((sm >> c) & 1u) continue;
+      uint64_t st = lg_jump_tab(ic[(size_t)y * G.inlet_cols + c], tab.data());
+      const uint32_t bits = lg_eq_cell(G, st);
+      for (int d = 0; d < 7; ++d) got[d] |= (uint64_t)((bits >> d) & 1u) << c;
+    }
+    for (int d = 0; d < 7; ++d)
+      if (got[d] != want[d]) {
+        ++bad;
+        break;
+      }
+  }
+  return bad;
 }
 
 uint64_t hs_charpoly(void) { return lgca::charpoly_low(); }

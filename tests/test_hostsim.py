@@ -172,3 +172,50 @@ def test_ragged_shapes_every_step(oracle, hs, w, h, rest, bands, inlet, units, p
     yc = float(np.float32(np.float32(h) * oracle.SQRT3_2 / np.float32(2)))
     compare(oracle, hs, w, h, rest, bands, inlet, 0xC0FFEE + w, 25, disc=(w / 3.0, yc, 3.5),
             units=units, passes=passes)
+
+
+@pytest.mark.parametrize("w,h,rest,bands,inlet", [
+    (128, 64, True, 4, 8), (65, 6, True, 1, 8), (100, 18, True, 3, 8), (129, 40, False, 4, 70),
+    (200, 40, True, 7, 130), (64, 32, True, 5, 0), (127, 8, False, 3, 127),
+])
+@pytest.mark.parametrize("ks,L", [(2, 6), (2, 14), (4, 18), (8, 32), (3, 7), (8, 17)])
+def test_temporal_blocking_every_group(oracle, hs, w, h, rest, bands, inlet, ks, L):
+    """The temporally blocked collide (tiles of L rows stepped ks steps with
+    2 ks halo rows, only the middle rows kept) gives the oracle's cells after
+    every group of ks steps, including tiles taller than the lattice and a
+    last tile that wraps past the last row."""
+    hs.hs_advance_tiled.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32]
+    density, inflow, seed = 0.22, (0.3, 0.0), 0xB10C + w
+    lat = oracle.Lattice(w, h, density, inflow, inlet, rest, bands, seed)
+    yc = float(np.float32(np.float32(h) * oracle.SQRT3_2 / np.float32(2)))
+    lat.add_solid_disc(w / 3.0, yc, 3.5)
+    lat.init_equilibrium()
+    sim = HostSim(hs, w, h, rest, seed, bands, inlet, oracle.thresholds(density, *inflow, rest))
+    sim.load(lat.cells)
+    for g in range(3):
+        lat.advance(ks)
+        hs.hs_advance_tiled(sim.p, ks, ks, L)
+        got, ref = sim.store(), lat.cells
+        if not np.array_equal(got, ref):
+            bad = np.nonzero(got != ref)[0]
+            pytest.fail(f"group {g + 1}: {bad.size} cells differ, first at "
+                        f"{[(int(b) % w, int(b) // w) for b in bad[:4]]}")
+    # a ragged final group (n not a multiple of ks)
+    lat.advance(ks + 1)
+    hs.hs_advance_tiled(sim.p, ks + 1, ks, L)
+    assert np.array_equal(sim.store(), lat.cells)
+
+
+def test_branch_free_masked_draw(hs):
+    """The resident bands' branch-free masked half-word draw equals the
+    branching one (digits and stream state) for random and full masks."""
+    assert hs.hs_draw_sel_check(0x5E1, 20000) == 0
+
+
+@pytest.mark.parametrize("w,h,cols,rest", [(4096, 64, 8, 1), (100, 40, 64, 1), (65, 30, 65, 0),
+                                            (300, 50, 37, 1), (64, 10, 1, 0)])
+def test_per_cell_inflow_jumps(hs, w, h, cols, rest):
+    """Per-cell inflow jumps give every fluid inflow cell the same sample as
+    the serial per-word inflow chain, with solid cells taking no draw."""
+    hs.hs_inflow_cells_check.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_uint64]
+    assert hs.hs_inflow_cells_check(w, h, min(cols, 64), rest, 0xF10 + w) == 0
