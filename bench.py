@@ -573,6 +573,8 @@ def main():
                          "(lg_set_temporal_depth; 0: the engine's default)")
     ap.add_argument("--tile-rows", type=int, default=0, metavar="R",
                     help="valid rows per band of the temporally blocked collide (0: by size)")
+    ap.add_argument("--resident", type=int, default=2, choices=(0, 1, 2),
+                    help="resident bands: 0 off, 1 on, 2 by the engine's size rule")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--ensemble", type=int, default=256, metavar="N",
@@ -637,6 +639,7 @@ def main():
         g.set_draw_ahead(args.draw_ahead)
     if args.temporal_depth:
         g.set_temporal_depth(args.temporal_depth, args.tile_rows)
+    g.set_resident(args.resident)
     g.attach_field(cfg["block"], cfg["block"])
     if args.warmup:
         g.advance_sampling(args.warmup, every, 1.0)
@@ -651,14 +654,22 @@ def main():
     traffic_alg = cells * BYTES_PER_CELL_UPDATE
     achieved = traffic_alg / (step_ms * 1e-3) / 1e9
     passes, cpl, dcpl, ahead = g.pipeline()  # what the timed steps ran
+    bands_res = g.resident()
     unit = "h" if cpl == 32 else "w"
     nh, dnh = 3 - cpl // 32, 3 - dcpl // 32
-    if passes == 1:
+    steps_per_launch = 1
+    if bands_res:
+        # one cooperative launch per chunk: a chunk ends at a sample or at
+        # 256 steps (the stream-table buffer)
+        kernel = f"lg_band_kernel<{'true' if cfg['w'] > 4032 else 'false'}>"
+        steps_per_launch = min(args.steps, every, 256)
+    elif passes == 1:
         kernel = f"lg_step_kernel_{unit}"
     elif ahead:
         kernel = f"lg_collide_kernel<{nh}>+lg_draw_kernel<{dnh}>"
     else:
         kernel = f"lg_step2_kernel<{nh},{dnh}>"
+    launches = -(-args.steps // steps_per_launch)
 
     out = {
         "metric": METRIC,
@@ -677,13 +688,19 @@ def main():
                    "rest": True, "bands": bands, "inlet_cols": cfg["inlet"],
                    "sample_every": every, "block": cfg["block"], "seed": cfg["seed"],
                    "cells_per_lane": cpl, "passes": passes,
-                   "draw_cells_per_lane": dcpl if passes == 2 else None,
-                   "draw_ahead": ahead if passes == 2 else None},
+                   "draw_cells_per_lane": dcpl if passes == 2 and not bands_res else None,
+                   "draw_ahead": ahead if passes == 2 and not bands_res else None,
+                   "resident_bands": ({"rows_per_band": bands_res[0], "bands": bands_res[1],
+                                       "steps_per_launch": steps_per_launch}
+                                      if bands_res else None)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(kernel, cfg["name"]),
-                     "kernel": kernel, "algorithmic_bytes_per_launch": int(traffic_alg),
-                     "mean_launch_ms": round(step_ms, 6)},
+                     "kernel": kernel,
+                     "algorithmic_bytes_per_launch": int(traffic_alg * steps_per_launch),
+                     "steps_per_launch": steps_per_launch,
+                     "mean_launch_ms": round(dev_ms / launches, 6),
+                     "mean_step_ms": round(step_ms, 6)},
     }
     if rank == 0 and not args.no_cpu:
         total = args.warmup + args.steps

@@ -177,14 +177,18 @@ int lg_set_draw_unit(lg_t* g, uint32_t cells_per_lane);
  * Results are identical for every choice. */
 int lg_set_temporal_depth(lg_t* g, uint32_t steps, uint32_t rows_per_tile);
 
-/* New. Resident bands (on = 1): each call's steps run in one cooperative
+/* New. Resident bands (on = 1; 2, the default, = on unless lg_set_passes,
+ * lg_set_draw_ahead or lg_set_temporal_depth chose another pipeline; 0 =
+ * off): each call's steps run in one cooperative
  * launch per chunk (a chunk ends at a sample or at 256 steps), in which one
  * workgroup per band of rows keeps its rows in registers for the whole
  * chunk, draws its own random digits one step ahead and exchanges only its
  * edge rows with the neighbouring bands through global memory and per-band
- * step counters. For rows of at most 64 words (4096 cells) and at most 12
- * rows per band at one band per CU; other lattices keep the other
- * pipelines. Results are identical. lg_get_resident returns 1 (and the band
+ * step counters, and a further wave builds each step's stream tables in LDS.
+ * For rows of at most 64 words (4096 cells) and at most 11 rows per band at
+ * one band per CU; other lattices keep the other pipelines. The size rule
+ * (2) takes it for rows of at least 48 words and calls whose first chunk
+ * has at least 4 steps. Results are identical. lg_get_resident returns 1 (and the band
  * height and count) when the next advance runs resident, else 0. A wait
  * between bands that runs out (it cannot hang the device: every wait is
  * bounded) makes the call fail with LG_EDEVICE and the lattice must be

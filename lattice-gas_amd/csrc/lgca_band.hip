@@ -28,6 +28,7 @@
 #include <string>
 
 #include "lgca_internal.h"
+#include "lgca_jump.hpp"
 
 namespace {
 
@@ -129,18 +130,63 @@ __device__ __forceinline__ void lg_band_digits(uint64_t fm, uint64_t cj0,
 
 }  // namespace
 
-// One band per workgroup (blockIdx.x), one row per wave, one word per lane;
-// A.n steps from A.src into A.dst. LDS: two exchange slots of the waves'
-// planes 1, 2, 4, 5 (blockDim / 64 x 4 KB).
+// The band row wave w holds. Waves go to the CU's four SIMDs in turn (wave
+// w to SIMD w % 4), so a band of B rows puts one wave more on SIMDs 0 ..
+// B % 4 - 1. The band's first and last rows wait for and publish to the
+// neighbouring bands; they go to the first waves on the other SIMDs, where
+// fewer waves share the issue slots around those waits, and the rows in
+// between fill the remaining waves in order.
+LG_HD inline uint32_t lg_band_row_of_wave(uint32_t w, uint32_t B) {
+  const uint32_t heavy = B % 4;
+  if (heavy == 0 || B < 6) return w;
+  const uint32_t wt = heavy, wb = heavy + 1 < 4 ? heavy + 1 : heavy + 4;  // light waves
+  if (w == wt) return 0;
+  if (w == wb) return B - 1;
+  // the other waves in order take rows 1 .. B - 2
+  return 1 + w - (w > wt ? 1u : 0u) - (w > wb ? 1u : 0u);
+}
+
+// The stream tables one band needs for step t into LDS (out: nstreams x
+// LG_WTAB entries): its row bands' collision streams rb0, rb0 + 1, ... and,
+// last, the inflow stream of the step's end (src/lattice.rs:233,242,276);
+// the same 4-bit window tables lg_tables_kernel builds. One lane per
+// (stream, window group): it walks to the group's first state and writes its
+// 16 entries; the lanes of one wave take the tasks in turn.
+__device__ __forceinline__ void lg_band_tables(const lgca_internal::LgBandArgs& A, uint32_t rb0,
+                                               uint32_t nsb, uint64_t t, uint32_t lane,
+                                               uint64_t* __restrict out) {
+  const uint32_t tasks = (nsb + 1) * 16;
+  for (uint32_t q = lane; q < tasks; q += 64) {
+    const uint32_t sidx = q >> 4, grp = q & 15u;
+    uint64_t st = lg_rng_init(sidx < nsb ? lg_band_seed(A.seed, t, rb0 + sidx)
+                                         : lg_inlet_seed(A.seed, t + 1));
+    for (uint32_t i = 0; i < 4 * grp; ++i) st = lg_xs(st);
+    const uint64_t b0 = st, b1 = lg_xs(b0), b2 = lg_xs(b1), b3 = lg_xs(b2);
+    uint64_t* o = out + (size_t)sidx * LG_WTAB + grp * 16;
+    LG_UNROLL
+    for (uint32_t v = 0; v < 16; ++v)
+      o[v] = ((v & 1u) ? b0 : 0ull) ^ ((v & 2u) ? b1 : 0ull) ^ ((v & 4u) ? b2 : 0ull) ^
+             ((v & 8u) ? b3 : 0ull);
+  }
+}
+
+// One band per workgroup (blockIdx.x), one row per wave, one word per lane,
+// plus one wave that builds the next step's stream tables; A.n steps from
+// A.src into A.dst. LDS: two exchange slots of the rows' planes 1, 2, 4, 5
+// (rows x 4 KB) and two slots of stream tables (A.nstreams x 4 KB).
 template <bool WAVEROW>
-__global__ void __launch_bounds__(64 * LG_BAND_MAX_ROWS) lg_band_kernel(lgca_internal::LgBandArgs A) {
-  extern __shared__ uint64_t xch[];  // [2 slots][NW waves][4 planes][64 lanes]
+__global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
+    lg_band_kernel(lgca_internal::LgBandArgs A) {
+  extern __shared__ uint64_t xch[];  // [2 slots][rows][4 planes][64 lanes]
   const LgGeom& G = A.G;
-  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, NW = blockDim.x >> 6;
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, NW = A.rows;
+  uint64_t* const tabs_lds = xch + (size_t)2 * NW * 256;  // [2 slots][nstreams][LG_WTAB]
+  const bool table_wave = w == NW;
   const uint32_t b = blockIdx.x, y0 = b * A.rows;
   const uint32_t Bb = min(A.rows, G.h - y0);  // rows of this band
-  const bool row_live = w < Bb;
-  const uint32_t y = y0 + (row_live ? w : 0u);
+  const uint32_t r = table_wave ? NW : lg_band_row_of_wave(w, A.rows);  // the band row it holds
+  const bool row_live = r < Bb;
+  const uint32_t y = y0 + (row_live ? r : 0u);
   const bool live = lane < G.wpr;
   const uint32_t j = live ? lane : 0u;
   const uint32_t cl = (j == 0) ? G.lastbit : 63u, cr = (j == G.lastword) ? G.lastbit : 63u;
@@ -148,9 +194,10 @@ __global__ void __launch_bounds__(64 * LG_BAND_MAX_ROWS) lg_band_kernel(lgca_int
   const int addr_l = (int)(4u * (live ? (j == 0 ? G.wpr - 1 : j - 1) : lane));
   const int addr_r = (int)(4u * (live ? (j + 1 == G.wpr ? 0u : j + 1) : lane));
   const uint32_t bu = (b + A.nb - 1) % A.nb, bd = (b + 1) % A.nb;
-  const bool top = w == 0, bot = w + 1 == Bb;
+  const bool top = r == 0, bot = r + 1 == Bb;
   const size_t T = G.total, word = (size_t)y * G.wpr + j;
-  const uint32_t per = G.nbands + 1;
+  // the row bands (collision streams) of this band's rows: rb0 .. rb0 + nsb - 1
+  const uint32_t rb0 = A.row_band[y0], nsb = A.row_band[y0 + Bb - 1] - rb0 + 1;
   const size_t hstride = (size_t)A.nb * 128;  // one halo kind of one slot: nb x 2 planes x 64
 
   uint64_t P[7];
@@ -173,8 +220,9 @@ __global__ void __launch_bounds__(64 * LG_BAND_MAX_ROWS) lg_band_kernel(lgca_int
   for (int q = 0; q < LG_KDIGITS; ++q) K[q] = 0;
   LG_UNROLL
   for (int d = 0; d < 7; ++d) I[d] = 0;
+  if (table_wave) lg_band_tables(A, rb0, nsb, A.t0, lane, tabs_lds);
   if (row_live) {
-    uint64_t* s0 = xch + (size_t)w * 256;
+    uint64_t* s0 = xch + (size_t)r * 256;
     s0[0 * 64 + lane] = P[1];
     s0[1 * 64 + lane] = P[2];
     s0[2 * 64 + lane] = P[4];
@@ -191,8 +239,8 @@ __global__ void __launch_bounds__(64 * LG_BAND_MAX_ROWS) lg_band_kernel(lgca_int
       uint64_t a1, a2, a4, a5;
       // planes 1, 2 of the row above
       if (!top) {
-        a1 = slot[((w - 1) * 4 + 0) * 64 + lane];
-        a2 = slot[((w - 1) * 4 + 1) * 64 + lane];
+        a1 = slot[((r - 1) * 4 + 0) * 64 + lane];
+        a2 = slot[((r - 1) * 4 + 1) * 64 + lane];
       } else if (s == 0) {
         const size_t up = (size_t)((y0 + G.h - 1) % G.h) * G.wpr + j;
         a1 = A.src[1 * T + up];
@@ -205,8 +253,8 @@ __global__ void __launch_bounds__(64 * LG_BAND_MAX_ROWS) lg_band_kernel(lgca_int
       }
       // planes 4, 5 of the row below
       if (!bot) {
-        a4 = slot[((w + 1) * 4 + 2) * 64 + lane];
-        a5 = slot[((w + 1) * 4 + 3) * 64 + lane];
+        a4 = slot[((r + 1) * 4 + 2) * 64 + lane];
+        a5 = slot[((r + 1) * 4 + 3) * 64 + lane];
       } else if (s == 0) {
         const size_t dn = (size_t)((y0 + Bb) % G.h) * G.wpr + j;
         a4 = A.src[4 * T + dn];
@@ -245,7 +293,7 @@ __global__ void __launch_bounds__(64 * LG_BAND_MAX_ROWS) lg_band_kernel(lgca_int
       // publish the new rows: to the neighbouring waves, and the band's edge
       // rows to the neighbouring bands
       const uint32_t ns = (s + 1) & 1u;
-      uint64_t* out = xch + ((size_t)ns * NW + w) * 256;
+      uint64_t* out = xch + ((size_t)ns * NW + r) * 256;
       out[0 * 64 + lane] = P[1];
       out[1 * 64 + lane] = P[2];
       out[2 * 64 + lane] = P[4];
@@ -263,10 +311,14 @@ __global__ void __launch_bounds__(64 * LG_BAND_MAX_ROWS) lg_band_kernel(lgca_int
         lg_band_publish(&A.flags[A.nb + b], A.epoch + s + 1);
       }
     }
+    // the next step's tables, beside the rows' work
+    if (table_wave && i + 1 < A.n)
+      lg_band_tables(A, rb0, nsb, A.t0 + i + 1, lane,
+                     tabs_lds + (size_t)((i + 1) & 1u) * A.nstreams * LG_WTAB);
     // step i's draws
     if (row_live && i < A.n) {
-      const uint64_t* tab = A.tabs + (size_t)i * per * LG_WTAB;
-      const uint64_t* itab = tab + (size_t)G.nbands * LG_WTAB;
+      const uint64_t* tab = tabs_lds + (size_t)(i & 1u) * A.nstreams * LG_WTAB;
+      const uint64_t* itab = tab + (size_t)nsb * LG_WTAB;
       uint64_t c0 = cj0, ci = ic, cw = ij;
 #if defined(__HIP_DEVICE_COMPILE__)
       // the jumps' window offsets are the same every step; left to itself
@@ -275,10 +327,10 @@ __global__ void __launch_bounds__(64 * LG_BAND_MAX_ROWS) lg_band_kernel(lgca_int
 #endif
       if (live) {
         if (G.rest)
-          lg_band_digits<LG_COLLIDE_REST_KBITS>(fm, c0, tab + (size_t)stream * LG_WTAB,
+          lg_band_digits<LG_COLLIDE_REST_KBITS>(fm, c0, tab + (size_t)(stream - rb0) * LG_WTAB,
                                                 LG_COLLIDE_REST_STRIDE, K);
         else
-          lg_band_digits<LG_COLLIDE_NOREST_KBITS>(fm, c0, tab + (size_t)stream * LG_WTAB,
+          lg_band_digits<LG_COLLIDE_NOREST_KBITS>(fm, c0, tab + (size_t)(stream - rb0) * LG_WTAB,
                                                   LG_COLLIDE_NOREST_STRIDE, K);
       }
       if (lane_inflow) {
@@ -337,7 +389,7 @@ int ensure_band_buffers(lg_t* g, uint32_t nb) {
 
 // Launch n steps of the resident-band kernel from the handle's tables
 // (tabs: the chunk's first step) on its stream, planes a -> b.
-int launch_band(lg_t* g, uint32_t n, const uint64_t* tabs) {
+int launch_band(lg_t* g, uint32_t n) {
   uint32_t rows = 0, nb = 0;
   if (!band_eligible(g, &rows, &nb)) return fail(LG_EINVAL, "lattice not eligible for resident bands");
   if (int rc = ensure_band_buffers(g, nb)) return rc;
@@ -350,7 +402,6 @@ int launch_band(lg_t* g, uint32_t n, const uint64_t* tabs) {
   A.row_band = g->row_band.p;
   A.ijump = g->ijump.p;
   A.icell = g->icell.p;
-  A.tabs = tabs;
   A.halo = g->band_halo.p;
   A.flags = g->band_flags.p;
   A.err = g->band_err_dev.p;
@@ -360,9 +411,22 @@ int launch_band(lg_t* g, uint32_t n, const uint64_t* tabs) {
   A.n = n;
   A.rows = rows;
   A.nb = nb;
+  A.seed = g->seed;
+  A.t0 = g->steps;
+  // stream tables per band: its row bands (at most `rows` of them) and the
+  // inflow stream
+  const uint32_t cpu_band = lgca::band_rows(g->G.h, g->bands);
+  A.nstreams = std::min<uint32_t>(rows, (rows + cpu_band - 2) / cpu_band + 1) + 1;
   void* args[] = {&A};
-  const dim3 grid(nb), block(64 * rows);
-  const size_t lds = (size_t)rows * 4096;
+  const dim3 grid(nb), block(64 * (rows + 1));
+  const size_t lds = (size_t)(rows + A.nstreams) * 4096;
+  if (lds > 64 * 1024 && !g->band_lds_raised) {
+    HIP_TRY(hipFuncSetAttribute((const void*)lg_band_kernel<true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    HIP_TRY(hipFuncSetAttribute((const void*)lg_band_kernel<false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    g->band_lds_raised = true;
+  }
   hipError_t e;
   if (g->G.wpr == 64)
     e = hipLaunchCooperativeKernel((const void*)lg_band_kernel<true>, grid, block, args, lds, g->stream);

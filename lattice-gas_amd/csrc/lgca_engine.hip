@@ -839,14 +839,16 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
     HIP_TRY(hipStreamWaitEvent(g->dstream, g->ev_start, 0));
   }
   uint32_t brows = 0, bnb = 0;
-  if (g->resident && band_eligible(g, &brows, &bnb)) {
-    // resident bands: per chunk, the chunk's stream tables, then one launch
+  // a call whose first chunk is shorter than a few steps pays the band
+  // launch's start more than it gains (one step per call at 2048 x 1280:
+  // 50 us resident against 25): it takes the other pipelines unless the
+  // caller asked for resident bands
+  const uint64_t first = std::min<uint64_t>(n, every > 0 ? every : n);
+  if (resident_active(g, &brows, &bnb) && (g->resident == 1 || first >= LG_BAND_MIN_STEPS)) {
+    // resident bands: one launch per chunk (it makes its own stream tables)
     while (left > 0) {
-      uint64_t chunk = std::min<uint64_t>(left, g->basis_steps);
+      uint64_t chunk = std::min<uint64_t>(left, LG_BAND_MAX_CHUNK);
       if (every > 0) chunk = std::min<uint64_t>(chunk, every - since);
-      hipLaunchKernelGGL(lg_tables_kernel, dim3(blocks_for((size_t)chunk * per * 16)), dim3(256), 0,
-                         g->stream, g->seed, g->steps, (uint32_t)chunk, g->G.nbands, g->basis.p);
-      HIP_TRY(hipGetLastError());
       if (timed) {
         while (g->tev.size() < 2 * (g->tev_used + 1)) {
           hipEvent_t e;
@@ -855,7 +857,7 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
         }
         HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used], g->stream));
       }
-      if (int rc = launch_band(g, (uint32_t)chunk, g->basis.p)) return rc;
+      if (int rc = launch_band(g, (uint32_t)chunk)) return rc;
       if (timed) HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used++ + 1], g->stream));
       std::swap(g->a.p, g->b.p);
       g->steps += chunk;
@@ -954,6 +956,17 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
 }
 
 }  // namespace
+
+bool lgca_internal::resident_active(const lg_t* g, uint32_t* rows, uint32_t* nb) {
+  if (g->resident == 0) return false;
+  // by the size rule: unless the caller chose another pipeline, and only
+  // for rows that fill most of a wave (the reference's 2048 x 1280 lattice,
+  // 32 words per row, runs 0.86 ms per 100 steps resident against 0.67)
+  if (g->resident == 2 &&
+      (g->passes != 0 || g->tdepth >= 2 || !g->auto_ahead || g->G.wpr < LG_BAND_MIN_WORDS))
+    return false;
+  return band_eligible(g, rows, nb);
+}
 
 uint32_t lgca_internal::step_passes(const lg_t* g) {
   if (g->passes) return g->passes;
@@ -1336,7 +1349,7 @@ int lg_set_temporal_depth(lg_t* g, uint32_t steps, uint32_t rows_per_tile) {
 
 int lg_set_resident(lg_t* g, uint32_t on) {
   if (!g) return fail(LG_EINVAL, "null handle");
-  if (on > 1) return fail(LG_EINVAL, "resident must be 0 or 1");
+  if (on > 2) return fail(LG_EINVAL, "resident must be 0 (off), 1 (on) or 2 (by the size rule)");
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->resident = on;
   return LG_OK;
@@ -1345,7 +1358,7 @@ int lg_set_resident(lg_t* g, uint32_t on) {
 int lg_get_resident(lg_t* g, uint32_t* rows_per_band, uint32_t* bands) {
   if (!g || !rows_per_band || !bands) return fail(LG_EINVAL, "null argument");
   uint32_t r = 0, n = 0;
-  const bool ok = g->resident && band_eligible(g, &r, &n);
+  const bool ok = resident_active(g, &r, &n);
   *rows_per_band = ok ? r : 0;
   *bands = ok ? n : 0;
   return ok ? 1 : 0;

@@ -81,8 +81,14 @@ void resolve_pipeline(lg_lattice* g);
 
 // Resident bands: rows per band at most (one wave per row), and the polls a
 // neighbour wait makes before it gives up (each sleeps ~64 clocks: ~0.1 s).
-#define LG_BAND_MAX_ROWS 12u
+#define LG_BAND_MAX_ROWS 11u
+// The size rule's resident bands: rows of at least this many words, calls
+// whose first chunk has at least this many steps.
+#define LG_BAND_MIN_WORDS 48u
+#define LG_BAND_MIN_STEPS 4u
 #define LG_BAND_SPIN_LIMIT (1u << 20)
+// Steps per resident launch at most (its step counters run below 2^21).
+#define LG_BAND_MAX_CHUNK (1u << 16)
 
 // Arguments of one resident-band launch (lgca_band.hip).
 struct LgBandArgs {
@@ -94,13 +100,14 @@ struct LgBandArgs {
   const uint32_t* row_band;
   const uint64_t* ijump;
   const uint64_t* icell;  // per-cell inflow jumps (inflow widths <= 64)
-  const uint64_t* tabs;  // stream tables of the launch's steps
   uint64_t* halo;        // [2 slots][2 kinds: top, bottom][nb][2 planes][64]
   uint64_t* flags;       // [2 kinds][nb]: steps published, from epoch
   uint64_t* err;         // epoch of a launch whose wait ran out (device)
   uint64_t* err_host;    // the same, in page-locked host memory
   uint64_t epoch;
+  uint64_t seed, t0;  // the run's seed, the launch's first step
   uint32_t n, rows, nb;
+  uint32_t nstreams;  // stream tables per band and step in LDS (band streams + inflow)
 };
 
 // Free a handle and everything it owns (after its streams drain).
@@ -119,7 +126,8 @@ int ensure_draw_buffers(lg_lattice* g);
 // tables at tabs, planes a -> b; and, once the stream has drained, whether a
 // launch since the last check ran out of a wait.
 bool band_eligible(const lg_lattice* g, uint32_t* rows, uint32_t* nb);
-int launch_band(lg_lattice* g, uint32_t n, const uint64_t* tabs);
+bool resident_active(const lg_lattice* g, uint32_t* rows, uint32_t* nb);
+int launch_band(lg_lattice* g, uint32_t n);
 int check_band(lg_lattice* g);
 
 }  // namespace lgca_internal
@@ -174,13 +182,15 @@ struct lg_lattice {
   // chosen by lattice height and CU count). Needs rows of at most 64 words.
   uint32_t tdepth = 1, trows = 0;
   int ncu = 256;
-  // Resident bands (lgca_band.hip): 0 = off, 1 = on where eligible.
-  uint32_t resident = 0;
+  // Resident bands (lgca_band.hip): 0 = off, 1 = on where eligible, 2 (the
+  // default) = where eligible unless another pipeline was chosen.
+  uint32_t resident = 2;
   DevBuf<uint64_t> band_halo, band_flags, band_err_dev;
   uint64_t* band_err_host = nullptr;  // page-locked
   uint32_t band_nb = 0;
   uint64_t band_epoch = 0, band_epoch_checked = 0;
   bool band_check = false;
+  bool band_lds_raised = false;
   hipStream_t dstream = nullptr;
   hipEvent_t ev_drawn[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
   hipEvent_t ev_start = nullptr;

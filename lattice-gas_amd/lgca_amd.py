@@ -280,9 +280,10 @@ class GpuLattice:
         (lg_set_temporal_depth); 1 turns it off."""
         _check(load_library().lg_set_temporal_depth(self._p, int(steps), int(rows_per_tile)))
 
-    def set_resident(self, on: bool):
-        """Resident bands (lg_set_resident): one persistent launch per chunk."""
-        _check(load_library().lg_set_resident(self._p, int(bool(on))))
+    def set_resident(self, on):
+        """Resident bands (lg_set_resident): one persistent launch per chunk.
+        True/1 on, False/0 off, 2 by the size rule (the default)."""
+        _check(load_library().lg_set_resident(self._p, int(on)))
 
     def resident(self):
         """(rows per band, bands) if the next advance runs resident, else None."""
