@@ -135,27 +135,13 @@ def ensemble_leg(n, steps, bands, seed=4):
     256..2048 in steps of 64, per-lattice seed, plate 10-25% of H tall at x in
     [W/8, W/3], inflow drift in [0.1, 0.3], `steps` steps each; one tile-table
     step kernel per step over all of them, replayed from a hipGraph with an
-    autotuned shape, against the plain loop that runs them one at a time."""
-    import numpy as np
-
-    from lgca_amd import Batch, GpuLattice
+    autotuned shape, against the plain loop that runs every one of them, one
+    at a time, for the same steps."""
+    from lgca_amd import Batch
     import workloads as W
 
-    rng = np.random.default_rng(seed)
-    specs = []
-    for i in range(n):
-        w = int(rng.integers(4, 33)) * 64
-        h = int(rng.integers(4, 33)) * 64
-        specs.append(dict(w=w, h=h, seed=int(rng.integers(1, 2**62)),
-                          rows=max(2, int(round(rng.uniform(0.10, 0.25) * h))),
-                          x=float(rng.uniform(w / 8.0, w / 3.0)), drift=float(rng.uniform(0.1, 0.3))))
-
-    def build(sp):
-        g = GpuLattice(sp["w"], sp["h"], True, sp["seed"], bands)
-        g.set_inlet(8, 0.25, (sp["drift"], 0.0))
-        g.add_solid_box(*W.plate_box(sp["w"], sp["h"], sp["x"], sp["rows"]))
-        g.init_equilibrium(0.25, 0.0, 0.0)
-        return g
+    specs = W.config4(n, bands=bands, seed=seed, steps=steps)
+    build = W.config_gpu
 
     cells = sum(sp["w"] * sp["h"] for sp in specs)
     lats = [build(sp) for sp in specs]
@@ -170,8 +156,8 @@ def ensemble_leg(n, steps, bands, seed=4):
     # the plain loop over the same lattices (rebuilt), one at a time
     t_plain = 0.0
     plain_cells = 0
-    probe = max(1, min(n, 32))  # time a fixed subset, scale to the whole set
-    for sp in specs[:probe]:
+    probe = n
+    for sp in specs:
         g = build(sp)
         t = time.perf_counter()
         g.advance(steps)
@@ -186,7 +172,7 @@ def ensemble_leg(n, steps, bands, seed=4):
             "autotuned_shape": {"tile": shape[0], "grid": shape[1], "chunk": shape[2],
                                 "passes": passes, "cells_per_lane": units,
                                 "draw_cells_per_lane": dunits},
-            "plain_loop_sample": f"first {probe} lattices, {steps} steps each"}
+            "plain_loop_sample": f"all {probe} lattices, {steps} steps each"}
 
 
 class _Skip(Exception):

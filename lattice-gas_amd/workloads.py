@@ -109,6 +109,27 @@ def config3(bands: int = 1, w: int = 32768, h: int = 16384) -> dict:
                 profile="shear")
 
 
+def config4(n: int = 256, bands: int = 16, seed: int = 4, steps: int = 3000) -> list:
+    """Batched ensemble: n independent lattices, widths and heights uniform on
+    256..2048 in steps of 64, a seed per lattice, rest particles at density
+    0.25, a plate 10-25% of H tall at x uniform in [W/8, W/3], 8 inflow
+    columns at a drift uniform in [0.1, 0.3]; `steps` steps each. One config
+    per lattice (config_gpu builds it)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        w = int(rng.integers(4, 33)) * 64
+        h = int(rng.integers(4, 33)) * 64
+        lseed = int(rng.integers(1, 2**62))
+        rows = max(2, int(round(rng.uniform(0.10, 0.25) * h)))
+        x = float(rng.uniform(w / 8.0, w / 3.0))
+        drift = float(rng.uniform(0.1, 0.3))
+        out.append(dict(name=f"fhp-ensemble-{i}-{w}x{h}", w=w, h=h, rest=True, density=0.25,
+                        init_u=(0.0, 0.0), inflow=(drift, 0.0), inlet=8, seed=lseed, bands=bands,
+                        plate=plate_box(w, h, x, rows), steps=steps))
+    return out
+
+
 def config_gpu(cfg: dict) -> GpuLattice:
     g = GpuLattice(cfg["w"], cfg["h"], cfg["rest"], cfg["seed"], cfg["bands"])
     g.set_inlet(cfg["inlet"], cfg["density"], cfg["inflow"])

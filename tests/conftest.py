@@ -36,6 +36,27 @@ def build_hostsim(force=False):
     return HOSTSIM_LIB
 
 
+HARNESS_SRC = os.path.join(ROOT, "tests", "harness", "sim_harness.cpp")
+HARNESS_BIN = os.path.join(ROOT, "tests", "harness", "build", "sim_harness")
+
+
+def build_harness():
+    """The C++ consumer of include/lgca.h (host compiler, linked against the
+    built liblgca_hip.so); rebuilt when older than its source, the header or
+    the library."""
+    lib = os.path.join(ROOT, "lattice-gas_amd", "liblgca_hip.so")
+    hdr = os.path.join(ROOT, "include", "lgca.h")
+    if not os.path.exists(lib):
+        return None
+    if not os.path.exists(HARNESS_BIN) or any(
+            os.path.getmtime(d) > os.path.getmtime(HARNESS_BIN) for d in (HARNESS_SRC, hdr, lib)):
+        os.makedirs(os.path.dirname(HARNESS_BIN), exist_ok=True)
+        subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-o", HARNESS_BIN,
+                        HARNESS_SRC, "-L", os.path.dirname(lib), "-llgca_hip",
+                        "-Wl,-rpath,$ORIGIN/../../../lattice-gas_amd"], check=True)
+    return HARNESS_BIN
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP engine)")
     config.addinivalue_line("markers", "slow: longer CPU replays")
