@@ -645,10 +645,12 @@ def main():
     nh, dnh = 3 - cpl // 32, 3 - dcpl // 32
     steps_per_launch = 1
     if bands_res:
-        # one cooperative launch per chunk: a chunk ends at a sample or at
-        # 256 steps (the stream-table buffer)
-        kernel = f"lg_band_kernel<{'true' if cfg['w'] > 4032 else 'false'}>"
-        steps_per_launch = min(args.steps, every, 256)
+        # one cooperative launch per chunk: a chunk ends at a sample (which
+        # the launch itself does: the kernel's sampling instance) or at 65536
+        # steps
+        kernel = (f"lg_band_kernel<{'true' if cfg['w'] > 4032 else 'false'},"
+                  f"{'true' if every else 'false'}>")
+        steps_per_launch = min(args.steps, every or args.steps, 65536)
     elif passes == 1:
         kernel = f"lg_step_kernel_{unit}"
     elif ahead:

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <string>
 
 #include "lgca_internal.h"
@@ -170,11 +171,100 @@ __device__ __forceinline__ void lg_band_tables(const lgca_internal::LgBandArgs& 
   }
 }
 
+// The coarse-graining of the launch's final state (src/render.rs:93-115, the
+// sums of lg_block_sums and the f32 update of lg_block_field), done by the
+// bands at the launch's end instead of by a second kernel. Each lane adds its
+// word's fluid-cell counts per block into the band's block sums in LDS (the
+// exchange slots, free by now); the band stores them for its block rows and
+// counts itself in at each block row; the band that arrives last at a block
+// row adds up that row's sums over the bands it spans and updates the field.
+// The sums are integers, so the result does not depend on the order of
+// arrival. The stores and counts move as agent-scope atomics like the halo
+// exchange, so the last band reads what the others stored.
+__device__ __forceinline__ void lg_band_sample(const lgca_internal::LgBandField& A, float alpha,
+                                               uint32_t rows, int32_t* acc, const uint64_t P[7],
+                                               uint64_t sm, bool own, uint32_t y, uint32_t j,
+                                               uint32_t b, uint32_t y0, uint32_t Bb) {
+  const uint32_t tid = threadIdx.x, nthr = blockDim.x;
+  const uint32_t per_band = A.nbr * A.bw * 4;
+  for (uint32_t i = tid; i < per_band; i += nthr) acc[i] = 0;
+  __syncthreads();
+  const uint32_t jb0 = y0 / A.by, xend = A.bw * A.bx;  // columns past the last whole block: none
+  if (own && y < A.bh * A.by && 64 * j < xend) {
+    const int CX2[6] = {2, 1, -1, -2, -1, 1};
+    const int CY2[6] = {0, 1, 1, 0, -1, -1};
+    const uint32_t k = y / A.by - jb0, c0 = 64 * j, c1 = min(c0 + 64, xend);
+    for (uint32_t ib = c0 / A.bx; ib * A.bx < c1; ++ib) {
+      const uint32_t lo = max(ib * A.bx, c0) - c0, hi = min((ib + 1) * A.bx, c1) - c0;
+      const uint64_t m = (hi - lo >= 64) ? ~0ull : (((1ull << (hi - lo)) - 1ull) << lo);
+      const uint64_t fm = m & ~sm;
+      int32_t mass = 0, px = 0, py = 0;
+      LG_UNROLL
+      for (int d = 0; d < 6; ++d) {
+        const int32_t c = __popcll(P[d] & fm);
+        mass += c;
+        px += CX2[d] * c;
+        py += CY2[d] * c;
+      }
+      mass += __popcll(P[6] & fm);
+      int32_t* a = acc + ((size_t)k * A.bw + ib) * 4;
+      atomicAdd(a + 0, mass);
+      atomicAdd(a + 1, px);
+      atomicAdd(a + 2, py);
+      atomicAdd(a + 3, (int32_t)__popcll(sm & m));
+    }
+  }
+  __syncthreads();
+  // block rows [jb0, jb0 + nk) hold rows of this band
+  const uint32_t jb1 = min(A.bh, (y0 + Bb - 1) / A.by + 1);
+  const uint32_t nk = jb0 < jb1 ? jb1 - jb0 : 0u;
+  int32_t* mine = A.part + (size_t)b * per_band;
+  for (uint32_t i = tid; i < nk * A.bw * 4; i += nthr)
+    __hip_atomic_store(mine + i, acc[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);  // this thread's stores are done
+  __syncthreads();                // ... and so are the whole band's
+  uint32_t fin = 0;
+  if (tid < nk) {
+    const uint32_t jb = jb0 + tid;
+    const uint32_t want = (jb * A.by + A.by - 1) / rows - (jb * A.by) / rows + 1;
+    const uint32_t old =
+        __hip_atomic_fetch_add(A.bcnt + jb, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    fin = old + 1 == want;
+    // the last band resets the count for the next launch: every band of
+    // this block row has counted itself in
+    if (fin) __hip_atomic_store(A.bcnt + jb, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  if (tid < nk) acc[tid] = (int32_t)fin;
+  __syncthreads();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  for (uint32_t q = 0; q < nk; ++q) {
+    if (!acc[q]) continue;
+    const uint32_t jb = jb0 + q;
+    const uint32_t bfirst = (jb * A.by) / rows, blast = (jb * A.by + A.by - 1) / rows;
+    for (uint32_t ib = tid; ib < A.bw; ib += nthr) {
+      int64_t s4[4] = {0, 0, 0, 0};
+      for (uint32_t bb = bfirst; bb <= blast; ++bb) {
+        const uint32_t kk = jb - (bb * rows) / A.by;
+        const int32_t* pp = A.part + (size_t)bb * per_band + ((size_t)kk * A.bw + ib) * 4;
+        LG_UNROLL
+        for (int c = 0; c < 4; ++c)
+          s4[c] += __hip_atomic_load(const_cast<int32_t*>(pp + c), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+      }
+      lg_block_field(A.bx, A.by, alpha, s4, A.ux, A.uy, A.rho, A.sol, A.sums, jb * A.bw + ib);
+    }
+  }
+}
+
 // One band per workgroup (blockIdx.x), one row per wave, one word per lane,
 // plus one wave that builds the next step's stream tables; A.n steps from
 // A.src into A.dst. LDS: two exchange slots of the rows' planes 1, 2, 4, 5
 // (rows x 4 KB) and two slots of stream tables (A.nstreams x 4 KB).
-template <bool WAVEROW>
+// SAMPLE: the launch ends with the fused coarse-graining (an instance of its
+// own, so launches without it keep the step loop's register allocation).
+template <bool WAVEROW, bool SAMPLE>
 __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
     lg_band_kernel(lgca_internal::LgBandArgs A) {
   extern __shared__ uint64_t xch[];  // [2 slots][rows][4 planes][64 lanes]
@@ -350,6 +440,19 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
     }
     __syncthreads();
   }
+  // the coarse-graining first: its hand-offs between bands then need not
+  // wait for the final state's stores to complete
+  if (SAMPLE) {
+    // the field's layout is read from device memory here, after the step
+    // loop: as kernel arguments its loads would be hoisted to the kernel's
+    // start and held in registers the loop needs
+    const lgca_internal::LgBandField* fs = A.fs;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(fs));
+#endif
+    lg_band_sample(*fs, A.alpha, A.rows, reinterpret_cast<int32_t*>(xch), P, sm, row_live && live,
+                   y, j, b, y0, Bb);
+  }
   if (row_live && live) {
     LG_UNROLL
     for (int d = 0; d < 7; ++d) A.dst[(size_t)d * T + word] = P[d];
@@ -387,12 +490,53 @@ int ensure_band_buffers(lg_t* g, uint32_t nb) {
   return LG_OK;
 }
 
-// Launch n steps of the resident-band kernel from the handle's tables
-// (tabs: the chunk's first step) on its stream, planes a -> b.
-int launch_band(lg_t* g, uint32_t n) {
+// Block rows one band of `rows` rows can touch, for blocks `by` rows tall.
+static uint32_t band_block_rows(uint32_t rows, uint32_t by) { return (rows - 1) / by + 2; }
+
+bool band_sample_fits(const lg_t* g) {
+  uint32_t rows = 0, nb = 0;
+  if (!g->has_field || g->bw == 0 || g->bh == 0 || !band_eligible(g, &rows, &nb)) return false;
+  // the block sums of one band live in the exchange slots (rows x 4 KB)
+  return (size_t)band_block_rows(rows, g->by) * g->bw * 16 <= (size_t)rows * 4096;
+}
+
+// Launch n steps of the resident-band kernel on the handle's stream, planes
+// a -> b; with `sample`, the launch also coarse-grains its final state into
+// the attached field with weight alpha.
+int launch_band(lg_t* g, uint32_t n, bool sample, float alpha) {
   uint32_t rows = 0, nb = 0;
   if (!band_eligible(g, &rows, &nb)) return fail(LG_EINVAL, "lattice not eligible for resident bands");
   if (int rc = ensure_band_buffers(g, nb)) return rc;
+  if (sample && !band_sample_fits(g)) return fail(LG_EINVAL, "field blocks do not fit a band's LDS");
+  const uint32_t nbr = sample ? band_block_rows(rows, g->by) : 0;
+  if (sample) {
+    const size_t np = (size_t)nb * nbr * g->bw * 4;
+    if (g->band_part.n < np) HIP_TRY(g->band_part.alloc(np));
+    if (g->band_bcnt.n < g->bh) {
+      HIP_TRY(g->band_bcnt.alloc(g->bh));
+      HIP_TRY(hipMemsetAsync(g->band_bcnt.p, 0, (size_t)g->bh * 4, g->stream));
+    }
+    if (!g->band_field.p) HIP_TRY(g->band_field.alloc(1));
+    LgBandField f;
+    std::memset(&f, 0, sizeof f);
+    f.bx = g->bx;
+    f.by = g->by;
+    f.bw = g->bw;
+    f.bh = g->bh;
+    f.nbr = nbr;
+    f.part = g->band_part.p;
+    f.bcnt = g->band_bcnt.p;
+    f.ux = g->fux.p;
+    f.uy = g->fuy.p;
+    f.rho = g->frho.p;
+    f.sol = g->fsol.p;
+    f.sums = g->fsums.p;
+    if (std::memcmp(&f, &g->band_field_host, sizeof f) != 0) {  // attached or resized since
+      HIP_TRY(hipStreamSynchronize(g->stream));  // no launch in flight reads the old one
+      HIP_TRY(hipMemcpy(g->band_field.p, &f, sizeof f, hipMemcpyHostToDevice));
+      g->band_field_host = f;
+    }
+  }
   LgBandArgs A;
   A.G = g->G;
   A.src = g->a.p;
@@ -417,21 +561,22 @@ int launch_band(lg_t* g, uint32_t n) {
   // inflow stream
   const uint32_t cpu_band = lgca::band_rows(g->G.h, g->bands);
   A.nstreams = std::min<uint32_t>(rows, (rows + cpu_band - 2) / cpu_band + 1) + 1;
+  A.alpha = alpha;
+  A.fs = g->band_field.p;
   void* args[] = {&A};
   const dim3 grid(nb), block(64 * (rows + 1));
   const size_t lds = (size_t)(rows + A.nstreams) * 4096;
+  const void* const kernels[2][2] = {
+      {(const void*)lg_band_kernel<false, false>, (const void*)lg_band_kernel<false, true>},
+      {(const void*)lg_band_kernel<true, false>, (const void*)lg_band_kernel<true, true>}};
   if (lds > 64 * 1024 && !g->band_lds_raised) {
-    HIP_TRY(hipFuncSetAttribute((const void*)lg_band_kernel<true>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    HIP_TRY(hipFuncSetAttribute((const void*)lg_band_kernel<false>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    for (int q = 0; q < 4; ++q)
+      HIP_TRY(hipFuncSetAttribute(kernels[q >> 1][q & 1], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024));
     g->band_lds_raised = true;
   }
-  hipError_t e;
-  if (g->G.wpr == 64)
-    e = hipLaunchCooperativeKernel((const void*)lg_band_kernel<true>, grid, block, args, lds, g->stream);
-  else
-    e = hipLaunchCooperativeKernel((const void*)lg_band_kernel<false>, grid, block, args, lds, g->stream);
+  const hipError_t e = hipLaunchCooperativeKernel(kernels[g->G.wpr == 64][sample], grid, block, args,
+                                                  lds, g->stream);
   if (e != hipSuccess)
     return fail(LG_EDEVICE, std::string("resident-band launch: ") + hipGetErrorString(e));
   g->band_check = true;

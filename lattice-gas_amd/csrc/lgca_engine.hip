@@ -591,6 +591,9 @@ void destroy_lattice(lg_lattice* g) {
   g->band_halo.release();
   g->band_flags.release();
   g->band_err_dev.release();
+  g->band_part.release();
+  g->band_bcnt.release();
+  g->band_field.release();
   if (g->band_err_host) (void)hipHostFree(g->band_err_host);
   g->band_err_host = nullptr;
   g->fux.release();
@@ -857,7 +860,11 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
         }
         HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used], g->stream));
       }
-      if (int rc = launch_band(g, (uint32_t)chunk)) return rc;
+      // a chunk followed by a sample coarse-grains its final state in the
+      // same launch where the field's blocks fit
+      const bool sample_after = every > 0 && (since + chunk >= every || left == chunk);
+      const bool fused = sample_after && band_sample_fits(g);
+      if (int rc = launch_band(g, (uint32_t)chunk, fused, alpha)) return rc;
       if (timed) HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used++ + 1], g->stream));
       std::swap(g->a.p, g->b.p);
       g->steps += chunk;
@@ -865,8 +872,10 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
       since += chunk;
       if (every > 0 && (since >= every || left == 0)) {
         since = 0;
-        int rc = sample_now(g, alpha);
-        if (rc) return rc;
+        if (!fused) {
+          int rc = sample_now(g, alpha);
+          if (rc) return rc;
+        }
       }
     }
     return LG_OK;

@@ -108,6 +108,21 @@ struct LgBandArgs {
   uint64_t seed, t0;  // the run's seed, the launch's first step
   uint32_t n, rows, nb;
   uint32_t nstreams;  // stream tables per band and step in LDS (band streams + inflow)
+  // Coarse-graining of the launch's final state with weight alpha (the
+  // kernel instance that does it); the field's layout in device memory at
+  // `fs`, read only after the step loop.
+  float alpha;
+  const struct LgBandField* fs;
+};
+
+// The attached field as the resident-band launch's fused coarse-graining
+// sees it: bx x by blocks, bw x bh of them, nbr block rows at most per band.
+struct LgBandField {
+  uint32_t bx, by, bw, bh, nbr;
+  int32_t* part;    // [nb][nbr][bw][4]: each band's block sums over its rows
+  uint32_t* bcnt;   // [bh]: bands arrived per block row (0 between launches)
+  float *ux, *uy, *rho, *sol;
+  int64_t* sums;
 };
 
 // Free a handle and everything it owns (after its streams drain).
@@ -127,7 +142,12 @@ int ensure_draw_buffers(lg_lattice* g);
 // launch since the last check ran out of a wait.
 bool band_eligible(const lg_lattice* g, uint32_t* rows, uint32_t* nb);
 bool resident_active(const lg_lattice* g, uint32_t* rows, uint32_t* nb);
-int launch_band(lg_lattice* g, uint32_t n);
+// Whether a band launch can take the field's coarse-graining into its end
+// (the block sums of one band fit the LDS it no longer needs by then).
+bool band_sample_fits(const lg_lattice* g);
+// n steps; with `sample`, the coarse-graining of the final state with
+// weight alpha (what sample_now would do next) is done by the same launch.
+int launch_band(lg_lattice* g, uint32_t n, bool sample = false, float alpha = 0.0f);
 int check_band(lg_lattice* g);
 
 }  // namespace lgca_internal
@@ -191,6 +211,10 @@ struct lg_lattice {
   uint64_t band_epoch = 0, band_epoch_checked = 0;
   bool band_check = false;
   bool band_lds_raised = false;
+  DevBuf<int32_t> band_part;   // per-band block sums of the fused coarse-graining
+  DevBuf<uint32_t> band_bcnt;  // bands arrived per block row
+  DevBuf<lgca_internal::LgBandField> band_field;  // what the launch reads of the field
+  lgca_internal::LgBandField band_field_host{};   // ... as last copied there
   hipStream_t dstream = nullptr;
   hipEvent_t ev_drawn[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
   hipEvent_t ev_start = nullptr;

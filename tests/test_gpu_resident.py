@@ -111,4 +111,58 @@ def test_plate_workload_9m_cells(oracle):
         assert np.array_equal(g.store(), lat.cells)
     for a, b in zip(g.read_block_sums(), f.block_sums(lat)):
         assert np.array_equal(a, b)
+    for a, b in zip(g.read_field(), f.read()):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    g.close()
+
+
+def replay_sampling(lat, f, n, every, alpha):
+    """What one advance_sampling(n, every, alpha) call does (src/sim.rs:119-143):
+    a sample after every `every` steps of the call and one at its end."""
+    left, since = n, 0
+    while left:
+        c = min(left, every - since)
+        lat.advance(c)
+        left -= c
+        since += c
+        if since >= every or left == 0:
+            since = 0
+            f.sample(lat, alpha)
+
+
+@pytest.mark.parametrize("w,h,bands,bx,by", [
+    (256, 2306, 16, 20, 20),   # ten-row bands, two bands per block row, blocks across words
+    (320, 600, 16, 7, 3),      # three-row bands, as tall as a block row
+    (1000, 26, 2, 100, 5),     # one-row bands, blocks wider than a word
+    (4096, 96, 3, 64, 40),     # 64-word rows, forty bands per block row, a partial block row
+    (129, 700, 4, 43, 9),      # a partial block column
+    (4096, 96, 3, 1, 1),       # blocks too many for a band's LDS: the separate kernel
+])
+def test_coarse_graining_in_the_band_launch(oracle, w, h, bands, bx, by):
+    """A chunk that ends at a sample coarse-grains its final state in the
+    band launch itself: block sums and the f32 running average
+    (src/render.rs:93-115, alpha 0.3) bit for bit with the oracle's, over
+    calls that sample inside and at their end."""
+    seed = 0xF1E1D + w + bx
+    say(f"fused sample {w}x{h} bands={bands} blocks {bx}x{by}")
+    lat = oracle.Lattice(w, h, 0.22, (0.3, 0.0), 8, True, bands, seed)
+    yc = float(np.float32(np.float32(h) * oracle.SQRT3_2 / np.float32(2)))
+    lat.add_solid_disc(w / 3.0, yc, 5.5)
+    lat.init_equilibrium()
+    g = GpuLattice(w, h, True, seed, bands)
+    g.set_inlet(8, 0.22, (0.3, 0.0))
+    g.load(lat.cells)
+    g.set_resident(True)
+    assert g.resident() is not None
+    g.attach_field(bx, by)
+    f = oracle.Field(w, h, bx, by)
+    for n, every in ((20, 100), (25, 10), (7, 3), (40, 40)):
+        replay_sampling(lat, f, n, every, 0.3)
+        g.advance_sampling(n, every, 0.3)
+        assert np.array_equal(g.store(), lat.cells), f"cells after {lat.steps} steps"
+        for name, a, b in zip(("ux", "uy", "rho", "solid"), g.read_field(), f.read()):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), \
+                f"field {name} after {lat.steps} steps"
+        for a, b in zip(g.read_block_sums(), f.block_sums(lat)):
+            assert np.array_equal(a, b), f"block sums after {lat.steps} steps"
     g.close()
