@@ -147,6 +147,19 @@ LG_HD inline uint32_t lg_band_row_of_wave(uint32_t w, uint32_t B) {
   return 1 + w - (w > wt ? 1u : 0u) - (w > wb ? 1u : 0u);
 }
 
+// The band row wave w holds when the band's first row has its draws split
+// between two helper waves (bands of B = 4m + 1 rows): waves 0 .. B - 2 hold
+// rows 1 .. B - 1 (the last row, which waits for and publishes to the band
+// below, on wave 1, which shares its SIMD with the table wave), wave B - 1
+// holds row 0 and only collides it; the table wave (B) and the helpers (B +
+// 1, B + 2) fill the other SIMDs' third slots, so every SIMD holds 2m rows
+// and a part of the remaining row's work.
+LG_HD inline uint32_t lg_band_row_split(uint32_t w, uint32_t B) {
+  if (w == B - 1) return 0;
+  if (B >= 3 && w <= 1) return w == 0 ? 1u : B - 1;
+  return w;
+}
+
 // The stream tables one band needs for step t into LDS (out: nstreams x
 // LG_WTAB entries): its row bands' collision streams rb0, rb0 + 1, ... and,
 // last, the inflow stream of the step's end (src/lattice.rs:233,242,276);
@@ -272,9 +285,20 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
   const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, NW = A.rows;
   uint64_t* const tabs_lds = xch + (size_t)2 * NW * 256;  // [2 slots][nstreams][LG_WTAB]
   const bool table_wave = w == NW;
+  // split draws: row 0's digits are drawn by two helper waves, half a word
+  // per lane each, into LDS ([2 slots][2 halves][LG_KDIGITS][64 lanes])
+  const bool split = A.split != 0;
+  const bool helper = split && w > NW;
+  const uint32_t hh = helper ? w - NW - 1 : 0u;  // the half a helper draws
+  const bool drawless = split && w + 1 == NW;    // row 0's wave: digits from the helpers
+  uint32_t* const kx = reinterpret_cast<uint32_t*>(tabs_lds + (size_t)2 * A.nstreams * LG_WTAB);
   const uint32_t b = blockIdx.x, y0 = b * A.rows;
   const uint32_t Bb = min(A.rows, G.h - y0);  // rows of this band
-  const uint32_t r = table_wave ? NW : lg_band_row_of_wave(w, A.rows);  // the band row it holds
+  // the band row it holds (helpers: none, but y below falls on row 0, whose
+  // words they draw)
+  const uint32_t r = (table_wave || helper) ? NW
+                     : split                ? lg_band_row_split(w, NW)
+                                            : lg_band_row_of_wave(w, NW);
   const bool row_live = r < Bb;
   const uint32_t y = y0 + (row_live ? r : 0u);
   const bool live = lane < G.wpr;
@@ -295,7 +319,7 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
   for (int d = 0; d < 7; ++d) P[d] = A.src[(size_t)d * T + word];
   const uint64_t sm = A.solid[word];
   const uint64_t fm = live ? ~sm & tail : 0ull;
-  const uint64_t cj0 = A.cjump[2 * word];
+  const uint64_t cj0 = A.cjump[2 * word + hh];  // a helper's half starts at its own jump
   const uint32_t stream = A.row_band[y];
   // inflow: for widths of at most 64 cells lane x samples cell x of the
   // row's first word from its own jump (icell); wider inflow runs the
@@ -369,10 +393,19 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
       uint64_t n[7];
       lg_gather(odd, cl, cr, P[0], a0l, a1, a1l, a2, a2r, P[3], a3r, a4, a4r, a5, a5l, P[6], n);
       uint32_t Kl[LG_KDIGITS], Kh[LG_KDIGITS];
-      LG_UNROLL
-      for (int q = 0; q < LG_KDIGITS; ++q) {
-        Kl[q] = (uint32_t)K[q];
-        Kh[q] = (uint32_t)(K[q] >> 32);
+      if (drawless) {
+        const uint32_t* kin = kx + (size_t)(s & 1u) * 2 * LG_KDIGITS * 64;
+        LG_UNROLL
+        for (int q = 0; q < LG_KDIGITS; ++q) {
+          Kl[q] = kin[q * 64 + lane];
+          Kh[q] = kin[(LG_KDIGITS + q) * 64 + lane];
+        }
+      } else {
+        LG_UNROLL
+        for (int q = 0; q < LG_KDIGITS; ++q) {
+          Kl[q] = (uint32_t)K[q];
+          Kh[q] = (uint32_t)(K[q] >> 32);
+        }
       }
       uint64_t res[7];
       lg_collide_word(G, n, sm, Kl, Kh, res);
@@ -415,7 +448,7 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
       // the compiler hoists them out of the loop and spills them
       asm volatile("" : "+v"(c0), "+v"(ci), "+v"(cw));
 #endif
-      if (live) {
+      if (live && !drawless) {
         if (G.rest)
           lg_band_digits<LG_COLLIDE_REST_KBITS>(fm, c0, tab + (size_t)(stream - rb0) * LG_WTAB,
                                                 LG_COLLIDE_REST_STRIDE, K);
@@ -437,6 +470,37 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
         const uint64_t apply = lg_inlet_apply(G, j, fm);
         if (apply) lg_inlet_bits(G, lg_jump_tab(cw, itab), apply, I);
       }
+    }
+    // step i's draws of row 0, one half per helper
+    if (helper && i < A.n) {
+      const uint64_t* tab = tabs_lds + (size_t)(i & 1u) * A.nstreams * LG_WTAB +
+                            (size_t)(stream - rb0) * LG_WTAB;
+      uint64_t c0 = cj0;
+#if defined(__HIP_DEVICE_COMPILE__)
+      asm volatile("" : "+v"(c0));
+#endif
+      uint32_t Kx[LG_KDIGITS];
+      LG_UNROLL
+      for (int q = 0; q < LG_KDIGITS; ++q) Kx[q] = 0;
+      if (live) {
+        const uint32_t fh = hh ? (uint32_t)(fm >> 32) : (uint32_t)fm;
+        const uint64_t st = lg_jump_tab(c0, tab);
+        const bool masked = __ballot(fh != ~0u) != 0;
+        if (G.rest) {
+          if (masked)
+            lg_draw_half_sel<LG_COLLIDE_REST_KBITS>(st, fh, LG_COLLIDE_REST_STRIDE, Kx);
+          else
+            lg_draw_half<LG_COLLIDE_REST_KBITS>(st, LG_COLLIDE_REST_STRIDE, Kx);
+        } else {
+          if (masked)
+            lg_draw_half_sel<LG_COLLIDE_NOREST_KBITS>(st, fh, LG_COLLIDE_NOREST_STRIDE, Kx);
+          else
+            lg_draw_half<LG_COLLIDE_NOREST_KBITS>(st, LG_COLLIDE_NOREST_STRIDE, Kx);
+        }
+      }
+      uint32_t* o = kx + ((size_t)(i & 1u) * 2 + hh) * LG_KDIGITS * 64;
+      LG_UNROLL
+      for (int q = 0; q < LG_KDIGITS; ++q) o[q * 64 + lane] = Kx[q];
     }
     __syncthreads();
   }
@@ -561,11 +625,14 @@ int launch_band(lg_t* g, uint32_t n, bool sample, float alpha) {
   // inflow stream
   const uint32_t cpu_band = lgca::band_rows(g->G.h, g->bands);
   A.nstreams = std::min<uint32_t>(rows, (rows + cpu_band - 2) / cpu_band + 1) + 1;
+  // bands of 4m + 1 rows split their first row's draws between two helper
+  // waves, where the workgroup has room for them
+  A.split = (rows % 4 == 1 && rows + 3 <= LG_BAND_MAX_ROWS + 1) ? 1u : 0u;
   A.alpha = alpha;
   A.fs = g->band_field.p;
   void* args[] = {&A};
-  const dim3 grid(nb), block(64 * (rows + 1));
-  const size_t lds = (size_t)(rows + A.nstreams) * 4096;
+  const dim3 grid(nb), block(64 * (rows + 1 + 2 * A.split));
+  const size_t lds = (size_t)(rows + A.nstreams) * 4096 + (A.split ? 2 * 2 * LG_KDIGITS * 64 * 4 : 0);
   const void* const kernels[2][2] = {
       {(const void*)lg_band_kernel<false, false>, (const void*)lg_band_kernel<false, true>},
       {(const void*)lg_band_kernel<true, false>, (const void*)lg_band_kernel<true, true>}};

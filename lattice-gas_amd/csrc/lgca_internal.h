@@ -108,6 +108,7 @@ struct LgBandArgs {
   uint64_t seed, t0;  // the run's seed, the launch's first step
   uint32_t n, rows, nb;
   uint32_t nstreams;  // stream tables per band and step in LDS (band streams + inflow)
+  uint32_t split;     // row 0's draws by two helper waves (bands of 4m + 1 rows)
   // Coarse-graining of the launch's final state with weight alpha (the
   // kernel instance that does it); the field's layout in device memory at
   // `fs`, read only after the step loop.

@@ -58,6 +58,8 @@ def run(oracle, w, h, rest, bands, inlet, seed, calls, density=0.22, inflow=(0.3
     (1000, 26, True, 2, 8),
     (256, 2306, True, 16, 8),   # ten-row bands, a six-row last band
     (320, 600, True, 16, 8),    # three-row bands
+    (192, 1200, True, 16, 8),   # five-row bands: the first row's draws on two helper waves
+    (130, 2250, False, 5, 70),  # nine-row bands with helpers, no rest, wide inflow
 ])
 def test_ragged_shapes(oracle, w, h, rest, bands, inlet):
     run(oracle, w, h, rest, bands, inlet, 0xBA5E + w, (1, 2, 9, 16, 5))
