@@ -277,7 +277,10 @@ __device__ __forceinline__ void lg_band_sample(const lgca_internal::LgBandField&
 // (rows x 4 KB) and two slots of stream tables (A.nstreams x 4 KB).
 // SAMPLE: the launch ends with the fused coarse-graining (an instance of its
 // own, so launches without it keep the step loop's register allocation).
-template <bool WAVEROW, bool SAMPLE>
+// WIDE: inflow wider than one word (the serial per-word chain); otherwise one
+// lane per inflow cell. Each choice is an instance of its own, so the other's
+// registers are not held across the step loop.
+template <bool WAVEROW, bool SAMPLE, bool WIDE>
 __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
     lg_band_kernel(lgca_internal::LgBandArgs A) {
   extern __shared__ uint64_t xch[];  // [2 slots][rows][4 planes][64 lanes]
@@ -324,12 +327,15 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
   // inflow: for widths of at most 64 cells lane x samples cell x of the
   // row's first word from its own jump (icell); wider inflow runs the
   // serial per-word chain (ijump) in the lanes of the inflow words
-  const bool lane_inflow = G.inlet_cols <= 64;
+  constexpr bool lane_inflow = !WIDE;
   const uint64_t sm0 = A.solid[(size_t)y * G.wpr];
   const bool icell_live = lane_inflow && lane < G.inlet_cols && !((sm0 >> lane) & 1ull);
   const uint64_t ic = icell_live ? A.icell[(size_t)y * G.inlet_cols + lane] : 0ull;
   const uint64_t ij = (!lane_inflow && j < G.iw) ? A.ijump[2 * ((size_t)y * G.iw + j)] : 0ull;
+  // the step's inflow: wide, the word's seven planes (I); narrow, the lane's
+  // cell (ibits), gathered across the wave where it is applied
   uint64_t K[LG_KDIGITS], I[7];
+  uint32_t ibits = 0;
   LG_UNROLL
   for (int q = 0; q < LG_KDIGITS; ++q) K[q] = 0;
   LG_UNROLL
@@ -409,6 +415,10 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
       }
       uint64_t res[7];
       lg_collide_word(G, n, sm, Kl, Kh, res);
+      if constexpr (!WIDE) {  // cells 0 .. inlet_cols - 1 of word 0
+        LG_UNROLL
+        for (int d = 0; d < 7; ++d) I[d] = __ballot((ibits >> d) & 1u);
+      }
       if (j < G.iw) lg_inflow_apply(G, j, sm, tail, I, res);
       LG_UNROLL
       for (int d = 0; d < 7; ++d) P[d] = res[d] & tail;
@@ -456,15 +466,12 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
           lg_band_digits<LG_COLLIDE_NOREST_KBITS>(fm, c0, tab + (size_t)(stream - rb0) * LG_WTAB,
                                                   LG_COLLIDE_NOREST_STRIDE, K);
       }
-      if (lane_inflow) {
-        if (G.inlet_cols > 0) {  // cell `lane` of word 0, gathered across the wave
-          uint32_t bits = 0;
-          if (icell_live) {
-            uint64_t st = lg_jump_tab(ci, itab);
-            bits = lg_eq_cell(G, st);
-          }
-          LG_UNROLL
-          for (int d = 0; d < 7; ++d) I[d] = __ballot((bits >> d) & 1u);
+      if constexpr (lane_inflow) {
+        // cell `lane` of word 0 (zero where it takes no inflow)
+        ibits = 0;
+        if (icell_live) {
+          uint64_t st = lg_jump_tab(ci, itab);
+          ibits = lg_eq_cell(G, st);
         }
       } else if (live && j < G.iw) {
         const uint64_t apply = lg_inlet_apply(G, j, fm);
@@ -633,17 +640,20 @@ int launch_band(lg_t* g, uint32_t n, bool sample, float alpha) {
   void* args[] = {&A};
   const dim3 grid(nb), block(64 * (rows + 1 + 2 * A.split));
   const size_t lds = (size_t)(rows + A.nstreams) * 4096 + (A.split ? 2 * 2 * LG_KDIGITS * 64 * 4 : 0);
-  const void* const kernels[2][2] = {
-      {(const void*)lg_band_kernel<false, false>, (const void*)lg_band_kernel<false, true>},
-      {(const void*)lg_band_kernel<true, false>, (const void*)lg_band_kernel<true, true>}};
+  // [rows of 64 words][sampling][inflow wider than a word]
+  const void* const kernels[2][2][2] = {
+      {{(const void*)lg_band_kernel<false, false, false>, (const void*)lg_band_kernel<false, false, true>},
+       {(const void*)lg_band_kernel<false, true, false>, (const void*)lg_band_kernel<false, true, true>}},
+      {{(const void*)lg_band_kernel<true, false, false>, (const void*)lg_band_kernel<true, false, true>},
+       {(const void*)lg_band_kernel<true, true, false>, (const void*)lg_band_kernel<true, true, true>}}};
   if (lds > 64 * 1024 && !g->band_lds_raised) {
-    for (int q = 0; q < 4; ++q)
-      HIP_TRY(hipFuncSetAttribute(kernels[q >> 1][q & 1], hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024));
+    for (int q = 0; q < 8; ++q)
+      HIP_TRY(hipFuncSetAttribute(kernels[q >> 2][(q >> 1) & 1][q & 1],
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     g->band_lds_raised = true;
   }
-  const hipError_t e = hipLaunchCooperativeKernel(kernels[g->G.wpr == 64][sample], grid, block, args,
-                                                  lds, g->stream);
+  const hipError_t e = hipLaunchCooperativeKernel(
+      kernels[g->G.wpr == 64][sample][g->G.inlet_cols > 64], grid, block, args, lds, g->stream);
   if (e != hipSuccess)
     return fail(LG_EDEVICE, std::string("resident-band launch: ") + hipGetErrorString(e));
   g->band_check = true;
