@@ -649,7 +649,7 @@ def main():
         # the launch itself does: the kernel's sampling instance) or at 65536
         # steps
         kernel = (f"lg_band_kernel<{'true' if cfg['w'] > 4032 else 'false'},"
-                  f"{'true' if every else 'false'}>")
+                  f"{'true' if every else 'false'},{'true' if cfg['inlet'] > 64 else 'false'}>")
         steps_per_launch = min(args.steps, every or args.steps, 65536)
     elif passes == 1:
         kernel = f"lg_step_kernel_{unit}"
