@@ -12,9 +12,11 @@ before the timed region starts.
 Prints one JSON line. `value` is cell updates per second (GCUPS) over the
 timed steps; `roofline.achieved` is the algorithmic traffic of one step
 (15 bits per cell: 7 planes read, 7 written, the solid plane read) divided by
-the mean device time of one step's kernels (the collide launch plus its share
-of the draw launch that made its random columns, or the one fused launch),
-from HIP event pairs on the engine's stream around each run of step launches.
+the mean device time per step, from HIP event pairs on the engine's stream
+around each launch of steps (at this size the resident-band kernel: one
+cooperative launch per chunk of steps, a chunk ending at a sample, which the
+launch itself takes); `roofline.traffic` is the HBM bytes of one such launch
+from the committed rocprofv3 summary of the same kernel and workload.
 `cpu_baseline` is the CPU oracle (a restatement of the reference's CPU path,
 one thread per row band, spawned per step as the reference does) replaying
 the same run on this host; its final cells are compared with the GPU's.
@@ -23,13 +25,17 @@ Further objects in the same line: `small_512x256` (the reference's 512 x 256
 CPU-runnable case on the GPU; its CPU rates and check sit under
 `cpu_baseline`), `shear_537m` (the 32768 x 16384 transport workload's step
 rate), `ensemble` (256 uneven lattices, batched vs one at a time),
-`serving` (a Poisson request stream, continuous batching vs the plain loop),
+`serving` (256 Poisson requests offered below the batched server's capacity,
+continuous batching vs the plain loop, with the offered load and whether each
+server keeps up), `serving_saturated` (64 requests at 50/s, above capacity: the
+servers' throughput when the queue never empties),
 `transport_measure` (the reference's --measure run on the engine: viscosity and
 advection factor with their wall time), `frames` (the frame writers on the
 reference's render-benchmark field, and what queueing a frame every 500 steps
 to the writer pool costs the headline run's stepping loop) and
 `reference_suite` (the reference's own GPU benchmark cases, same lattices and
-harness, beside its published figures).
+harness, beside its published figures) and `autotune` (the pipeline lg_autotune
+keeps for the headline workload).
 """
 from __future__ import annotations
 
@@ -147,11 +153,12 @@ def ensemble_leg(n, steps, bands, seed=4):
     lats = [build(sp) for sp in specs]
     b = Batch(lats)
     t0 = time.perf_counter()
-    used = b.autotune(budget_steps=min(400, steps // 2))
+    used = b.autotune(budget_steps=min(512, steps // 4))
     b.advance(steps - used)
     t_batch = time.perf_counter() - t0
     shape = b.shape()
     passes, units, dunits = b.pipeline()
+    waves = b.waves()
     b.close()
     # the plain loop over the same lattices (rebuilt), one at a time
     t_plain = 0.0
@@ -169,10 +176,38 @@ def ensemble_leg(n, steps, bands, seed=4):
     return {"lattices": n, "steps": steps, "cells": cells, "bands": bands,
             "batched_gcups": round(gc_b, 3), "plain_loop_gcups": round(gc_p, 3),
             "speedup": round(gc_b / gc_p, 3), "batched_s": round(t_batch, 4),
+            "autotune_steps": used,
             "autotuned_shape": {"tile": shape[0], "grid": shape[1], "chunk": shape[2],
                                 "passes": passes, "cells_per_lane": units,
-                                "draw_cells_per_lane": dunits},
+                                "draw_cells_per_lane": dunits, "waves_per_simd": waves},
             "plain_loop_sample": f"all {probe} lattices, {steps} steps each"}
+
+
+def autotune_leg(bands):
+    """lg_autotune on a lattice of the headline workload (built apart from the
+    timed one): which step pipeline -- the size rule, resident bands on or
+    off, one-launch two-pass, draw-ahead, fused, temporally blocked collides
+    of depth 2, 4, 8 -- it keeps, and the rate of the kept pipeline over 200
+    more steps."""
+    import workloads as W
+
+    cfg = W.config2(bands=bands)
+    g = W.config_gpu(cfg)
+    t0 = time.perf_counter()
+    used = g.autotune(0)
+    t_tune = time.perf_counter() - t0
+    passes, cpl, dcpl, ahead = g.pipeline()
+    depth, rows = g.temporal_depth()
+    res = g.resident()
+    g.advance(20)
+    t0 = time.perf_counter()
+    g.advance(200)
+    dt = time.perf_counter() - t0
+    g.close()
+    return {"workload": cfg["name"], "used_steps": used, "tune_s": round(t_tune, 4),
+            "kept": {"resident_bands": list(res) if res else None, "passes": passes,
+                     "draw_ahead": ahead, "temporal_depth": depth},
+            "kept_gcups_200_steps": round(cfg["w"] * cfg["h"] * 200 / dt / 1e9, 3)}
 
 
 class _Skip(Exception):
@@ -497,13 +532,25 @@ def serving_leg(n, rate, seed=5):
     """Served stream: Poisson arrivals, log-uniform sizes 256x256..4096x2304,
     steps 500-20000, random plate and drift, HBM budget 200 GB, 32x32-block
     velocity snapshot every 100 steps; continuous batching against the plain
-    sequential loop on the same trace."""
+    sequential loop on the same trace.
+
+    `offered_gcups` is the trace's cell updates over the span of its arrivals:
+    the rate the stream asks for. A server whose `gcups` (updates over its
+    makespan) is below that is overloaded: its queue grows for as long as
+    requests keep arriving, and its latencies measure the trace's length, not
+    a steady state; `load` is offered over delivered."""
     import serve
 
     trace = serve.make_trace(n, rate=rate, seed=seed)
+    span = trace[-1].arrival - trace[0].arrival
+    offered = sum(r.cells * r.steps for r in trace) / span / 1e9 if span > 0 else float("inf")
     s_b, _ = serve.serve_batched(trace, every=100, block=32)
     s_p, _ = serve.serve_plain(trace, every=100, block=32)
-    return {"requests": n, "rate_per_s": rate, "batched": s_b, "plain_loop": s_p,
+    for s in (s_b, s_p):
+        s["load"] = round(offered / s["gcups"], 3)
+        s["state"] = "overloaded" if offered > s["gcups"] else "keeps up"
+    return {"requests": n, "rate_per_s": rate, "arrival_span_s": round(span, 3),
+            "offered_gcups": round(offered, 1), "batched": s_b, "plain_loop": s_p,
             "throughput_ratio": round(s_b["gcups"] / s_p["gcups"], 3)}
 
 
@@ -566,15 +613,23 @@ def main():
     ap.add_argument("--ensemble", type=int, default=256, metavar="N",
                     help="lattices in the batched-ensemble leg (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=3000)
-    ap.add_argument("--serve", type=int, default=64, metavar="N",
-                    help="requests in the serving leg (0 skips it)")
-    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve", type=int, default=256, metavar="N",
+                    help="requests in the steady-state serving leg (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=22.0,
+                    help="arrivals per second of the steady-state leg (about 420 GCUPS offered, "
+                         "under the batched server's capacity)")
+    ap.add_argument("--serve-saturated", type=int, default=64, metavar="N",
+                    help="requests in the saturated serving leg at --serve-saturated-rate "
+                         "(0 skips it)")
+    ap.add_argument("--serve-saturated-rate", type=float, default=50.0)
     ap.add_argument("--no-transport", action="store_true",
                     help="skip the transport-measure leg (the reference's --measure)")
     ap.add_argument("--no-frames", action="store_true",
                     help="skip the frame-output leg")
     ap.add_argument("--no-reference-suite", action="store_true",
                     help="skip the reference's own GPU benchmark cases")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the single-lattice autotune leg (lg_autotune on the headline workload)")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 512 x 256 leg (keeps profiles to the headline lattice)")
     ap.add_argument("--shear-steps", type=int, default=400, metavar="N",
@@ -730,6 +785,11 @@ def main():
             out["reference_suite"] = reference_suite_leg(bands)
         except Exception as e:
             out["reference_suite"] = {"error": repr(e)}
+    if not args.no_autotune:
+        try:
+            out["autotune"] = autotune_leg(bands)
+        except Exception as e:
+            out["autotune"] = {"error": repr(e)}
     if args.ensemble:
         try:
             out["ensemble"] = ensemble_leg(args.ensemble, args.ensemble_steps, bands)
@@ -740,6 +800,11 @@ def main():
             out["serving"] = serving_leg(args.serve, args.serve_rate)
         except Exception as e:
             out["serving"] = {"error": repr(e)}
+    if args.serve_saturated:
+        try:
+            out["serving_saturated"] = serving_leg(args.serve_saturated, args.serve_saturated_rate)
+        except Exception as e:
+            out["serving_saturated"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
 
 

@@ -180,7 +180,9 @@ int lg_set_temporal_depth(lg_t* g, uint32_t steps, uint32_t rows_per_tile);
 /* New. Resident bands (on = 1; 2, the default, = on unless lg_set_passes,
  * lg_set_draw_ahead or lg_set_temporal_depth chose another pipeline; 0 =
  * off): each call's steps run in one cooperative
- * launch per chunk (a chunk ends at a sample or at 256 steps), in which one
+ * launch per chunk (a chunk ends at a sample or at 65536 steps; a chunk that
+ * ends at a sample takes it in the same launch where a band's block sums fit
+ * its LDS), in which one
  * workgroup per band of rows keeps its rows in registers for the whole
  * chunk, draws its own random digits one step ahead and exchanges only its
  * edge rows with the neighbouring bands through global memory and per-band
@@ -195,6 +197,18 @@ int lg_set_temporal_depth(lg_t* g, uint32_t steps, uint32_t rows_per_tile);
  * reloaded. */
 int lg_set_resident(lg_t* g, uint32_t on);
 int lg_get_resident(lg_t* g, uint32_t* rows_per_band, uint32_t* bands);
+/* New: the temporal depth and tile rows set by lg_set_temporal_depth. */
+int lg_get_temporal_depth(lg_t* g, uint32_t* steps, uint32_t* rows_per_tile);
+/* New: time the step pipelines this lattice can run -- the size rule,
+ * resident bands on and off, the two-pass step in one launch, draw-ahead by
+ * 16 steps, the fused step, and the temporally blocked collide at depths 2,
+ * 4 and 8 (rows of at most 64 words) -- on the lattice itself, and keep the
+ * fastest. Each candidate runs one warm call of up to 32 steps and is timed
+ * as the best of three more; the steps are real steps of the lattice
+ * (results are identical in every pipeline), reported in *used_steps;
+ * budget_steps = 0: no limit. The choice is read back with lg_get_pipeline,
+ * lg_get_resident and lg_get_temporal_depth. */
+int lg_autotune(lg_t* g, uint64_t budget_steps, uint64_t* used_steps);
 
 /* New: advance n steps and report the device time of the step kernels in ms:
  * the sum over HIP event pairs recorded on the handle's stream around each
@@ -297,6 +311,13 @@ size_t lg_batch_snapshot_noalloc_max(void);
  * tiles), chunk = steps per graph replay (even, at most 256). */
 int lg_batch_set_shape(lg_batch_t* b, uint32_t tile, uint32_t grid, uint32_t chunk);
 int lg_batch_get_shape(lg_batch_t* b, uint32_t* tile, uint32_t* grid, uint32_t* chunk);
+/* New: register target of the batch's step kernel, in waves per SIMD (0: the
+ * pipeline's default, 3 for the fused step and 5 for the two-pass step).
+ * Whole-word lanes also run at 2 or 4 (fused) and 4 or 6 (two-pass); other
+ * values (2..6) fall back to the default, and lg_batch_get_waves reports the
+ * target actually used. Results are identical for every choice. */
+int lg_batch_set_waves(lg_batch_t* b, uint32_t waves_per_simd);
+int lg_batch_get_waves(lg_batch_t* b, uint32_t* waves_per_simd);
 /* Cells per lane (64 or 32) for the batch's step kernel, as lg_set_work_unit. */
 int lg_batch_set_units(lg_batch_t* b, uint32_t cells_per_lane);
 int lg_batch_get_units(lg_batch_t* b, uint32_t* cells_per_lane);
@@ -314,9 +335,11 @@ int lg_batch_sample(lg_batch_t* b, float alpha);
  * stream; the next synchronous call on the handle sees it). */
 int lg_sample_field_async(lg_t* g, float alpha);
 /* Time candidate launch pipelines and shapes (kernels per step, cells per
- * lane for the collide and draw work, tile, grid) on the batch itself, most
- * promising first, and keep the fastest. The timed steps are real steps of
- * every member, reported in *used_steps; budget_steps = 0: no limit. */
+ * lane for the collide and draw work, tile, grid, waves per SIMD) on the
+ * batch itself, most promising first, and keep the fastest: each candidate
+ * runs one warm graph replay of up to 32 steps and is timed as the best of
+ * three more. The timed steps are real steps of every member, reported in
+ * *used_steps; budget_steps = 0: no limit. */
 int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps);
 
 /* ---- Frame output (host code; no GPU except lg_frames_write_lattice) ------

@@ -83,13 +83,14 @@ __global__ void __launch_bounds__(256) lg_tables_batch_kernel(const LgDesc* __re
 
 // Step k of the chunk for every tile. A tile is TILE consecutive thread
 // indices of one member, mapped to work units exactly as lg_step_kernel_{w,h} map
-// them (the member's static word order, NH units per word). Register target:
-// three waves per SIMD, which leaves the tile loop room to keep everything in
-// registers (134 VGPRs, no scratch); at four waves it spills 32 bytes a lane
-// and the 256-lattice ensemble runs 3% slower. (The single-lattice fused
-// kernel is the other way round: 7% slower at three waves.)
-template <int TILE, int NH>
-__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(3))) LG_FLAT lg_step_batch_kernel(const LgDesc* __restrict D,
+// them (the member's static word order, NH units per word). Register target
+// WPE waves per SIMD: by default three, which leaves the tile loop room to
+// keep everything in registers (134 VGPRs, no scratch); at four waves it
+// spills 32 bytes a lane. (The single-lattice fused kernel is the other way
+// round: 7% slower at three waves.) Whole-word lanes also come at two and
+// four waves per SIMD, for lg_batch_set_waves and the autotuner.
+template <int TILE, int NH, int WPE>
+__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(WPE))) LG_FLAT lg_step_batch_kernel(const LgDesc* __restrict D,
                                                              const uint2* __restrict tiles,
                                                              uint32_t ntiles, uint32_t k) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -111,11 +112,12 @@ __global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(3))) 
 // collide tile runs units of step k from digit slot kpar + k, a draw tile
 // draws units of step k + 1 into the other slot, both exactly as
 // lg_step2_kernel maps them for a single lattice.
-// Register target as the single-lattice kernel's (about 100 VGPRs); left to
-// itself the compiler unrolls the tile loop into 225 VGPRs for whole-word
-// units, 2 waves per SIMD.
-template <int TILE, int NH, int DNH>
-__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(5))) LG_FLAT
+// Register target WPE waves per SIMD: by default five, as the single-lattice
+// kernel's (about 100 VGPRs); left to itself the compiler unrolls the tile
+// loop into 225 VGPRs for whole-word units, 2 waves per SIMD. Whole-word
+// collide and draw lanes also come at four and six.
+template <int TILE, int NH, int DNH, int WPE>
+__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(WPE))) LG_FLAT
     lg_step2_batch_kernel(const LgDesc* __restrict D, const uint2* __restrict tiles,
                           uint32_t ntiles, uint32_t k) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -217,10 +219,11 @@ struct lg_batch {
   uint32_t units = 1;    // collide (or fused) lanes: 1 = 64 cells, 2 = 32 cells
   uint32_t passes = 2;   // 2: two-pass step, 1: fused step (set by size at creation)
   uint32_t draw_units = 1;  // draw lanes of the two-pass step
+  uint32_t waves = 0;       // step kernel's register target, waves per SIMD (0: default)
   // graph for `chunk` steps with the current shape
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  uint32_t graph_chunk = 0, graph_tile = 0, graph_grid = 0, graph_key = 0;
+  uint32_t graph_chunk = 0, graph_tile = 0, graph_grid = 0, graph_key = 0, graph_waves = 0;
   int ncu = 256;
   // host staging of the last uploads, alive until the stream has consumed them
   std::vector<LgDesc> host_desc;
@@ -355,17 +358,36 @@ int build_tiles(lg_batch* b, DevBuf<uint2>& buf, uint32_t& n, std::vector<uint2>
   return LG_OK;
 }
 
+// The register target (waves per SIMD) the batch's step kernel runs at: the
+// requested one where that pipeline has an instance for it, else the
+// pipeline's default.
+uint32_t effective_waves(const lg_batch* b) {
+  if (b->passes == 1) {
+    if (b->units == 1 && (b->waves == 2 || b->waves == 4)) return b->waves;
+    return 3;
+  }
+  if (b->units == 1 && b->draw_units == 1 && (b->waves == 4 || b->waves == 6)) return b->waves;
+  return 5;
+}
+
 // Enqueue `count` steps (count <= kmax) on the batch stream.
 template <int TILE>
 void launch_steps(lg_batch* b, uint32_t grid, const uint2* tiles, uint32_t ntiles, uint32_t k,
                   bool first) {
   const dim3 gd(grid), bd(TILE);
+  const uint32_t wpe = effective_waves(b);
   if (b->passes == 1) {
-    if (b->units == 1)
-      hipLaunchKernelGGL((lg_step_batch_kernel<TILE, 1>), gd, bd, 0, b->stream, b->desc.p, tiles,
+    if (b->units == 2)
+      hipLaunchKernelGGL((lg_step_batch_kernel<TILE, 2, 3>), gd, bd, 0, b->stream, b->desc.p, tiles,
+                         ntiles, k);
+    else if (wpe == 2)
+      hipLaunchKernelGGL((lg_step_batch_kernel<TILE, 1, 2>), gd, bd, 0, b->stream, b->desc.p, tiles,
+                         ntiles, k);
+    else if (wpe == 4)
+      hipLaunchKernelGGL((lg_step_batch_kernel<TILE, 1, 4>), gd, bd, 0, b->stream, b->desc.p, tiles,
                          ntiles, k);
     else
-      hipLaunchKernelGGL((lg_step_batch_kernel<TILE, 2>), gd, bd, 0, b->stream, b->desc.p, tiles,
+      hipLaunchKernelGGL((lg_step_batch_kernel<TILE, 1, 3>), gd, bd, 0, b->stream, b->desc.p, tiles,
                          ntiles, k);
     return;
   }
@@ -377,18 +399,24 @@ void launch_steps(lg_batch* b, uint32_t grid, const uint2* tiles, uint32_t ntile
       hipLaunchKernelGGL((lg_draw0_batch_kernel<TILE, 2>), gd, bd, 0, b->stream, b->desc.p, tiles,
                          ntiles);
   }
-  if (b->units == 1 && b->draw_units == 1)
-    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 1, 1>), gd, bd, 0, b->stream, b->desc.p, tiles,
-                       ntiles, k);
+  if (b->units == 1 && b->draw_units == 1 && wpe == 4)
+    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 1, 1, 4>), gd, bd, 0, b->stream, b->desc.p,
+                       tiles, ntiles, k);
+  else if (b->units == 1 && b->draw_units == 1 && wpe == 6)
+    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 1, 1, 6>), gd, bd, 0, b->stream, b->desc.p,
+                       tiles, ntiles, k);
+  else if (b->units == 1 && b->draw_units == 1)
+    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 1, 1, 5>), gd, bd, 0, b->stream, b->desc.p,
+                       tiles, ntiles, k);
   else if (b->units == 1)
-    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 1, 2>), gd, bd, 0, b->stream, b->desc.p, tiles,
-                       ntiles, k);
+    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 1, 2, 5>), gd, bd, 0, b->stream, b->desc.p,
+                       tiles, ntiles, k);
   else if (b->draw_units == 1)
-    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 2, 1>), gd, bd, 0, b->stream, b->desc.p, tiles,
-                       ntiles, k);
+    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 2, 1, 5>), gd, bd, 0, b->stream, b->desc.p,
+                       tiles, ntiles, k);
   else
-    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 2, 2>), gd, bd, 0, b->stream, b->desc.p, tiles,
-                       ntiles, k);
+    hipLaunchKernelGGL((lg_step2_batch_kernel<TILE, 2, 2, 5>), gd, bd, 0, b->stream, b->desc.p,
+                       tiles, ntiles, k);
 }
 
 int enqueue_chunk(lg_batch* b, uint32_t count) {
@@ -427,7 +455,8 @@ int ensure_tiles(lg_batch* b) {
 
 int ensure_graph(lg_batch* b) {
   if (b->exec && b->graph_chunk == b->chunk && b->graph_tile == b->tile &&
-      b->graph_grid == b->grid && b->graph_key == pipeline_key(b))
+      b->graph_grid == b->grid && b->graph_key == pipeline_key(b) &&
+      b->graph_waves == effective_waves(b))
     return LG_OK;
   drop_graph(b);
   HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -445,6 +474,7 @@ int ensure_graph(lg_batch* b) {
   b->graph_tile = b->tile;
   b->graph_grid = b->grid;
   b->graph_key = pipeline_key(b);
+  b->graph_waves = effective_waves(b);
   return LG_OK;
 }
 
@@ -772,6 +802,20 @@ int lg_batch_set_shape(lg_batch_t* b, uint32_t tile, uint32_t grid, uint32_t chu
   return LG_OK;
 }
 
+int lg_batch_set_waves(lg_batch_t* b, uint32_t waves_per_simd) {
+  if (!b) return fail(LG_EINVAL, "null batch");
+  if (waves_per_simd != 0 && (waves_per_simd < 2 || waves_per_simd > 6))
+    return fail(LG_EINVAL, "waves per SIMD must be 0 (default) or 2..6");
+  b->waves = waves_per_simd;
+  return LG_OK;
+}
+
+int lg_batch_get_waves(lg_batch_t* b, uint32_t* waves_per_simd) {
+  if (!b || !waves_per_simd) return fail(LG_EINVAL, "null argument");
+  *waves_per_simd = effective_waves(b);
+  return LG_OK;
+}
+
 int lg_batch_get_units(lg_batch_t* b, uint32_t* cells_per_lane) {
   if (!b || !cells_per_lane) return fail(LG_EINVAL, "null argument");
   *cells_per_lane = b->units == 1 ? 64u : 32u;
@@ -795,21 +839,22 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
   int rc = upload_desc(b, nullptr, ~0ull);
   if (rc) return rc;
   struct Cand {
-    uint32_t passes, units, draw_units, tile, grid;
+    uint32_t passes, units, draw_units, tile, grid, waves;
   };
   // Most promising first, so that a budget that runs out keeps the best so
   // far: the pipeline the size rule picks leads (at 256 lattices of 256..2048
   // a side the fused step's best shape, 128-word tiles, runs 958 GCUPS
   // against 920 for 256-word tiles; a single lattice's size favours the
-  // two-pass step).
+  // two-pass step), then its register targets (waves per SIMD), then the
+  // other tile sizes, grids and lane widths.
   const uint32_t g16 = 16u * (uint32_t)b->ncu, g32 = 32u * (uint32_t)b->ncu;
   const Cand two_pass[] = {
-      {2, 1, 1, 256, 0}, {2, 1, 1, 128, 0},   {2, 1, 2, 256, 0},
-      {2, 2, 2, 256, 0}, {2, 1, 1, 256, g16}, {2, 2, 1, 256, 0},
+      {2, 1, 1, 256, 0, 0},   {2, 1, 1, 256, 0, 4}, {2, 1, 1, 256, 0, 6}, {2, 1, 1, 128, 0, 0},
+      {2, 1, 2, 256, 0, 0},   {2, 2, 2, 256, 0, 0}, {2, 1, 1, 256, g16, 0}, {2, 2, 1, 256, 0, 0},
   };
   const Cand fused[] = {
-      {1, 1, 1, 128, 0}, {1, 1, 1, 256, 0}, {1, 1, 1, 256, g32}, {1, 2, 1, 256, 0},
-      {1, 1, 1, 256, g16},
+      {1, 1, 1, 128, 0, 0},   {1, 1, 1, 128, 0, 4}, {1, 1, 1, 128, 0, 2}, {1, 1, 1, 256, 0, 0},
+      {1, 1, 1, 256, g32, 0}, {1, 2, 1, 256, 0, 0}, {1, 1, 1, 256, g16, 0},
   };
   size_t words = 0;
   for (lg_t* g : b->members) words += g->G.total;
@@ -823,24 +868,30 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
   }
   const uint32_t chunk = std::min<uint32_t>(32, b->kmax & ~1u);  // >= 2 (lg_batch_create)
   double best = 1e300;
-  Cand bestc = {b->passes, b->units, b->draw_units, b->tile, b->grid};
+  Cand bestc = {b->passes, b->units, b->draw_units, b->tile, b->grid, b->waves};
   uint64_t used = 0;
+  // one warm replay (it captures the graph), then the best of three timed ones
+  constexpr int kTimed = 3;
   for (const Cand& c : cands) {
-    const uint64_t trial = chunk;  // one warm replay (captures the graph), one timed
-    if (budget_steps && used + 2 * trial > budget_steps) break;
+    const uint64_t trial = chunk;
+    if (budget_steps && used + (1 + kTimed) * trial > budget_steps) break;
     b->passes = c.passes;
     b->units = c.units;
     b->draw_units = c.draw_units;
     b->tile = c.tile;
     b->grid = c.grid;
+    b->waves = c.waves;
     b->chunk = chunk;
     if ((rc = run_steps(b, trial, true))) return abandon_queued(b, rc);
     HIP_TRY(hipStreamSynchronize(b->stream));
-    auto t0 = std::chrono::steady_clock::now();
-    if ((rc = run_steps(b, trial, true))) return abandon_queued(b, rc);
-    HIP_TRY(hipStreamSynchronize(b->stream));
-    double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    used += 2 * trial;
+    double dt = 1e300;
+    for (int r = 0; r < kTimed; ++r) {
+      auto t0 = std::chrono::steady_clock::now();
+      if ((rc = run_steps(b, trial, true))) return abandon_queued(b, rc);
+      HIP_TRY(hipStreamSynchronize(b->stream));
+      dt = std::min(dt, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+    used += (1 + kTimed) * trial;
     if (dt < best) {
       best = dt;
       bestc = c;
@@ -852,6 +903,7 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
   b->draw_units = bestc.draw_units;
   b->tile = bestc.tile;
   b->grid = bestc.grid;
+  b->waves = bestc.waves;
   b->chunk = chunk;
   *used_steps = used;
   return LG_OK;

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1362,6 +1363,61 @@ int lg_set_resident(lg_t* g, uint32_t on) {
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->resident = on;
   return LG_OK;
+}
+
+int lg_get_temporal_depth(lg_t* g, uint32_t* steps, uint32_t* rows_per_tile) {
+  if (!g || !steps || !rows_per_tile) return fail(LG_EINVAL, "null argument");
+  *steps = g->tdepth;
+  *rows_per_tile = g->trows;
+  return LG_OK;
+}
+
+int lg_autotune(lg_t* g, uint64_t budget_steps, uint64_t* used_steps) {
+  if (!g || !used_steps) return fail(LG_EINVAL, "null argument");
+  *used_steps = 0;
+  if (int rc = settle(g)) return rc;
+  struct Cand {
+    uint32_t resident, passes;
+    int ahead;   // -1: by the size rule
+    uint32_t depth;
+  };
+  // the size rule first, so that a budget that runs out keeps it
+  std::vector<Cand> cands = {{2, 0, -1, 1}, {0, 2, 0, 1}, {0, 2, 16, 1}, {0, 1, -1, 1}};
+  uint32_t rows = 0, nb = 0;
+  if (band_eligible(g, &rows, &nb)) cands.insert(cands.begin() + 1, Cand{1, 0, -1, 1});
+  if (g->G.wpr <= 64)
+    for (uint32_t k : {2u, 4u, 8u}) cands.push_back(Cand{0, 2, 16, k});
+  auto apply = [&](const Cand& c) -> int {
+    int rc = lg_set_temporal_depth(g, 1, 0);
+    if (!rc) rc = lg_set_passes(g, c.passes);  // 0 returns the draw unit and draw-ahead to the rule
+    if (!rc && c.ahead >= 0) rc = lg_set_draw_ahead(g, (uint32_t)c.ahead);
+    if (!rc && c.depth > 1) rc = lg_set_temporal_depth(g, c.depth, 0);
+    if (!rc) rc = lg_set_resident(g, c.resident);
+    return rc;
+  };
+  const uint64_t trial = 32;
+  constexpr int kTimed = 3;
+  double best = 1e300;
+  size_t bi = 0;
+  uint64_t used = 0;
+  for (size_t i = 0; i < cands.size(); ++i) {
+    if (budget_steps && used + (1 + kTimed) * trial > budget_steps) break;
+    if (int rc = apply(cands[i])) return rc;
+    if (int rc = lg_advance(g, trial)) return rc;  // warm: streams, buffers, code
+    double dt = 1e300;
+    for (int r = 0; r < kTimed; ++r) {
+      auto t0 = std::chrono::steady_clock::now();
+      if (int rc = lg_advance(g, trial)) return rc;
+      dt = std::min(dt, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+    used += (1 + kTimed) * trial;
+    if (dt < best) {
+      best = dt;
+      bi = i;
+    }
+  }
+  *used_steps = used;
+  return apply(cands[bi]);
 }
 
 int lg_get_resident(lg_t* g, uint32_t* rows_per_band, uint32_t* bands) {

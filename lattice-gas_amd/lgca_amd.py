@@ -69,9 +69,13 @@ def load_library(path: str | None = None):
         "lg_set_temporal_depth": (i32, [vp, C.c_uint32, C.c_uint32]),
         "lg_set_resident": (i32, [vp, C.c_uint32]),
         "lg_get_resident": (i32, [vp, P(C.c_uint32), P(C.c_uint32)]),
+        "lg_get_temporal_depth": (i32, [vp, P(C.c_uint32), P(C.c_uint32)]),
+        "lg_autotune": (i32, [vp, u64, P(u64)]),
         "lg_get_pipeline": (i32, [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "lg_batch_set_units": (i32, [vp, C.c_uint32]),
         "lg_batch_get_units": (i32, [vp, P(C.c_uint32)]),
+        "lg_batch_set_waves": (i32, [vp, C.c_uint32]),
+        "lg_batch_get_waves": (i32, [vp, P(C.c_uint32)]),
         "lg_batch_set_passes": (i32, [vp, C.c_uint32]),
         "lg_batch_set_draw_unit": (i32, [vp, C.c_uint32]),
         "lg_batch_get_pipeline": (i32, [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
@@ -291,6 +295,19 @@ class GpuLattice:
         on = _check(load_library().lg_get_resident(self._p, C.byref(r), C.byref(n)))
         return (r.value, n.value) if on else None
 
+    def temporal_depth(self):
+        """(steps per collide launch, valid rows per tile; 0 = by size)."""
+        k, r = C.c_uint32(), C.c_uint32()
+        _check(load_library().lg_get_temporal_depth(self._p, C.byref(k), C.byref(r)))
+        return k.value, r.value
+
+    def autotune(self, budget_steps: int = 0) -> int:
+        """Time the step pipelines on this lattice and keep the fastest
+        (lg_autotune); returns the steps taken (real steps of the lattice)."""
+        used = C.c_uint64()
+        _check(load_library().lg_autotune(self._p, int(budget_steps), C.byref(used)))
+        return int(used.value)
+
     def advance_timed(self, n: int, sample_every: int = 0, alpha: float = 0.0) -> float:
         """Advance and return the device milliseconds of the step kernels,
         from HIP event pairs on the engine's stream around each run of step
@@ -447,6 +464,15 @@ class Batch:
         u = C.c_uint32()
         _check(load_library().lg_batch_get_units(self._p, C.byref(u)))
         return u.value
+
+    def set_waves(self, waves_per_simd: int):
+        """Register target of the step kernel (waves per SIMD; 0 = default)."""
+        _check(load_library().lg_batch_set_waves(self._p, int(waves_per_simd)))
+
+    def waves(self) -> int:
+        w = C.c_uint32()
+        _check(load_library().lg_batch_get_waves(self._p, C.byref(w)))
+        return w.value
 
     def set_passes(self, passes: int):
         _check(load_library().lg_batch_set_passes(self._p, int(passes)))

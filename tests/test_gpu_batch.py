@@ -58,6 +58,29 @@ def test_batch_equals_solo_runs(shape, pipeline):
         assert np.array_equal(g.store(), s.store())
 
 
+@pytest.mark.parametrize("passes,waves", [(1, 2), (1, 3), (1, 4), (2, 4), (2, 5), (2, 6), (1, 6)])
+def test_register_targets_keep_results_exact(passes, waves):
+    """Every waves-per-SIMD instance of the batch step kernels (and a target
+    the pipeline has no instance for, which runs its default) gives the
+    members' own runs."""
+    print(f"batch waves passes={passes} waves={waves}", flush=True)
+    solo = build_all()
+    for g in solo:
+        g.advance(21)
+    batched = build_all()
+    b = Batch(batched)
+    b.set_passes(passes)
+    b.set_units(64)
+    b.set_draw_unit(64)
+    b.set_waves(waves)
+    default = 3 if passes == 1 else 5
+    assert b.waves() == (waves if (passes, waves) in ((1, 2), (1, 4), (2, 4), (2, 6)) else default)
+    b.set_shape(128, 0, 8)
+    b.advance(21)
+    for s, g in zip(solo, batched):
+        assert np.array_equal(g.store(), s.store())
+
+
 def test_graph_replays_with_eager_work_between():
     """Each replay checked against eager stepping, with the members' own
     eager calls (store, solo advance) between replays."""

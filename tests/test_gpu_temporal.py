@@ -106,3 +106,31 @@ def test_plate_workload_9m_cells(oracle, depth):
         g.advance(20)
         assert np.array_equal(g.store(), lat.cells)
     g.close()
+
+
+@pytest.mark.parametrize("w,h,bands,inlet", [(4096, 2304, 16, 8), (320, 600, 16, 8), (1000, 26, 2, 70)])
+def test_autotune_keeps_results_exact(oracle, w, h, bands, inlet):
+    """lg_autotune tries every pipeline this lattice can run (resident bands,
+    one-launch two-pass, draw-ahead, fused, temporally blocked collides of
+    depth 2, 4 and 8) on the lattice itself; its steps and the steps after
+    it, in the pipeline it keeps, equal the CPU path's."""
+    seed = 0xA70 + w
+    print(f"autotune {w}x{h} bands={bands} inlet={inlet}", flush=True)
+    lat = oracle.Lattice(w, h, 0.22, (0.3, 0.0), inlet, True, bands, seed)
+    yc = float(np.float32(np.float32(h) * oracle.SQRT3_2 / np.float32(2)))
+    lat.add_solid_disc(w / 3.0, yc, 3.5)
+    lat.init_equilibrium()
+    lat.set_parallel(True)
+    g = GpuLattice(w, h, True, seed, bands)
+    g.set_inlet(inlet, 0.22, (0.3, 0.0))
+    g.load(lat.cells)
+    used = g.autotune(0)
+    k, _ = g.temporal_depth()
+    print(f"  used {used} steps, pipeline {g.pipeline()}, resident {g.resident()}, depth {k}",
+          flush=True)
+    assert used > 0 and used % 32 == 0
+    g.advance(17)
+    lat.advance(used + 17)
+    assert g.steps == lat.steps
+    assert np.array_equal(g.store(), lat.cells)
+    g.close()
