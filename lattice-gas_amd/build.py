@@ -18,6 +18,8 @@ HIPCC_FLAGS = [
     # packed fp32 ops are kept out of every HIP build on these boxes
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
+    # resolve every symbol at load time, not at a call's first use
+    "-Wl,-z,now",
 ]
 
 
