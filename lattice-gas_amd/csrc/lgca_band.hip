@@ -654,8 +654,14 @@ int launch_band(lg_t* g, uint32_t n, bool sample, float alpha) {
   }
   const hipError_t e = hipLaunchCooperativeKernel(
       kernels[g->G.wpr == 64][sample][g->G.inlet_cols > 64], grid, block, args, lds, g->stream);
-  if (e != hipSuccess)
-    return fail(LG_EDEVICE, std::string("resident-band launch: ") + hipGetErrorString(e));
+  if (e != hipSuccess) {
+    // Refused (the runtime can decline a cooperative launch, e.g. while
+    // another process holds the device's cooperative-launch resources):
+    // nothing ran; this lattice takes the other pipelines from here on.
+    (void)hipGetLastError();
+    g->band_refused = true;
+    return kBandRefused;
+  }
   g->band_check = true;
   return LG_OK;
 }

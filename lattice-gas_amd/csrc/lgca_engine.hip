@@ -849,7 +849,9 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
   // caller asked for resident bands
   const uint64_t first = std::min<uint64_t>(n, every > 0 ? every : n);
   if (resident_active(g, &brows, &bnb) && (g->resident == 1 || first >= LG_BAND_MIN_STEPS)) {
-    // resident bands: one launch per chunk (it makes its own stream tables)
+    // resident bands: one launch per chunk (it makes its own stream tables);
+    // if the runtime declines the launch, the rest of the call takes the
+    // other pipelines below
     while (left > 0) {
       uint64_t chunk = std::min<uint64_t>(left, LG_BAND_MAX_CHUNK);
       if (every > 0) chunk = std::min<uint64_t>(chunk, every - since);
@@ -865,7 +867,9 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
       // same launch where the field's blocks fit
       const bool sample_after = every > 0 && (since + chunk >= every || left == chunk);
       const bool fused = sample_after && band_sample_fits(g);
-      if (int rc = launch_band(g, (uint32_t)chunk, fused, alpha)) return rc;
+      const int brc = launch_band(g, (uint32_t)chunk, fused, alpha);
+      if (brc == kBandRefused) break;
+      if (brc) return brc;
       if (timed) HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used++ + 1], g->stream));
       std::swap(g->a.p, g->b.p);
       g->steps += chunk;
@@ -879,7 +883,7 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
         }
       }
     }
-    return LG_OK;
+    if (left == 0) return LG_OK;
   }
   while (left > 0) {
     const uint32_t extra = (passes == 2 && !ahead) ? 1u : 0u;  // see the two-pass branch
@@ -968,14 +972,15 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
 }  // namespace
 
 bool lgca_internal::resident_active(const lg_t* g, uint32_t* rows, uint32_t* nb) {
-  if (g->resident == 0) return false;
-  // by the size rule: unless the caller chose another pipeline, and only
-  // for rows that fill most of a wave (the reference's 2048 x 1280 lattice,
-  // 32 words per row, runs 0.86 ms per 100 steps resident against 0.67)
-  if (g->resident == 2 &&
-      (g->passes != 0 || g->tdepth >= 2 || !g->auto_ahead || g->G.wpr < LG_BAND_MIN_WORDS))
-    return false;
-  return band_eligible(g, rows, nb);
+  if (g->resident == 0 || g->band_refused) return false;
+  if (!band_eligible(g, rows, nb)) return false;
+  if (g->resident == 1) return true;
+  // by the size rule: unless the caller chose another pipeline, and where
+  // the bands' step beats the two-pass step (see LG_BAND_WORDS_PER_ROW)
+  if (g->passes != 0 || g->tdepth >= 2 || !g->auto_ahead) return false;
+  const int64_t need = (int64_t)LG_BAND_WORDS_PER_ROW * *rows - LG_BAND_WORDS_BASE -
+                       (g->G.inlet_cols > 0 ? LG_BAND_INFLOW_WORDS : 0);
+  return (int64_t)g->G.total >= need;
 }
 
 uint32_t lgca_internal::step_passes(const lg_t* g) {

@@ -82,10 +82,20 @@ void resolve_pipeline(lg_lattice* g);
 // Resident bands: rows per band at most (one wave per row), and the polls a
 // neighbour wait makes before it gives up (each sleeps ~64 clocks: ~0.1 s).
 #define LG_BAND_MAX_ROWS 11u
-// The size rule's resident bands: rows of at least this many words, calls
-// whose first chunk has at least this many steps.
-#define LG_BAND_MIN_WORDS 48u
+// The size rule's resident bands: calls whose first chunk has at least
+// LG_BAND_MIN_STEPS steps, on lattices where a step of B-row bands (about
+// 3.6 + 0.72 B us, whatever the row width) beats the two-pass step (about
+// 5.2 us + 0.06 us per 1000 words, 1.5 us more with inflow): at least
+// LG_BAND_WORDS_PER_ROW x B - LG_BAND_WORDS_BASE words per plane, less
+// LG_BAND_INFLOW_WORDS with inflow. (Measured, 100 steps per call: 2048 x
+// 1280 6.5 against 6.9 us, with inflow 6.9 against 8.9; 1024 x 1024 with
+// inflow 6.7 against 7.7; 256 x 256 4.3 against 5.5; 1024 x 2304 with
+// inflow, 9-row bands of 16 words, 9.8 against 9.0; 512 x 1280 6.4 against
+// 5.7.)
 #define LG_BAND_MIN_STEPS 4u
+#define LG_BAND_WORDS_PER_ROW 12000
+#define LG_BAND_WORDS_BASE 26700
+#define LG_BAND_INFLOW_WORDS 25000
 #define LG_BAND_SPIN_LIMIT (1u << 20)
 // Steps per resident launch at most (its step counters run below 2^21).
 #define LG_BAND_MAX_CHUNK (1u << 16)
@@ -148,6 +158,9 @@ bool resident_active(const lg_lattice* g, uint32_t* rows, uint32_t* nb);
 bool band_sample_fits(const lg_lattice* g);
 // n steps; with `sample`, the coarse-graining of the final state with
 // weight alpha (what sample_now would do next) is done by the same launch.
+// Returns kBandRefused (and nothing ran) when the runtime declined the
+// cooperative launch; the lattice then runs the other pipelines.
+constexpr int kBandRefused = 1;
 int launch_band(lg_lattice* g, uint32_t n, bool sample = false, float alpha = 0.0f);
 int check_band(lg_lattice* g);
 
@@ -212,6 +225,7 @@ struct lg_lattice {
   uint64_t band_epoch = 0, band_epoch_checked = 0;
   bool band_check = false;
   bool band_lds_raised = false;
+  bool band_refused = false;  // the runtime declined a cooperative launch
   DevBuf<int32_t> band_part;   // per-band block sums of the fused coarse-graining
   DevBuf<uint32_t> band_bcnt;  // bands arrived per block row
   DevBuf<lgca_internal::LgBandField> band_field;  // what the launch reads of the field

@@ -62,7 +62,16 @@ int main() {
     bad = 1;
   }
   // Sim::advance, then Sim::sample (src/sim.rs:148-165)
+  uint32_t rows = 0, nbands = 0;
+  const int resident_before = lg_get_resident(g, &rows, &nbands);
   bad |= check(lg_advance(g, steps), "lg_advance");
+  // the engine may run resident bands; if the runtime declines their
+  // cooperative launch (another process holding the device's cooperative
+  // launch resources), the same steps run in the other pipelines
+  std::printf("harness: resident bands %s\n",
+              resident_before != 1                    ? "not used"
+              : lg_get_resident(g, &rows, &nbands) == 1 ? "used"
+                                                        : "declined by the runtime, other pipeline used");
   if (lg_steps(g) != steps) bad = 1;
   bad |= check(lg_sample_field(g, 1.0f), "lg_sample_field");
   size_t bw = 0, bh = 0, bx = 0, by = 0;

@@ -28,7 +28,7 @@ def test_ensemble_members_equal_their_solo_runs(oracle):
     print(f"ensemble: {len(cfgs)} members, {sum(c['w'] * c['h'] for c in cfgs)} cells", flush=True)
     members = [W.config_gpu(c) for c in cfgs]
     b = Batch(members)
-    used = b.autotune(budget_steps=96)
+    used = b.autotune(budget_steps=256)  # two candidates: 32 warm + 3 x 32 timed steps each
     assert used > 0
     tile, grid, chunk = b.shape()
     n = 2 * chunk + 7          # two graph replays and a remainder
