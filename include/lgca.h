@@ -189,11 +189,13 @@ int lg_set_temporal_depth(lg_t* g, uint32_t steps, uint32_t rows_per_tile);
  * step counters, and a further wave builds each step's stream tables in LDS.
  * For rows of at most 64 words (4096 cells) and at most 11 rows per band at
  * one band per CU; other lattices keep the other pipelines. The size rule
- * (2) takes it for calls whose first chunk has at least 4 steps, on
- * lattices where bands of B rows have enough words per plane to beat the
- * two-pass step (12000 B - 26700 words, 25000 fewer with inflow: measured
- * step costs). Results are identical. lg_get_resident returns 1 (and the band
- * height and count) when the next advance runs resident, else 0. A wait
+ * (2) takes it where, by measured step costs, bands of B rows step faster
+ * than the two-pass step, for calls whose first chunk (the call, or up to
+ * its first sample) saves more step time than the band launch costs (about
+ * 12 us). Results are identical. lg_get_resident returns 1 (and the band
+ * height and count) when resident bands are in use for calls of enough
+ * steps, else 0. If the runtime declines the cooperative launch, the lattice
+ * runs the other pipelines from then on. A wait
  * between bands that runs out (it cannot hang the device: every wait is
  * bounded) makes the call fail with LG_EDEVICE and the lattice must be
  * reloaded. */

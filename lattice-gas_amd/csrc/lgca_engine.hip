@@ -843,12 +843,12 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
     HIP_TRY(hipStreamWaitEvent(g->dstream, g->ev_start, 0));
   }
   uint32_t brows = 0, bnb = 0;
-  // a call whose first chunk is shorter than a few steps pays the band
-  // launch's start more than it gains (one step per call at 2048 x 1280:
-  // 50 us resident against 25): it takes the other pipelines unless the
-  // caller asked for resident bands
+  // a call whose first chunk is too short to repay the band launch (one
+  // step per call at 2048 x 1280: 50 us resident against 25) takes the
+  // other pipelines unless the caller asked for resident bands
   const uint64_t first = std::min<uint64_t>(n, every > 0 ? every : n);
-  if (resident_active(g, &brows, &bnb) && (g->resident == 1 || first >= LG_BAND_MIN_STEPS)) {
+  if (resident_active(g, &brows, &bnb) &&
+      (g->resident == 1 || (double)first * band_gain_us(g, brows) >= LG_BAND_LAUNCH_US)) {
     // resident bands: one launch per chunk (it makes its own stream tables);
     // if the runtime declines the launch, the rest of the call takes the
     // other pipelines below
@@ -976,11 +976,16 @@ bool lgca_internal::resident_active(const lg_t* g, uint32_t* rows, uint32_t* nb)
   if (!band_eligible(g, rows, nb)) return false;
   if (g->resident == 1) return true;
   // by the size rule: unless the caller chose another pipeline, and where
-  // the bands' step beats the two-pass step (see LG_BAND_WORDS_PER_ROW)
+  // the bands' step beats the two-pass step (calls also need a first chunk
+  // long enough to repay the band launch: enqueue_steps)
   if (g->passes != 0 || g->tdepth >= 2 || !g->auto_ahead) return false;
-  const int64_t need = (int64_t)LG_BAND_WORDS_PER_ROW * *rows - LG_BAND_WORDS_BASE -
-                       (g->G.inlet_cols > 0 ? LG_BAND_INFLOW_WORDS : 0);
-  return (int64_t)g->G.total >= need;
+  return band_gain_us(g, *rows) > 0.0;
+}
+
+double lgca_internal::band_gain_us(const lg_t* g, uint32_t rows) {
+  const double resident = 3.6 + 0.72 * rows;
+  const double two_pass = 5.2 + 0.06e-3 * (double)g->G.total + (g->G.inlet_cols > 0 ? 1.5 : 0.0);
+  return two_pass - resident;
 }
 
 uint32_t lgca_internal::step_passes(const lg_t* g) {

@@ -82,20 +82,21 @@ void resolve_pipeline(lg_lattice* g);
 // Resident bands: rows per band at most (one wave per row), and the polls a
 // neighbour wait makes before it gives up (each sleeps ~64 clocks: ~0.1 s).
 #define LG_BAND_MAX_ROWS 11u
-// The size rule's resident bands: calls whose first chunk has at least
-// LG_BAND_MIN_STEPS steps, on lattices where a step of B-row bands (about
-// 3.6 + 0.72 B us, whatever the row width) beats the two-pass step (about
-// 5.2 us + 0.06 us per 1000 words, 1.5 us more with inflow): at least
-// LG_BAND_WORDS_PER_ROW x B - LG_BAND_WORDS_BASE words per plane, less
-// LG_BAND_INFLOW_WORDS with inflow. (Measured, 100 steps per call: 2048 x
-// 1280 6.5 against 6.9 us, with inflow 6.9 against 8.9; 1024 x 1024 with
-// inflow 6.7 against 7.7; 256 x 256 4.3 against 5.5; 1024 x 2304 with
-// inflow, 9-row bands of 16 words, 9.8 against 9.0; 512 x 1280 6.4 against
-// 5.7.)
-#define LG_BAND_MIN_STEPS 4u
-#define LG_BAND_WORDS_PER_ROW 12000
-#define LG_BAND_WORDS_BASE 26700
-#define LG_BAND_INFLOW_WORDS 25000
+// The size rule's resident bands, from measured step costs: a step of B-row
+// bands costs about 3.6 + 0.72 B us whatever the row width, the two-pass
+// step about 5.2 us + 0.06 us per 1000 words, 1.5 us more with inflow, and
+// a band launch about LG_BAND_LAUNCH_US more than the other pipelines'
+// launches of the same steps (its first step's tables and draws are not
+// overlapped: a 5-step launch at 4096 x 2304 takes 71.6 us, 10.3 us per step
+// in longer ones; the two-pass step's call has its draw-only launch); a call takes resident bands when its first
+// chunk's steps save more than that (band_gain_us). (Measured, 100 steps
+// per call: 2048 x 1280 6.5 against 6.9 us, with inflow 6.9 against 8.9;
+// 1024 x 1024 with inflow 6.7 against 7.7; 256 x 256 4.3 against 5.5;
+// 4096 x 2304 with inflow 10.3 against 14.3; 1024 x 2304 with inflow,
+// 9-row bands of 16 words, 9.8 against 9.0; 512 x 1280 6.4 against 5.7.)
+#define LG_BAND_LAUNCH_US 12.0
+// Step time saved by resident bands of `rows` rows (us; negative: lost).
+double band_gain_us(const lg_lattice* g, uint32_t rows);
 #define LG_BAND_SPIN_LIMIT (1u << 20)
 // Steps per resident launch at most (its step counters run below 2^21).
 #define LG_BAND_MAX_CHUNK (1u << 16)
