@@ -168,3 +168,32 @@ def test_coarse_graining_in_the_band_launch(oracle, w, h, bands, bx, by):
         for a, b in zip(g.read_block_sums(), f.block_sums(lat)):
             assert np.array_equal(a, b), f"block sums after {lat.steps} steps"
     g.close()
+
+
+def test_concurrent_lattices_on_their_own_streams(oracle):
+    """Four lattices whose advances are queued at once on their own streams,
+    each with resident-band launches of its own: every one equals its CPU
+    replay (the runtime may run the cooperative launches one after another,
+    or decline one, in which case that lattice takes another pipeline)."""
+    say("concurrent resident lattices")
+    shapes = [(4096, 2304, 16), (2048, 1280, 8), (256, 256, 4), (1000, 26, 2)]
+    runs = []
+    for w, h, bands in shapes:
+        seed = 0xC0C0 + w
+        lat = oracle.Lattice(w, h, 0.22, (0.3, 0.0), 8, True, bands, seed)
+        lat.init_equilibrium()
+        lat.set_parallel(True)
+        g = GpuLattice(w, h, True, seed, bands)
+        g.set_inlet(8, 0.22, (0.3, 0.0))
+        g.load(lat.cells)
+        g.set_resident(True)
+        runs.append((lat, g))
+    for rep in range(3):
+        for _, g in runs:
+            g.advance_async(37)
+        for lat, g in runs:
+            lat.advance(37)
+            g.wait()
+            assert np.array_equal(g.store(), lat.cells), f"{lat.w}x{lat.h} after {lat.steps} steps"
+    for _, g in runs:
+        g.close()
