@@ -195,7 +195,8 @@ int lg_set_temporal_depth(lg_t* g, uint32_t steps, uint32_t rows_per_tile);
  * 12 us). Results are identical. lg_get_resident returns 1 (and the band
  * height and count) when resident bands are in use for calls of enough
  * steps, else 0. If the runtime declines the cooperative launch, the lattice
- * runs the other pipelines from then on. A wait
+ * runs the other pipelines from then on (until lg_set_resident is called
+ * again). A wait
  * between bands that runs out (it cannot hang the device: every wait is
  * bounded) makes the call fail with LG_EDEVICE and the lattice must be
  * reloaded. */

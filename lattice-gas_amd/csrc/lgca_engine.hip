@@ -1372,6 +1372,7 @@ int lg_set_resident(lg_t* g, uint32_t on) {
   if (on > 2) return fail(LG_EINVAL, "resident must be 0 (off), 1 (on) or 2 (by the size rule)");
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->resident = on;
+  g->band_refused = false;  // a declined cooperative launch is tried again
   return LG_OK;
 }
 
