@@ -197,3 +197,44 @@ def test_concurrent_lattices_on_their_own_streams(oracle):
             assert np.array_equal(g.store(), lat.cells), f"{lat.w}x{lat.h} after {lat.steps} steps"
     for _, g in runs:
         g.close()
+
+
+def test_field_reattached_and_restart_between_resident_calls(oracle):
+    """Resident launches that coarse-grain in the launch across a field
+    re-attached with other blocks (more block rows: the arrival counts and
+    the block sums are reallocated) and across a restart with new cells and
+    seed: cells and field bits equal the oracle's throughout."""
+    w, h, bands = 512, 300, 6
+    say(f"resident reattach/restart {w}x{h}")
+    lat = oracle.Lattice(w, h, 0.22, (0.3, 0.0), 8, True, bands, 0xFEED)
+    lat.init_equilibrium()
+    g = GpuLattice(w, h, True, 0xFEED, bands)
+    g.set_inlet(8, 0.22, (0.3, 0.0))
+    g.load(lat.cells)
+    g.set_resident(True)
+
+    def check(f):
+        assert np.array_equal(g.store(), lat.cells), f"cells after {lat.steps} steps"
+        for a, b in zip(g.read_field(), f.read()):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f"field after {lat.steps}"
+
+    for bx, by in ((16, 16), (8, 4), (40, 30)):
+        g.attach_field(bx, by)
+        f = oracle.Field(w, h, bx, by)
+        for n, every in ((30, 10), (17, 40)):
+            replay_sampling(lat, f, n, every, 0.5)
+            g.advance_sampling(n, every, 0.5)
+            check(f)
+    # restart: new cells (the oracle's after more steps) and a new seed
+    lat2 = oracle.Lattice(w, h, 0.22, (0.3, 0.0), 8, True, bands, 0xBEEF)
+    lat2.init_equilibrium()
+    lat2.advance(7)
+    cells = lat2.cells.copy()
+    lat.restart(cells, 0xABC)
+    g.restart(cells, 0xABC)
+    f = oracle.Field(w, h, 40, 30)
+    g.attach_field(40, 30)
+    replay_sampling(lat, f, 25, 10, 0.5)
+    g.advance_sampling(25, 10, 0.5)
+    check(f)
+    g.close()
