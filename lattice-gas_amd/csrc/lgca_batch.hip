@@ -84,11 +84,11 @@ __global__ void __launch_bounds__(256) lg_tables_batch_kernel(const LgDesc* __re
 // Step k of the chunk for every tile. A tile is TILE consecutive thread
 // indices of one member, mapped to work units exactly as lg_step_kernel_{w,h} map
 // them (the member's static word order, NH units per word). Register target
-// WPE waves per SIMD: by default three, which leaves the tile loop room to
-// keep everything in registers (134 VGPRs, no scratch); at four waves it
-// spills 32 bytes a lane. (The single-lattice fused kernel is the other way
-// round: 7% slower at three waves.) Whole-word lanes also come at two and
-// four waves per SIMD, for lg_batch_set_waves and the autotuner.
+// WPE waves per SIMD: three leaves the tile loop room to keep everything in
+// registers (134 VGPRs, no scratch); four spills 32 bytes a lane but hides
+// more of the dependency waits and is the default for whole-word lanes
+// (effective_waves); two and three are there for lg_batch_set_waves and the
+// autotuner.
 template <int TILE, int NH, int WPE>
 __global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(WPE))) LG_FLAT lg_step_batch_kernel(const LgDesc* __restrict D,
                                                              const uint2* __restrict tiles,
@@ -363,7 +363,10 @@ int build_tiles(lg_batch* b, DevBuf<uint2>& buf, uint32_t& n, std::vector<uint2>
 // pipeline's default.
 uint32_t effective_waves(const lg_batch* b) {
   if (b->passes == 1) {
-    if (b->units == 1 && (b->waves == 2 || b->waves == 4)) return b->waves;
+    // whole-word lanes: four by default (measured on the C4 ensemble mix,
+    // GCUPS at three / four waves per SIMD: 16 members 711 / 734, 36 members
+    // 876 / 910, 64 members 951 / 952, all 256 members 951 / 1020)
+    if (b->units == 1) return (b->waves >= 2 && b->waves <= 4) ? b->waves : 4;
     return 3;
   }
   if (b->units == 1 && b->draw_units == 1 && (b->waves == 4 || b->waves == 6)) return b->waves;
@@ -853,7 +856,7 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
       {2, 1, 2, 256, 0, 0},   {2, 2, 2, 256, 0, 0}, {2, 1, 1, 256, g16, 0}, {2, 2, 1, 256, 0, 0},
   };
   const Cand fused[] = {
-      {1, 1, 1, 128, 0, 0},   {1, 1, 1, 128, 0, 4}, {1, 1, 1, 128, 0, 2}, {1, 1, 1, 256, 0, 0},
+      {1, 1, 1, 128, 0, 0},   {1, 1, 1, 128, 0, 3}, {1, 1, 1, 128, 0, 2}, {1, 1, 1, 256, 0, 0},
       {1, 1, 1, 256, g32, 0}, {1, 2, 1, 256, 0, 0}, {1, 1, 1, 256, g16, 0},
   };
   size_t words = 0;
