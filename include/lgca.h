@@ -317,11 +317,10 @@ size_t lg_batch_snapshot_noalloc_max(void);
 int lg_batch_set_shape(lg_batch_t* b, uint32_t tile, uint32_t grid, uint32_t chunk);
 int lg_batch_get_shape(lg_batch_t* b, uint32_t* tile, uint32_t* grid, uint32_t* chunk);
 /* New: register target of the batch's step kernel, in waves per SIMD (0: the
- * pipeline's default: 4 for the fused step with 64-cell lanes, 3 with 32-cell
- * lanes, 5 for the two-pass step). Whole-word lanes also run at 2 or 3
- * (fused) and 4 or 6 (two-pass); other values (2..6) fall back to the
- * default, and lg_batch_get_waves reports the target actually used. Results
- * are identical for every choice. */
+ * pipeline's default, 3 for the fused step and 5 for the two-pass step).
+ * Whole-word lanes also run at 2 or 4 (fused) and 4 or 6 (two-pass); other
+ * values (2..6) fall back to the default, and lg_batch_get_waves reports the
+ * target actually used. Results are identical for every choice. */
 int lg_batch_set_waves(lg_batch_t* b, uint32_t waves_per_simd);
 int lg_batch_get_waves(lg_batch_t* b, uint32_t* waves_per_simd);
 /* Cells per lane (64 or 32) for the batch's step kernel, as lg_set_work_unit. */

@@ -84,11 +84,10 @@ __global__ void __launch_bounds__(256) lg_tables_batch_kernel(const LgDesc* __re
 // Step k of the chunk for every tile. A tile is TILE consecutive thread
 // indices of one member, mapped to work units exactly as lg_step_kernel_{w,h} map
 // them (the member's static word order, NH units per word). Register target
-// WPE waves per SIMD: three leaves the tile loop room to keep everything in
-// registers (134 VGPRs, no scratch); four spills 32 bytes a lane but hides
-// more of the dependency waits and is the default for whole-word lanes
-// (effective_waves); two and three are there for lg_batch_set_waves and the
-// autotuner.
+// WPE waves per SIMD: three, the default, leaves the tile loop room to keep
+// everything in registers (134 VGPRs, no scratch); four spills 32 bytes a
+// lane but hides more of the dependency waits, which pays on some mixes;
+// two and four are there for lg_batch_set_waves and the autotuner.
 template <int TILE, int NH, int WPE>
 __global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(WPE))) LG_FLAT lg_step_batch_kernel(const LgDesc* __restrict D,
                                                              const uint2* __restrict tiles,
@@ -363,10 +362,12 @@ int build_tiles(lg_batch* b, DevBuf<uint2>& buf, uint32_t& n, std::vector<uint2>
 // pipeline's default.
 uint32_t effective_waves(const lg_batch* b) {
   if (b->passes == 1) {
-    // whole-word lanes: four by default (measured on the C4 ensemble mix,
-    // GCUPS at three / four waves per SIMD: 16 members 711 / 734, 36 members
-    // 876 / 910, 64 members 951 / 952, all 256 members 951 / 1020)
-    if (b->units == 1) return (b->waves >= 2 && b->waves <= 4) ? b->waves : 4;
+    // whole-word lanes: three by default. Which is faster depends on the
+    // mix (GCUPS at three / four waves per SIMD: the 256-lattice ensemble
+    // 951 / 1020, 36 of its members 876 / 910; batches of 10-40 served
+    // requests, up to 4096 x 2304 with inflow, 692 / 683 to 962 / 929): the
+    // autotuner decides per batch.
+    if (b->units == 1) return (b->waves >= 2 && b->waves <= 4) ? b->waves : 3;
     return 3;
   }
   if (b->units == 1 && b->draw_units == 1 && (b->waves == 4 || b->waves == 6)) return b->waves;
@@ -856,7 +857,7 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
       {2, 1, 2, 256, 0, 0},   {2, 2, 2, 256, 0, 0}, {2, 1, 1, 256, g16, 0}, {2, 2, 1, 256, 0, 0},
   };
   const Cand fused[] = {
-      {1, 1, 1, 128, 0, 0},   {1, 1, 1, 128, 0, 3}, {1, 1, 1, 128, 0, 2}, {1, 1, 1, 256, 0, 0},
+      {1, 1, 1, 128, 0, 0},   {1, 1, 1, 128, 0, 4}, {1, 1, 1, 128, 0, 2}, {1, 1, 1, 256, 0, 0},
       {1, 1, 1, 256, g32, 0}, {1, 2, 1, 256, 0, 0}, {1, 1, 1, 256, g16, 0},
   };
   size_t words = 0;

@@ -73,7 +73,7 @@ def test_register_targets_keep_results_exact(passes, waves):
     b.set_units(64)
     b.set_draw_unit(64)
     b.set_waves(waves)
-    default = 4 if passes == 1 else 5
+    default = 3 if passes == 1 else 5
     assert b.waves() == (waves if (passes, waves) in ((1, 2), (1, 3), (1, 4), (2, 4), (2, 6))
                          else default)
     b.set_shape(128, 0, 8)

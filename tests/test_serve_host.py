@@ -5,6 +5,7 @@ import threading
 import time
 
 import numpy as np
+import pytest
 
 import serve
 
@@ -34,12 +35,31 @@ def test_footprint_counts_planes_tables_and_field():
 
 
 class _Lattice:
-    """Stand-in for a resident lattice: counts its own steps."""
+    """Stand-in for a resident lattice: counts its own steps. With `bands`
+    set it says it runs resident bands, so that a run with few others on the
+    device advances on its own (advance_async / wait / read_field)."""
     live = {}
+    bands = False
+    alone_calls = 0
 
     def __init__(self, r):
-        self.r, self.steps, self.closed = r, 0, False
+        self.r, self.steps, self.closed, self.queued = r, 0, False, 0
         _Lattice.live[r.rid] = self
+
+    def resident(self):
+        return (1, 1) if _Lattice.bands else None
+
+    def advance_async(self, n, every, alpha):
+        assert not self.closed and 0 < n <= every and alpha == 1.0
+        self.queued = n
+        _Lattice.alone_calls += 1
+
+    def wait(self):
+        self.steps += self.queued
+        self.queued = 0
+
+    def read_field(self):
+        return (np.full(1, self.steps, np.float32),) * 4
 
     def close(self):
         assert not self.closed, "a lattice freed twice"
@@ -83,7 +103,8 @@ class _Batch:
         self.closed = True
 
 
-def test_batched_scheduler_snapshots_budget_and_frees(monkeypatch):
+@pytest.mark.parametrize("bands", [False, True])
+def test_batched_scheduler_snapshots_budget_and_frees(monkeypatch, bands):
     """The continuous-batching loop with stand-ins for the device objects:
     every run's snapshots fall at its own multiples of `every` and its last
     step, every lattice is freed exactly once and never while its batch
@@ -103,6 +124,8 @@ def test_batched_scheduler_snapshots_budget_and_frees(monkeypatch):
             high[0] = max(high[0], live)
         return _Lattice.live[r.rid]
 
+    monkeypatch.setattr(_Lattice, "bands", bands)
+    monkeypatch.setattr(_Lattice, "alone_calls", 0)
     monkeypatch.setattr(serve, "build", build)
     monkeypatch.setattr(serve, "Batch", _Batch)
     monkeypatch.setattr(serve, "footprint", lambda r, block: size[r.rid])
@@ -115,3 +138,6 @@ def test_batched_scheduler_snapshots_budget_and_frees(monkeypatch):
         assert all(int(snap[1][0]) == snap[0] for snap in res[r.rid].snapshots)
         assert res[r.rid].done >= res[r.rid].start >= 0.0
     assert s["requests"] == len(trace) and s["peak_resident"] >= 2
+    # with resident bands, runs with at most one other on the device advance
+    # on their own (the start and the tail of the stream here); without, never
+    assert (_Lattice.alone_calls > 0) == bands
