@@ -619,9 +619,9 @@ def main():
     ap.add_argument("--serve", type=int, default=256, metavar="N",
                     help="requests in the steady-state serving leg (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=22.0,
-                    help="arrivals per second of the steady-state leg (about 420 GCUPS offered, "
+                    help="arrivals per second of the steady-state leg (about 375 GCUPS offered, "
                          "under the batched server's capacity)")
-    ap.add_argument("--serve-saturated", type=int, default=64, metavar="N",
+    ap.add_argument("--serve-saturated", type=int, default=256, metavar="N",
                     help="requests in the saturated serving leg at --serve-saturated-rate "
                          "(0 skips it)")
     ap.add_argument("--serve-saturated-rate", type=float, default=50.0)
