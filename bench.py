@@ -706,8 +706,16 @@ def main():
         # one cooperative launch per chunk: a chunk ends at a sample (which
         # the launch itself does: the kernel's sampling instance) or at 65536
         # steps
+        # the instance's inflow mode as launch_band picks it: 0 the table
+        # wave draws the band's inflow cells, 1 each row's lanes, 2 the
+        # serial per-word chain (inflow wider than 64 cells)
+        cpr = 1
+        while cpr < min(cfg["inlet"], 64):
+            cpr *= 2
+        rows = bands_res[0]
+        inflow = 2 if cfg["inlet"] > 64 else (0 if -(-rows // (64 // cpr)) <= 3 else 1)
         kernel = (f"lg_band_kernel<{'true' if cfg['w'] > 4032 else 'false'},"
-                  f"{'true' if every else 'false'},{'true' if cfg['inlet'] > 64 else 'false'}>")
+                  f"{'true' if every else 'false'},{inflow}>")
         steps_per_launch = min(args.steps, every or args.steps, 65536)
     elif passes == 1:
         kernel = f"lg_step_kernel_{unit}"

@@ -107,24 +107,41 @@ __device__ __forceinline__ uint64_t lg_band_right(uint64_t x, int addr) {
 // stream table of that step) for fluid mask fm: both halves, the low half's
 // digits in the low 32 bits of K[b] -- lg_draw_unit<1>'s digits, kept in
 // registers. A wave whose lanes all draw every cell of a half takes the
-// plain chain for it; otherwise the whole wave takes the branch-free masked
-// chain, which continues through the masked cells, so the upper half starts
-// where the lower one ended without a second jump.
+// plain chain for it. Where some lane's half has cells without a draw, and
+// in every lane those form at most one run (a plate, a row's tail), the wave
+// takes the plain chain and moves the digits past the run
+// (lg_draw_half_run); the upper half then starts from its own jump (cjhi:
+// the word's cjump[2 * word + 1]) rather than where the lower one ended.
+// Otherwise the whole wave takes the branch-free masked chain, which
+// continues through the masked cells, so the upper half starts where the
+// lower one ended without a second jump.
 template <int NK>
 __device__ __forceinline__ void lg_band_digits(uint64_t fm, uint64_t cj0,
+                                               const uint64_t* __restrict cjhi,
                                                const uint64_t* __restrict bt, uint32_t stride,
                                                uint64_t K[LG_KDIGITS]) {
   uint32_t Kl[NK], Kh[NK];
   uint64_t st = lg_jump_tab(cj0, bt);
   const uint32_t f0 = (uint32_t)fm, f1 = (uint32_t)(fm >> 32);
-  if (__ballot(f0 != ~0u))
-    st = lg_draw_half_sel<NK>(st, f0, stride, Kl);
-  else
+  if (__ballot(f0 != ~0u)) {
+    if (__ballot(!lg_one_run(f0))) {
+      st = lg_draw_half_sel<NK>(st, f0, stride, Kl);
+    } else {
+      const uint64_t cj1 = *cjhi;
+      lg_draw_half_run<NK>(st, f0, stride, Kl);
+      st = lg_jump_tab(cj1, bt);
+    }
+  } else {
     st = lg_draw_half<NK>(st, stride, Kl);
-  if (__ballot(f1 != ~0u))
-    lg_draw_half_sel<NK>(st, f1, stride, Kh);
-  else
+  }
+  if (__ballot(f1 != ~0u)) {
+    if (__ballot(!lg_one_run(f1)))
+      lg_draw_half_sel<NK>(st, f1, stride, Kh);
+    else
+      lg_draw_half_run<NK>(st, f1, stride, Kh);
+  } else {
     lg_draw_half<NK>(st, stride, Kh);
+  }
   LG_UNROLL
   for (int b = 0; b < LG_KDIGITS; ++b) K[b] = b < NK ? ((uint64_t)Kh[b] << 32) | Kl[b] : 0ull;
 }
@@ -164,23 +181,52 @@ LG_HD inline uint32_t lg_band_row_split(uint32_t w, uint32_t B) {
 // LG_WTAB entries): its row bands' collision streams rb0, rb0 + 1, ... and,
 // last, the inflow stream of the step's end (src/lattice.rs:233,242,276);
 // the same 4-bit window tables lg_tables_kernel builds. One lane per
-// (stream, window group): it walks to the group's first state and writes its
+// (stream, window group): it reaches the group's first state 4g from the
+// seeded start through the T^(4g) windows in LDS (pw) and writes the group's
 // 16 entries; the lanes of one wave take the tasks in turn.
 __device__ __forceinline__ void lg_band_tables(const lgca_internal::LgBandArgs& A, uint32_t rb0,
                                                uint32_t nsb, uint64_t t, uint32_t lane,
+                                               const uint64_t* __restrict pw,
                                                uint64_t* __restrict out) {
   const uint32_t tasks = (nsb + 1) * 16;
   for (uint32_t q = lane; q < tasks; q += 64) {
     const uint32_t sidx = q >> 4, grp = q & 15u;
-    uint64_t st = lg_rng_init(sidx < nsb ? lg_band_seed(A.seed, t, rb0 + sidx)
-                                         : lg_inlet_seed(A.seed, t + 1));
-    for (uint32_t i = 0; i < 4 * grp; ++i) st = lg_xs(st);
-    const uint64_t b0 = st, b1 = lg_xs(b0), b2 = lg_xs(b1), b3 = lg_xs(b2);
-    uint64_t* o = out + (size_t)sidx * LG_WTAB + grp * 16;
+    const uint64_t st = lg_rng_init(sidx < nsb ? lg_band_seed(A.seed, t, rb0 + sidx)
+                                               : lg_inlet_seed(A.seed, t + 1));
+    lg_tab_group(lg_xs_pow4(st, grp, pw), out + (size_t)sidx * LG_WTAB + grp * 16);
+  }
+}
+
+// Step t's inflow cells of the band's rows, drawn by the table wave (inflow
+// of at most 64 cells, lanes per row `cpr` a power of two): lane q of chunk c
+// takes cell q % cpr of band row c * (64 / cpr) + q / cpr from its own jump
+// (icell; zero for a solid cell, which takes no draw) against the inflow
+// stream's table itab; each row's seven state planes of cells 0 .. cols - 1
+// go to out[row * 8 + d], which the row's collide of step t applies.
+__device__ __forceinline__ void lg_band_inflow(const LgGeom& G, const uint64_t* __restrict icell,
+                                               uint32_t y0, uint32_t Bb, uint32_t cpr,
+                                               uint32_t lane, const uint64_t* __restrict itab,
+                                               uint64_t* __restrict out) {
+  const uint32_t cols = G.inlet_cols, rpc = 64u / cpr;
+  const uint64_t fieldmask = cpr >= 64 ? ~0ull : (1ull << cpr) - 1ull;
+  for (uint32_t r0 = 0; r0 < Bb; r0 += rpc) {
+    const uint32_t r = r0 + lane / cpr, x = lane % cpr;
+    uint32_t s = 0;
+    if (r < Bb && x < cols) {
+      const uint64_t c = icell[(size_t)(y0 + r) * cols + x];
+      if (c) {
+        uint64_t st = lg_jump_tab(c, itab);
+        s = lg_eq_cell(G, st);
+      }
+    }
+    uint64_t m[7];
     LG_UNROLL
-    for (uint32_t v = 0; v < 16; ++v)
-      o[v] = ((v & 1u) ? b0 : 0ull) ^ ((v & 2u) ? b1 : 0ull) ^ ((v & 4u) ? b2 : 0ull) ^
-             ((v & 8u) ? b3 : 0ull);
+    for (int d = 0; d < 7; ++d) m[d] = __ballot((s >> d) & 1u);
+    if (lane < rpc && r0 + lane < Bb) {
+      uint64_t* o = out + (size_t)(r0 + lane) * 8;
+      LG_UNROLL
+      for (int d = 0; d < 7; ++d) o[d] = (m[d] >> ((lane * cpr) & 63u)) & fieldmask;
+    }
   }
 }
 
@@ -274,13 +320,18 @@ __device__ __forceinline__ void lg_band_sample(const lgca_internal::LgBandField&
 // One band per workgroup (blockIdx.x), one row per wave, one word per lane,
 // plus one wave that builds the next step's stream tables; A.n steps from
 // A.src into A.dst. LDS: two exchange slots of the rows' planes 1, 2, 4, 5
-// (rows x 4 KB) and two slots of stream tables (A.nstreams x 4 KB).
+// (rows x 4 KB), two slots of stream tables (A.nstreams x 4 KB), the helper
+// waves' digits (split bands), the T^(4g) windows (32 KB) and two slots of
+// the table wave's inflow planes (rows x 64 B).
 // SAMPLE: the launch ends with the fused coarse-graining (an instance of its
 // own, so launches without it keep the step loop's register allocation).
-// WIDE: inflow wider than one word (the serial per-word chain); otherwise one
-// lane per inflow cell. Each choice is an instance of its own, so the other's
-// registers are not held across the step loop.
-template <bool WAVEROW, bool SAMPLE, bool WIDE>
+// INFLOW (how the inflow cells draw): 0, the table wave draws every inflow
+// cell of the band (inflow of at most 64 cells, few cells per band); 1, in
+// each row's wave lane x draws cell x (at most 64 cells); 2, the serial
+// per-word chain in the lanes of the inflow words (wider inflow). Each choice
+// is an instance of its own, so the others' registers are not held across
+// the step loop.
+template <bool WAVEROW, bool SAMPLE, int INFLOW>
 __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
     lg_band_kernel(lgca_internal::LgBandArgs A) {
   extern __shared__ uint64_t xch[];  // [2 slots][rows][4 planes][64 lanes]
@@ -295,6 +346,10 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
   const uint32_t hh = helper ? w - NW - 1 : 0u;  // the half a helper draws
   const bool drawless = split && w + 1 == NW;    // row 0's wave: digits from the helpers
   uint32_t* const kx = reinterpret_cast<uint32_t*>(tabs_lds + (size_t)2 * A.nstreams * LG_WTAB);
+  // [16 groups][16 windows][16 values], then [2 slots][rows][8] inflow planes
+  uint64_t* const pw_lds =
+      tabs_lds + (size_t)2 * A.nstreams * LG_WTAB + (split ? 2 * LG_KDIGITS * 64 : 0);
+  uint64_t* const ilds = pw_lds + 16 * 256;
   const uint32_t b = blockIdx.x, y0 = b * A.rows;
   const uint32_t Bb = min(A.rows, G.h - y0);  // rows of this band
   // the band row it holds (helpers: none, but y below falls on row 0, whose
@@ -324,14 +379,14 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
   const uint64_t fm = live ? ~sm & tail : 0ull;
   const uint64_t cj0 = A.cjump[2 * word + hh];  // a helper's half starts at its own jump
   const uint32_t stream = A.row_band[y];
-  // inflow: for widths of at most 64 cells lane x samples cell x of the
-  // row's first word from its own jump (icell); wider inflow runs the
-  // serial per-word chain (ijump) in the lanes of the inflow words
-  constexpr bool lane_inflow = !WIDE;
+  // inflow: INFLOW 1, lane x samples cell x of the row's first word from its
+  // own jump (icell); INFLOW 2, the serial per-word chain (ijump) in the
+  // lanes of the inflow words; INFLOW 0, the table wave draws them all
+  constexpr bool lane_inflow = INFLOW == 1;
   const uint64_t sm0 = A.solid[(size_t)y * G.wpr];
   const bool icell_live = lane_inflow && lane < G.inlet_cols && !((sm0 >> lane) & 1ull);
   const uint64_t ic = icell_live ? A.icell[(size_t)y * G.inlet_cols + lane] : 0ull;
-  const uint64_t ij = (!lane_inflow && j < G.iw) ? A.ijump[2 * ((size_t)y * G.iw + j)] : 0ull;
+  const uint64_t ij = (INFLOW == 2 && j < G.iw) ? A.ijump[2 * ((size_t)y * G.iw + j)] : 0ull;
   // the step's inflow: wide, the word's seven planes (I); narrow, the lane's
   // cell (ibits), gathered across the wave where it is applied
   uint64_t K[LG_KDIGITS], I[7];
@@ -340,7 +395,9 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
   for (int q = 0; q < LG_KDIGITS; ++q) K[q] = 0;
   LG_UNROLL
   for (int d = 0; d < 7; ++d) I[d] = 0;
-  if (table_wave) lg_band_tables(A, rb0, nsb, A.t0, lane, tabs_lds);
+  for (uint32_t q = threadIdx.x; q < 16 * 256; q += blockDim.x) pw_lds[q] = A.pw[q];
+  __syncthreads();
+  if (table_wave) lg_band_tables(A, rb0, nsb, A.t0, lane, pw_lds, tabs_lds);
   if (row_live) {
     uint64_t* s0 = xch + (size_t)r * 256;
     s0[0 * 64 + lane] = P[1];
@@ -415,9 +472,13 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
       }
       uint64_t res[7];
       lg_collide_word(G, n, sm, Kl, Kh, res);
-      if constexpr (!WIDE) {  // cells 0 .. inlet_cols - 1 of word 0
+      if constexpr (INFLOW == 1) {  // cells 0 .. inlet_cols - 1 of word 0
         LG_UNROLL
         for (int d = 0; d < 7; ++d) I[d] = __ballot((ibits >> d) & 1u);
+      } else if constexpr (INFLOW == 0) {  // drawn by the table wave
+        const uint64_t* iin = ilds + ((size_t)(s & 1u) * NW + r) * 8;
+        LG_UNROLL
+        for (int d = 0; d < 7; ++d) I[d] = iin[d];
       }
       if (j < G.iw) lg_inflow_apply(G, j, sm, tail, I, res);
       LG_UNROLL
@@ -444,10 +505,15 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
         lg_band_publish(&A.flags[A.nb + b], A.epoch + s + 1);
       }
     }
-    // the next step's tables, beside the rows' work
+    // the next step's tables, beside the rows' work, and (INFLOW 0) this
+    // step's inflow cells
     if (table_wave && i + 1 < A.n)
-      lg_band_tables(A, rb0, nsb, A.t0 + i + 1, lane,
+      lg_band_tables(A, rb0, nsb, A.t0 + i + 1, lane, pw_lds,
                      tabs_lds + (size_t)((i + 1) & 1u) * A.nstreams * LG_WTAB);
+    if (INFLOW == 0 && table_wave && i < A.n && G.inlet_cols > 0)
+      lg_band_inflow(G, A.icell, y0, Bb, A.icpr, lane,
+                     tabs_lds + (size_t)(i & 1u) * A.nstreams * LG_WTAB + (size_t)nsb * LG_WTAB,
+                     ilds + (size_t)(i & 1u) * NW * 8);
     // step i's draws
     if (row_live && i < A.n) {
       const uint64_t* tab = tabs_lds + (size_t)(i & 1u) * A.nstreams * LG_WTAB;
@@ -459,11 +525,14 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
       asm volatile("" : "+v"(c0), "+v"(ci), "+v"(cw));
 #endif
       if (live && !drawless) {
+        const uint64_t* cjhi = A.cjump + 2 * word + 1;
         if (G.rest)
-          lg_band_digits<LG_COLLIDE_REST_KBITS>(fm, c0, tab + (size_t)(stream - rb0) * LG_WTAB,
+          lg_band_digits<LG_COLLIDE_REST_KBITS>(fm, c0, cjhi,
+                                                tab + (size_t)(stream - rb0) * LG_WTAB,
                                                 LG_COLLIDE_REST_STRIDE, K);
         else
-          lg_band_digits<LG_COLLIDE_NOREST_KBITS>(fm, c0, tab + (size_t)(stream - rb0) * LG_WTAB,
+          lg_band_digits<LG_COLLIDE_NOREST_KBITS>(fm, c0, cjhi,
+                                                  tab + (size_t)(stream - rb0) * LG_WTAB,
                                                   LG_COLLIDE_NOREST_STRIDE, K);
       }
       if constexpr (lane_inflow) {
@@ -473,7 +542,7 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
           uint64_t st = lg_jump_tab(ci, itab);
           ibits = lg_eq_cell(G, st);
         }
-      } else if (live && j < G.iw) {
+      } else if (INFLOW == 2 && live && j < G.iw) {
         const uint64_t apply = lg_inlet_apply(G, j, fm);
         if (apply) lg_inlet_bits(G, lg_jump_tab(cw, itab), apply, I);
       }
@@ -492,14 +561,21 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
       if (live) {
         const uint32_t fh = hh ? (uint32_t)(fm >> 32) : (uint32_t)fm;
         const uint64_t st = lg_jump_tab(c0, tab);
+        // each half starts at its own jump: a half with one run of cells
+        // without a draw needs no end state (lg_draw_half_run)
         const bool masked = __ballot(fh != ~0u) != 0;
+        const bool runs = masked && __ballot(!lg_one_run(fh)) == 0;
         if (G.rest) {
-          if (masked)
+          if (runs)
+            lg_draw_half_run<LG_COLLIDE_REST_KBITS>(st, fh, LG_COLLIDE_REST_STRIDE, Kx);
+          else if (masked)
             lg_draw_half_sel<LG_COLLIDE_REST_KBITS>(st, fh, LG_COLLIDE_REST_STRIDE, Kx);
           else
             lg_draw_half<LG_COLLIDE_REST_KBITS>(st, LG_COLLIDE_REST_STRIDE, Kx);
         } else {
-          if (masked)
+          if (runs)
+            lg_draw_half_run<LG_COLLIDE_NOREST_KBITS>(st, fh, LG_COLLIDE_NOREST_STRIDE, Kx);
+          else if (masked)
             lg_draw_half_sel<LG_COLLIDE_NOREST_KBITS>(st, fh, LG_COLLIDE_NOREST_STRIDE, Kx);
           else
             lg_draw_half<LG_COLLIDE_NOREST_KBITS>(st, LG_COLLIDE_NOREST_STRIDE, Kx);
@@ -555,6 +631,12 @@ int ensure_band_buffers(lg_t* g, uint32_t nb) {
   if (!g->band_err_host) {
     HIP_TRY(hipHostMalloc((void**)&g->band_err_host, 8, hipHostMallocDefault));
     *g->band_err_host = 0;
+  }
+  if (!g->band_pw.p) {
+    std::vector<uint64_t> pw;
+    lgca::xs_pow4_windows(pw);
+    HIP_TRY(g->band_pw.alloc(pw.size()));
+    HIP_TRY(hipMemcpy(g->band_pw.p, pw.data(), pw.size() * 8, hipMemcpyHostToDevice));
   }
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->band_nb = nb;  // the epoch keeps growing: cleared flags lie below any target
@@ -637,23 +719,40 @@ int launch_band(lg_t* g, uint32_t n, bool sample, float alpha) {
   A.split = (rows % 4 == 1 && rows + 3 <= LG_BAND_MAX_ROWS + 1) ? 1u : 0u;
   A.alpha = alpha;
   A.fs = g->band_field.p;
+  A.pw = g->band_pw.p;
+  // inflow: the table wave draws every inflow cell of the band where that
+  // takes it at most three passes over its lanes (64 / icpr rows a pass),
+  // else each row's lanes draw their own (at most 64 cells) or the serial
+  // per-word chain runs (wider)
+  const uint32_t cols = g->G.inlet_cols;
+  uint32_t cpr = 1;
+  while (cpr < cols && cpr < 64) cpr <<= 1;
+  A.icpr = cpr;
+  const int inflow = cols > 64 ? 2 : ((rows + 64 / cpr - 1) / (64 / cpr) <= 3 ? 0 : 1);
   void* args[] = {&A};
   const dim3 grid(nb), block(64 * (rows + 1 + 2 * A.split));
-  const size_t lds = (size_t)(rows + A.nstreams) * 4096 + (A.split ? 2 * 2 * LG_KDIGITS * 64 * 4 : 0);
-  // [rows of 64 words][sampling][inflow wider than a word]
-  const void* const kernels[2][2][2] = {
-      {{(const void*)lg_band_kernel<false, false, false>, (const void*)lg_band_kernel<false, false, true>},
-       {(const void*)lg_band_kernel<false, true, false>, (const void*)lg_band_kernel<false, true, true>}},
-      {{(const void*)lg_band_kernel<true, false, false>, (const void*)lg_band_kernel<true, false, true>},
-       {(const void*)lg_band_kernel<true, true, false>, (const void*)lg_band_kernel<true, true, true>}}};
+  // [rows][stream tables][helpers' digits][T^(4g) windows][inflow planes]
+  const size_t lds = (size_t)(rows + A.nstreams) * 4096 +
+                     (A.split ? 2 * 2 * LG_KDIGITS * 64 * 4 : 0) + 16 * 256 * 8 +
+                     (size_t)2 * rows * 8 * 8;
+  // [rows of 64 words][sampling][inflow]
+  const void* const kernels[2][2][3] = {
+      {{(const void*)lg_band_kernel<false, false, 0>, (const void*)lg_band_kernel<false, false, 1>,
+        (const void*)lg_band_kernel<false, false, 2>},
+       {(const void*)lg_band_kernel<false, true, 0>, (const void*)lg_band_kernel<false, true, 1>,
+        (const void*)lg_band_kernel<false, true, 2>}},
+      {{(const void*)lg_band_kernel<true, false, 0>, (const void*)lg_band_kernel<true, false, 1>,
+        (const void*)lg_band_kernel<true, false, 2>},
+       {(const void*)lg_band_kernel<true, true, 0>, (const void*)lg_band_kernel<true, true, 1>,
+        (const void*)lg_band_kernel<true, true, 2>}}};
   if (lds > 64 * 1024 && !g->band_lds_raised) {
-    for (int q = 0; q < 8; ++q)
-      HIP_TRY(hipFuncSetAttribute(kernels[q >> 2][(q >> 1) & 1][q & 1],
+    for (int q = 0; q < 12; ++q)
+      HIP_TRY(hipFuncSetAttribute(kernels[q / 6][(q / 3) & 1][q % 3],
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     g->band_lds_raised = true;
   }
-  const hipError_t e = hipLaunchCooperativeKernel(
-      kernels[g->G.wpr == 64][sample][g->G.inlet_cols > 64], grid, block, args, lds, g->stream);
+  const hipError_t e = hipLaunchCooperativeKernel(kernels[g->G.wpr == 64][sample][inflow], grid,
+                                                  block, args, lds, g->stream);
   if (e != hipSuccess) {
     // Refused (the runtime can decline a cooperative launch, e.g. while
     // another process holds the device's cooperative-launch resources):

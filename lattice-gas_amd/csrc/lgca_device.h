@@ -105,6 +105,40 @@ LG_HD inline uint64_t lg_jump_tab(uint64_t c, const uint64_t* __restrict tab) {
   const uint64_t x4 = lg_op3<0x96>(e[12], e[13], e[14]);
   return lg_op3<0x96>(x0, x1, x2) ^ lg_op3<0x96>(x3, x4, e[15]);
 }
+// State 4g of a stream from its start s (g < 16): T^(4g) s through 4-bit
+// windows of s against pw = lgca::xs_pow4_windows (16 lookups).
+LG_HD inline uint64_t lg_xs_pow4(uint64_t s, uint32_t g, const uint64_t* __restrict pw) {
+  const uint64_t* __restrict t = pw + (size_t)g * 256;
+  uint64_t e[16];
+  LG_UNROLL
+  for (int w = 0; w < 16; ++w) e[w] = t[w * 16 + (uint32_t)((s >> (4 * w)) & 15u)];
+  const uint64_t x0 = lg_op3<0x96>(e[0], e[1], e[2]), x1 = lg_op3<0x96>(e[3], e[4], e[5]);
+  const uint64_t x2 = lg_op3<0x96>(e[6], e[7], e[8]), x3 = lg_op3<0x96>(e[9], e[10], e[11]);
+  const uint64_t x4 = lg_op3<0x96>(e[12], e[13], e[14]);
+  return lg_op3<0x96>(x0, x1, x2) ^ lg_op3<0x96>(x3, x4, e[15]);
+}
+// Window group g (entries g * 16 .. g * 16 + 15) of a stream's tables from
+// its state 4g: entry v is the XOR of states 4g + b over the set bits b of v.
+LG_HD inline void lg_tab_group(uint64_t b0, uint64_t* __restrict o) {
+  const uint64_t b1 = lg_xs(b0), b2 = lg_xs(b1), b3 = lg_xs(b2);
+  const uint64_t b01 = b0 ^ b1, b23 = b2 ^ b3;
+  o[0] = 0;
+  o[1] = b0;
+  o[2] = b1;
+  o[3] = b01;
+  o[4] = b2;
+  o[5] = b0 ^ b2;
+  o[6] = b1 ^ b2;
+  o[7] = b01 ^ b2;
+  o[8] = b3;
+  o[9] = b0 ^ b3;
+  o[10] = b1 ^ b3;
+  o[11] = b01 ^ b3;
+  o[12] = b23;
+  o[13] = b0 ^ b23;
+  o[14] = b1 ^ b23;
+  o[15] = b01 ^ b23;
+}
 // Fill one stream's window tables from its seeded start state.
 LG_HD inline void lg_build_tab(uint64_t s, uint64_t* __restrict tab) {
   uint64_t B[64];
@@ -263,6 +297,33 @@ LG_HD inline uint64_t lg_draw_half(uint64_t st, uint32_t stride, uint32_t P[NK])
     P[b] = b ? ((acc >> b) | (acc << (32 - b))) : acc;
   }
   return st;
+}
+
+// Whether the cells of a half that take no draw (the zero bits of fh) form
+// at most one run of consecutive cells (a plate, a disc's chord, a row's
+// tail end).
+LG_HD inline bool lg_one_run(uint32_t fh) {
+  const uint32_t z = ~fh;
+  return ((z + (z & (0u - z))) & z) == 0u;
+}
+
+// The digits of a half whose non-drawing cells form one run [a, a + len)
+// (lg_one_run): cells before the run take the plain chain's draws 0 .. a - 1,
+// the run takes none (digits zero, as lg_draw_half_sel gives), and a cell x
+// after it takes draw x - len, which the plain chain put at cell x - len: one
+// shift and two masks per digit plane instead of the masked chain. The
+// chain's end state is that of 32 draws, not the half's 32 - len: a caller
+// that continues the stream jumps to the next half's start instead.
+template <int NK>
+LG_HD inline void lg_draw_half_run(uint64_t st, uint32_t fh, uint32_t stride, uint32_t K[NK]) {
+  lg_draw_half<NK>(st, stride, K);
+  const uint32_t z = ~fh;
+  const uint32_t len = (uint32_t)__builtin_popcount(z);
+  const uint32_t a = z ? (uint32_t)__builtin_ctz(z) : 32u, b = a + len;  // b <= 32
+  const uint32_t before = a >= 32 ? ~0u : (1u << a) - 1u;
+  const uint32_t after = b >= 32 ? 0u : ~((1u << b) - 1u);
+  LG_UNROLL
+  for (int q = 0; q < NK; ++q) K[q] = (K[q] & before) | ((K[q] << (len & 31u)) & after);
 }
 
 // The same 32 cells when only the cells in fh draw (solid sites and cells

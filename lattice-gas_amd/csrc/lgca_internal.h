@@ -120,6 +120,13 @@ struct LgBandArgs {
   uint32_t n, rows, nb;
   uint32_t nstreams;  // stream tables per band and step in LDS (band streams + inflow)
   uint32_t split;     // row 0's draws by two helper waves (bands of 4m + 1 rows)
+  // window tables of T^(4g) (lgca::xs_pow4_windows), copied into LDS at the
+  // launch's start: the table wave reaches each window group's first state
+  // with 16 lookups
+  const uint64_t* pw;
+  // inflow drawn by the table wave: lanes per band row (a power of two, at
+  // least the inflow width)
+  uint32_t icpr;
   // Coarse-graining of the launch's final state with weight alpha (the
   // kernel instance that does it); the field's layout in device memory at
   // `fs`, read only after the step loop.
@@ -221,6 +228,7 @@ struct lg_lattice {
   // default) = where eligible unless another pipeline was chosen.
   uint32_t resident = 2;
   DevBuf<uint64_t> band_halo, band_flags, band_err_dev;
+  DevBuf<uint64_t> band_pw;  // window tables of T^(4g) (lgca::xs_pow4_windows)
   uint64_t* band_err_host = nullptr;  // page-locked
   uint32_t band_nb = 0;
   uint64_t band_epoch = 0, band_epoch_checked = 0;

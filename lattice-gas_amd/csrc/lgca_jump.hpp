@@ -246,4 +246,29 @@ inline void build_inflow_cells(uint32_t w, uint32_t h, uint32_t inlet_cols, bool
     }
 }
 
+// Window tables of the maps T^(4g), g = 0..15 (T^0 the identity): T^(4g) is
+// linear over GF(2)^64 and the same for every stream and step, so
+//     T^(4g) s = XOR over w = 0..15 of pw[g * 256 + w * 16 + ((s >> 4w) & 15)],
+//     pw[g * 256 + w * 16 + v] = T^(4g) (v << 4w).
+// A lane then reaches state 4g of a freshly seeded stream with 16 lookups
+// instead of walking 4g steps (lg_xs_pow4).
+inline void xs_pow4_windows(std::vector<uint64_t>& pw) {
+  pw.assign((size_t)16 * 256, 0);
+  for (uint32_t g = 0; g < 16; ++g) {
+    uint64_t col[64];  // T^(4g) of each basis vector
+    for (uint32_t i = 0; i < 64; ++i) {
+      uint64_t x = 1ull << i;
+      for (uint32_t k = 0; k < 4 * g; ++k) x = xs_step(x);
+      col[i] = x;
+    }
+    for (uint32_t w = 0; w < 16; ++w)
+      for (uint32_t v = 0; v < 16; ++v) {
+        uint64_t e = 0;
+        for (uint32_t b = 0; b < 4; ++b)
+          if ((v >> b) & 1u) e ^= col[4 * w + b];
+        pw[(size_t)g * 256 + w * 16 + v] = e;
+      }
+  }
+}
+
 }  // namespace lgca

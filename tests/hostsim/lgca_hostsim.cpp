@@ -301,6 +301,64 @@ int hs_draw_sel_check(uint64_t seed, int n) {
   return bad;
 }
 
+// The single-run draw (lg_draw_half_run) against the branching masked draw
+// (lg_draw_masked32) on n pseudo-random states and masks whose non-drawing
+// cells form one run (anywhere, of any length, including none and all 32),
+// and lg_one_run against a cell-by-cell count of runs on arbitrary masks.
+// Returns the number of mismatches.
+int hs_draw_run_check(uint64_t seed, int n) {
+  uint64_t x = seed | 1u;
+  int bad = 0;
+  for (int i = 0; i < n; ++i) {
+    x = lg_xs(x);
+    const uint64_t st = lg_rng_init(x);
+    x = lg_xs(x);
+    const uint32_t a = (uint32_t)(x % 33), len = (uint32_t)((x >> 8) % (33 - a));
+    const uint64_t run = len ? (((len >= 32 ? ~0ull : (1ull << len) - 1ull)) << a) : 0ull;
+    const uint32_t fh = ~(uint32_t)run;
+    uint32_t A[5], B[5], A3[3], B3[3];
+    lg_draw_half_run<5>(st, fh, 30, A);
+    lg_draw_masked32<5>(st, fh, 30, B);
+    for (int b = 0; b < 5; ++b) bad += A[b] != B[b];
+    lg_draw_half_run<3>(st, fh, 6, A3);
+    lg_draw_masked32<3>(st, fh, 6, B3);
+    for (int b = 0; b < 3; ++b) bad += A3[b] != B3[b];
+    bad += !lg_one_run(fh);
+    // arbitrary masks: at most one run of zero bits?
+    x = lg_xs(x);
+    const uint32_t m = (uint32_t)(x >> 13) | (i % 3 == 0 ? (uint32_t)(x >> 29) : 0u);
+    int runs = 0;
+    for (int c = 0; c < 32; ++c)
+      if (!((m >> c) & 1u) && (c == 0 || ((m >> (c - 1)) & 1u))) ++runs;
+    bad += lg_one_run(m) != (runs <= 1);
+  }
+  return bad;
+}
+
+// The T^(4g) windows (lgca::xs_pow4_windows, lg_xs_pow4) against 4g
+// sequential steps, and a stream's window tables assembled group by group
+// (lg_tab_group) against lg_build_tab, on n pseudo-random states. Returns the
+// number of mismatches.
+int hs_pow4_check(uint64_t seed, int n) {
+  std::vector<uint64_t> pw, ref(LG_WTAB), tab(LG_WTAB);
+  lgca::xs_pow4_windows(pw);
+  uint64_t x = seed | 1u;
+  int bad = 0;
+  for (int i = 0; i < n; ++i) {
+    x = lg_xs(x);
+    const uint64_t s = lg_rng_init(x);
+    uint64_t t = s;
+    for (uint32_t g = 0; g < 16; ++g) {
+      bad += lg_xs_pow4(s, g, pw.data()) != t;
+      lg_tab_group(lg_xs_pow4(s, g, pw.data()), tab.data() + g * 16);
+      for (int k = 0; k < 4; ++k) t = lg_xs(t);
+    }
+    lg_build_tab(s, ref.data());
+    for (int e = 0; e < LG_WTAB; ++e) bad += tab[e] != ref[e];
+  }
+  return bad;
+}
+
 // Per-cell inflow jumps (build_inflow_cells) against the per-word ones
 // (build_tables' ijump): for a random solid layout and inflow width <= 64,
 // every fluid inflow cell sampled from its own jump gives the bits the

@@ -212,6 +212,23 @@ def test_branch_free_masked_draw(hs):
     assert hs.hs_draw_sel_check(0x5E1, 20000) == 0
 
 
+def test_single_run_draw(hs):
+    """The resident bands' plain-chain draw for a half whose non-drawing cells
+    form one run (digits moved past the run) equals the branching masked
+    draw, for runs anywhere and of any length; lg_one_run says whether a
+    mask's zero bits form at most one run."""
+    hs.hs_draw_run_check.argtypes = [C.c_uint64, C.c_int]
+    assert hs.hs_draw_run_check(0x2A11, 20000) == 0
+
+
+def test_pow4_windows_reach_group_starts(hs):
+    """State 4g of a stream through the T^(4g) windows equals 4g sequential
+    xorshift steps, and tables assembled group by group from those states
+    equal the tables built from the first 64 states."""
+    hs.hs_pow4_check.argtypes = [C.c_uint64, C.c_int]
+    assert hs.hs_pow4_check(0x9047, 500) == 0
+
+
 @pytest.mark.parametrize("w,h,cols,rest", [(4096, 64, 8, 1), (100, 40, 64, 1), (65, 30, 65, 0),
                                             (300, 50, 37, 1), (64, 10, 1, 0)])
 def test_per_cell_inflow_jumps(hs, w, h, cols, rest):
