@@ -104,9 +104,16 @@ __device__ __forceinline__ uint64_t lg_band_right(uint64_t x, int addr) {
 }
 
 // The collision digits of word (y, j) for one step (bt: its row band's
-// stream table of that step) for fluid mask fm: both halves, the low half's
-// digits in the low 32 bits of K[b] -- lg_draw_unit<1>'s digits, kept in
-// registers. A wave whose lanes all draw every cell of a half takes the
+// stream table of that step) for fluid mask fm: both halves -- lg_draw_unit<1>'s
+// digits -- into the row's digit slot in LDS (kout[q * 64 + lane] the low
+// half's digit q, kout[(LG_KDIGITS + q) * 64 + lane] the high half's), where
+// the row's collide of the step reads them (in LDS rather than registers
+// across the barrier, which leaves the step loop's registers to the rest).
+// LEAD (every half of the lattice has at most one run of cells without a
+// draw): cj0 is the word's lead jump (lgca::build_lead_jumps), len draws
+// before its first draw where the low half has such a run; both halves take
+// the plain chain, and the digits are moved past the runs (lg_run_fix_lead,
+// lg_run_fix) in the waves where some lane needs it. A wave whose lanes all draw every cell of a half takes the
 // plain chain for it. Where some lane's half has cells without a draw, and
 // in every lane those form at most one run (a plate, a row's tail), the wave
 // takes the plain chain and moves the digits past the run
@@ -119,10 +126,22 @@ template <int NK>
 __device__ __forceinline__ void lg_band_digits(uint64_t fm, uint64_t cj0,
                                                const uint64_t* __restrict cjhi,
                                                const uint64_t* __restrict bt, uint32_t stride,
-                                               uint64_t K[LG_KDIGITS]) {
+                                               uint32_t lane, uint32_t* __restrict kout,
+                                               bool lead) {
   uint32_t Kl[NK], Kh[NK];
   uint64_t st = lg_jump_tab(cj0, bt);
   const uint32_t f0 = (uint32_t)fm, f1 = (uint32_t)(fm >> 32);
+  if (lead) {
+    st = lg_draw_half<NK>(st, stride, Kl);
+    if (__ballot(f0 != ~0u)) lg_run_fix_lead<NK>(f0, Kl);
+    LG_UNROLL
+    for (int q = 0; q < LG_KDIGITS; ++q) kout[q * 64 + lane] = q < NK ? Kl[q] : 0u;
+    lg_draw_half<NK>(st, stride, Kh);
+    if (__ballot(f1 != ~0u)) lg_run_fix<NK>(f1, Kh);
+    LG_UNROLL
+    for (int q = 0; q < LG_KDIGITS; ++q) kout[(LG_KDIGITS + q) * 64 + lane] = q < NK ? Kh[q] : 0u;
+    return;
+  }
   if (__ballot(f0 != ~0u)) {
     if (__ballot(!lg_one_run(f0))) {
       st = lg_draw_half_sel<NK>(st, f0, stride, Kl);
@@ -134,6 +153,8 @@ __device__ __forceinline__ void lg_band_digits(uint64_t fm, uint64_t cj0,
   } else {
     st = lg_draw_half<NK>(st, stride, Kl);
   }
+  LG_UNROLL
+  for (int q = 0; q < LG_KDIGITS; ++q) kout[q * 64 + lane] = q < NK ? Kl[q] : 0u;
   if (__ballot(f1 != ~0u)) {
     if (__ballot(!lg_one_run(f1)))
       lg_draw_half_sel<NK>(st, f1, stride, Kh);
@@ -143,7 +164,7 @@ __device__ __forceinline__ void lg_band_digits(uint64_t fm, uint64_t cj0,
     lg_draw_half<NK>(st, stride, Kh);
   }
   LG_UNROLL
-  for (int b = 0; b < LG_KDIGITS; ++b) K[b] = b < NK ? ((uint64_t)Kh[b] << 32) | Kl[b] : 0ull;
+  for (int q = 0; q < LG_KDIGITS; ++q) kout[(LG_KDIGITS + q) * 64 + lane] = q < NK ? Kh[q] : 0u;
 }
 
 }  // namespace
@@ -321,8 +342,9 @@ __device__ __forceinline__ void lg_band_sample(const lgca_internal::LgBandField&
 // plus one wave that builds the next step's stream tables; A.n steps from
 // A.src into A.dst. LDS: two exchange slots of the rows' planes 1, 2, 4, 5
 // (rows x 4 KB), two slots of stream tables (A.nstreams x 4 KB), the helper
-// waves' digits (split bands), the T^(4g) windows (32 KB) and two slots of
-// the table wave's inflow planes (rows x 64 B).
+// waves' digits (split bands), the T^(4g) windows (32 KB), two slots of
+// the table wave's inflow planes (rows x 64 B) and each row's digits for its
+// next collide (rows x 2.5 KB): at most 157 KB (11 rows, 12 streams).
 // SAMPLE: the launch ends with the fused coarse-graining (an instance of its
 // own, so launches without it keep the step loop's register allocation).
 // INFLOW (how the inflow cells draw): 0, the table wave draws every inflow
@@ -350,6 +372,8 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
   uint64_t* const pw_lds =
       tabs_lds + (size_t)2 * A.nstreams * LG_WTAB + (split ? 2 * LG_KDIGITS * 64 : 0);
   uint64_t* const ilds = pw_lds + 16 * 256;
+  // [rows][2 halves][LG_KDIGITS][64 lanes]: each row's digits for its next collide
+  uint32_t* const kr = reinterpret_cast<uint32_t*>(ilds + (size_t)2 * NW * 8);
   const uint32_t b = blockIdx.x, y0 = b * A.rows;
   const uint32_t Bb = min(A.rows, G.h - y0);  // rows of this band
   // the band row it holds (helpers: none, but y below falls on row 0, whose
@@ -377,7 +401,8 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
   for (int d = 0; d < 7; ++d) P[d] = A.src[(size_t)d * T + word];
   const uint64_t sm = A.solid[word];
   const uint64_t fm = live ? ~sm & tail : 0ull;
-  const uint64_t cj0 = A.cjump[2 * word + hh];  // a helper's half starts at its own jump
+  // a helper's half starts at its own jump; a row's word at its lead jump
+  const uint64_t cj0 = (A.lead_ok && !helper) ? A.lead[word] : A.cjump[2 * word + hh];
   const uint32_t stream = A.row_band[y];
   // inflow: INFLOW 1, lane x samples cell x of the row's first word from its
   // own jump (icell); INFLOW 2, the serial per-word chain (ijump) in the
@@ -389,10 +414,8 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
   const uint64_t ij = (INFLOW == 2 && j < G.iw) ? A.ijump[2 * ((size_t)y * G.iw + j)] : 0ull;
   // the step's inflow: wide, the word's seven planes (I); narrow, the lane's
   // cell (ibits), gathered across the wave where it is applied
-  uint64_t K[LG_KDIGITS], I[7];
+  uint64_t I[7];
   uint32_t ibits = 0;
-  LG_UNROLL
-  for (int q = 0; q < LG_KDIGITS; ++q) K[q] = 0;
   LG_UNROLL
   for (int d = 0; d < 7; ++d) I[d] = 0;
   for (uint32_t q = threadIdx.x; q < 16 * 256; q += blockDim.x) pw_lds[q] = A.pw[q];
@@ -464,10 +487,11 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
           Kh[q] = kin[(LG_KDIGITS + q) * 64 + lane];
         }
       } else {
+        const uint32_t* kin = kr + (size_t)r * 2 * LG_KDIGITS * 64;
         LG_UNROLL
         for (int q = 0; q < LG_KDIGITS; ++q) {
-          Kl[q] = (uint32_t)K[q];
-          Kh[q] = (uint32_t)(K[q] >> 32);
+          Kl[q] = kin[q * 64 + lane];
+          Kh[q] = kin[(LG_KDIGITS + q) * 64 + lane];
         }
       }
       uint64_t res[7];
@@ -529,11 +553,15 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
         if (G.rest)
           lg_band_digits<LG_COLLIDE_REST_KBITS>(fm, c0, cjhi,
                                                 tab + (size_t)(stream - rb0) * LG_WTAB,
-                                                LG_COLLIDE_REST_STRIDE, K);
+                                                LG_COLLIDE_REST_STRIDE, lane,
+                                                kr + (size_t)r * 2 * LG_KDIGITS * 64,
+                                                A.lead_ok != 0);
         else
           lg_band_digits<LG_COLLIDE_NOREST_KBITS>(fm, c0, cjhi,
                                                   tab + (size_t)(stream - rb0) * LG_WTAB,
-                                                  LG_COLLIDE_NOREST_STRIDE, K);
+                                                  LG_COLLIDE_NOREST_STRIDE, lane,
+                                                  kr + (size_t)r * 2 * LG_KDIGITS * 64,
+                                                  A.lead_ok != 0);
       }
       if constexpr (lane_inflow) {
         // cell `lane` of word 0 (zero where it takes no inflow)
@@ -720,6 +748,8 @@ int launch_band(lg_t* g, uint32_t n, bool sample, float alpha) {
   A.alpha = alpha;
   A.fs = g->band_field.p;
   A.pw = g->band_pw.p;
+  A.lead = g->lead.p;
+  A.lead_ok = g->lead_ok ? 1u : 0u;
   // inflow: the table wave draws every inflow cell of the band where that
   // takes it at most three passes over its lanes (64 / icpr rows a pass),
   // else each row's lanes draw their own (at most 64 cells) or the serial
@@ -732,9 +762,10 @@ int launch_band(lg_t* g, uint32_t n, bool sample, float alpha) {
   void* args[] = {&A};
   const dim3 grid(nb), block(64 * (rows + 1 + 2 * A.split));
   // [rows][stream tables][helpers' digits][T^(4g) windows][inflow planes]
+  // [rows' digits]
   const size_t lds = (size_t)(rows + A.nstreams) * 4096 +
                      (A.split ? 2 * 2 * LG_KDIGITS * 64 * 4 : 0) + 16 * 256 * 8 +
-                     (size_t)2 * rows * 8 * 8;
+                     (size_t)2 * rows * 8 * 8 + (size_t)rows * 2 * LG_KDIGITS * 64 * 4;
   // [rows of 64 words][sampling][inflow]
   const void* const kernels[2][2][3] = {
       {{(const void*)lg_band_kernel<false, false, 0>, (const void*)lg_band_kernel<false, false, 1>,

@@ -308,15 +308,13 @@ LG_HD inline bool lg_one_run(uint32_t fh) {
 }
 
 // The digits of a half whose non-drawing cells form one run [a, a + len)
-// (lg_one_run): cells before the run take the plain chain's draws 0 .. a - 1,
-// the run takes none (digits zero, as lg_draw_half_sel gives), and a cell x
-// after it takes draw x - len, which the plain chain put at cell x - len: one
-// shift and two masks per digit plane instead of the masked chain. The
-// chain's end state is that of 32 draws, not the half's 32 - len: a caller
-// that continues the stream jumps to the next half's start instead.
+// (lg_one_run), from the plain chain's: cells before the run take the
+// chain's draws 0 .. a - 1, the run takes none (digits zero, as
+// lg_draw_half_sel gives), and a cell x after it takes draw x - len, which
+// the plain chain put at cell x - len: one shift and two masks per digit
+// plane instead of the masked chain.
 template <int NK>
-LG_HD inline void lg_draw_half_run(uint64_t st, uint32_t fh, uint32_t stride, uint32_t K[NK]) {
-  lg_draw_half<NK>(st, stride, K);
+LG_HD inline void lg_run_fix(uint32_t fh, uint32_t K[NK]) {
   const uint32_t z = ~fh;
   const uint32_t len = (uint32_t)__builtin_popcount(z);
   const uint32_t a = z ? (uint32_t)__builtin_ctz(z) : 32u, b = a + len;  // b <= 32
@@ -324,6 +322,29 @@ LG_HD inline void lg_draw_half_run(uint64_t st, uint32_t fh, uint32_t stride, ui
   const uint32_t after = b >= 32 ? 0u : ~((1u << b) - 1u);
   LG_UNROLL
   for (int q = 0; q < NK; ++q) K[q] = (K[q] & before) | ((K[q] << (len & 31u)) & after);
+}
+// The same when the chain started len draws before the half's first draw
+// (its jump is z^(n - len), lgca::build_lead_jumps): chain cell i holds draw
+// i - len, so a cell x after the run finds its draw in place and a cell x
+// before it at chain cell x + len; and the chain's end state is that of the
+// half's last draw, from which the next half continues.
+template <int NK>
+LG_HD inline void lg_run_fix_lead(uint32_t fh, uint32_t K[NK]) {
+  const uint32_t z = ~fh;
+  const uint32_t len = (uint32_t)__builtin_popcount(z);
+  const uint32_t a = z ? (uint32_t)__builtin_ctz(z) : 32u, b = a + len;
+  const uint32_t before = a >= 32 ? ~0u : (1u << a) - 1u;
+  const uint32_t after = b >= 32 ? 0u : ~((1u << b) - 1u);
+  LG_UNROLL
+  for (int q = 0; q < NK; ++q) K[q] = ((K[q] >> (len & 31u)) & before) | (K[q] & after);
+}
+// The digits of such a half by the plain chain from the half's first draw;
+// the chain's end state is that of 32 draws, not the half's 32 - len: a
+// caller that continues the stream jumps to the next half's start instead.
+template <int NK>
+LG_HD inline void lg_draw_half_run(uint64_t st, uint32_t fh, uint32_t stride, uint32_t K[NK]) {
+  lg_draw_half<NK>(st, stride, K);
+  lg_run_fix<NK>(fh, K);
 }
 
 // The same 32 cells when only the cells in fh draw (solid sites and cells

@@ -643,6 +643,10 @@ int rebuild_tables(lg_t* g, const std::vector<uint64_t>& solid_host) {
     HIP_TRY(g->icell.alloc(ic.size()));
     HIP_TRY(hipMemcpyAsync(g->icell.p, ic.data(), ic.size() * 8, hipMemcpyHostToDevice, g->stream));
   }
+  std::vector<uint64_t> lead;
+  g->lead_ok = lgca::build_lead_jumps(g->G.w, g->G.h, solid_host, cj, lead);
+  HIP_TRY(g->lead.alloc(lead.size()));
+  HIP_TRY(hipMemcpyAsync(g->lead.p, lead.data(), lead.size() * 8, hipMemcpyHostToDevice, g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->tables_dirty = false;
   return LG_OK;

@@ -127,6 +127,11 @@ struct LgBandArgs {
   // inflow drawn by the table wave: lanes per band row (a power of two, at
   // least the inflow width)
   uint32_t icpr;
+  // lead jumps (lgca::build_lead_jumps) and whether every half of the
+  // lattice has at most one run of cells without a draw (then the rows'
+  // draws take the plain chain from the lead jumps)
+  const uint64_t* lead;
+  uint32_t lead_ok;
   // Coarse-graining of the launch's final state with weight alpha (the
   // kernel instance that does it); the field's layout in device memory at
   // `fs`, read only after the step loop.
@@ -193,6 +198,10 @@ struct lg_lattice {
   // planes: a (current), b (next); solid; static tables
   DevBuf<uint64_t> a, b, solid, cjump, ijump, basis;
   DevBuf<uint64_t> icell;  // per-cell inflow jumps, inflow widths <= 64 (resident bands)
+  // resident bands: per-word lead jumps, and whether every half of the
+  // lattice has at most one run of cells without a draw (lgca::build_lead_jumps)
+  DevBuf<uint64_t> lead;
+  bool lead_ok = false;
   DevBuf<uint32_t> row_band, order;
   uint32_t units = 1;  // 1: a lane steps a 64-cell word; 2: a 32-cell half
   // passes per step: 1 = one fused kernel per step; 2 = one launch per step

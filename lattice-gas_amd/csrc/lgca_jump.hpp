@@ -246,6 +246,40 @@ inline void build_inflow_cells(uint32_t w, uint32_t h, uint32_t inlet_cols, bool
     }
 }
 
+// Jumps to a point before each word's first draw, for the resident bands'
+// plain-chain draw of a half whose cells without a draw form one run: lead[w]
+// = cjump[2 w] * z^(-len) mod P, len the count of the word's low-half cells
+// that take no draw when those form one run (else len = 0), so the chain of
+// 32 draws ends on the half's last draw (lg_run_fix_lead). z^(-len) =
+// z^(2^64 - 1 - len): P is primitive, so z has order 2^64 - 1. Returns
+// whether every half of every word has at most one such run (the layouts
+// the single-run draw covers: plates, boxes, discs, a row's tail).
+inline bool build_lead_jumps(uint32_t w, uint32_t h, const std::vector<uint64_t>& solid,
+                             const std::vector<uint64_t>& cjump, std::vector<uint64_t>& lead) {
+  const uint64_t p = charpoly_low();
+  const uint32_t wpr = (w + 63) / 64;
+  uint64_t zinv[33];
+  for (uint32_t k = 0; k <= 32; ++k) zinv[k] = k ? poly_pow(~0ull - k, p) : 1ull;
+  auto one_run = [](uint32_t fh) {
+    const uint32_t z = ~fh;
+    return ((z + (z & (0u - z))) & z) == 0u;
+  };
+  bool all = true;
+  lead.assign((size_t)wpr * h, 0);
+  for (uint32_t y = 0; y < h; ++y)
+    for (uint32_t j = 0; j < wpr; ++j) {
+      const size_t word = (size_t)y * wpr + j;
+      const uint32_t ncell = (j + 1) * 64 <= w ? 64 : w - j * 64;
+      const uint64_t cells = ncell == 64 ? ~0ull : ((1ull << ncell) - 1);
+      const uint64_t fl = cells & ~solid[word];
+      const uint32_t f0 = (uint32_t)fl, f1 = (uint32_t)(fl >> 32);
+      all = all && one_run(f0) && one_run(f1);
+      const uint32_t len = one_run(f0) ? (uint32_t)__builtin_popcount(~f0) : 0u;
+      lead[word] = len ? poly_mulmod(cjump[2 * word], zinv[len], p) : cjump[2 * word];
+    }
+  return all;
+}
+
 // Window tables of the maps T^(4g), g = 0..15 (T^0 the identity): T^(4g) is
 // linear over GF(2)^64 and the same for every stream and step, so
 //     T^(4g) s = XOR over w = 0..15 of pw[g * 256 + w * 16 + ((s >> 4w) & 15)],

@@ -105,7 +105,7 @@ def build(r: Request, block: int) -> GpuLattice:
 
 def footprint(r: Request, block: int) -> int:
     """Device bytes of one resident run: two plane sets, the solid plane,
-    jump polynomials, word order, the two digit and inflow slots of the
+    jump polynomials (two per word, and the lead jump), word order, the two digit and inflow slots of the
     two-pass step, the stream tables, the field and the byte staging buffer
     of load/store."""
     wpr = (r.w + 63) // 64
@@ -114,7 +114,7 @@ def footprint(r: Request, block: int) -> int:
     nb = -(-r.h // (-(-r.h // min(r.bands, r.h))))
     tab_steps = max(3, min(256, (32 << 20) // ((nb + 1) * 256 * 8)))
     blocks = (r.w // block) * (r.h // block)
-    return (words * (8 * (14 + 1 + 2) + 4 + 80) + iw_h * (16 + 112) + 4 * r.h
+    return (words * (8 * (14 + 1 + 2 + 1) + 4 + 80) + iw_h * (16 + 112) + 4 * r.h
             + tab_steps * (nb + 1) * 2048 + 48 * blocks + r.cells)
 
 

@@ -335,6 +335,62 @@ int hs_draw_run_check(uint64_t seed, int n) {
   return bad;
 }
 
+// The lead jumps (lgca::build_lead_jumps) on a w x h layout of boxes (one
+// run per half where a box edge falls inside a half, plus the row's tail):
+// for every word, the plain chain from its lead jump with lg_run_fix_lead
+// gives the low half's masked digits (lg_draw_masked32 from the half's own
+// jump) and ends on the state the high half starts from (its own jump); the
+// high half's plain chain with lg_run_fix gives its masked digits. Returns
+// the number of mismatches, or -1 if build_lead_jumps did not report the
+// layout as single-run.
+int hs_lead_check(uint32_t w, uint32_t h, uint64_t seed) {
+  const uint32_t wpr = (w + 63) / 64;
+  std::vector<uint64_t> solid((size_t)wpr * h, 0), cj, ij, lead;
+  std::vector<uint32_t> rb, ord;
+  uint64_t x = seed | 1u;
+  // one box row-chord per row, some rows none; clear of the half that holds
+  // a ragged row's tail, so that no half has two runs
+  const uint32_t room = (w % 64) ? 64 * (wpr - 1) + ((w % 64) > 32 ? 32 : 0) : w;
+  for (uint32_t y = 0; y < h && room > 0; ++y) {
+    x = lg_xs(x);
+    if (x % 3 == 0) continue;
+    const uint32_t x0 = (uint32_t)((x >> 8) % room), len = 1 + (uint32_t)((x >> 24) % 9);
+    for (uint32_t c = x0; c < std::min(room, x0 + len); ++c)
+      solid[(size_t)y * wpr + c / 64] |= 1ull << (c % 64);
+  }
+  lgca::build_tables(w, h, 3, 0, true, solid, cj, rb, ij, ord);
+  if (!lgca::build_lead_jumps(w, h, solid, cj, lead)) return -1;
+  {  // two runs in one half: not covered
+    std::vector<uint64_t> s2 = solid, l2;
+    s2[0] |= (1ull << 2) | (1ull << 5);
+    std::vector<uint64_t> cj2;
+    lgca::build_tables(w, h, 3, 0, true, s2, cj2, rb, ij, ord);
+    if (lgca::build_lead_jumps(w, h, s2, cj2, l2)) return -2;
+  }
+  std::vector<uint64_t> tab(LG_WTAB);
+  lg_build_tab(lg_rng_init(seed ^ 0x1EAD), tab.data());
+  int bad = 0;
+  for (uint32_t y = 0; y < h; ++y)
+    for (uint32_t j = 0; j < wpr; ++j) {
+      const size_t word = (size_t)y * wpr + j;
+      const uint32_t ncell = (j + 1) * 64 <= w ? 64 : w - j * 64;
+      const uint64_t cells = ncell == 64 ? ~0ull : ((1ull << ncell) - 1);
+      const uint64_t fm = cells & ~solid[word];
+      uint32_t A[5], B[5];
+      const uint64_t end = lg_draw_half<5>(lg_jump_tab(lead[word], tab.data()), 30, A);
+      lg_run_fix_lead<5>((uint32_t)fm, A);
+      lg_draw_masked32<5>(lg_jump_tab(cj[2 * word], tab.data()), (uint32_t)fm, 30, B);
+      for (int b = 0; b < 5; ++b) bad += A[b] != B[b];
+      const uint64_t hi = lg_jump_tab(cj[2 * word + 1], tab.data());
+      bad += end != hi;
+      lg_draw_half<5>(end, 30, A);
+      lg_run_fix<5>((uint32_t)(fm >> 32), A);
+      lg_draw_masked32<5>(hi, (uint32_t)(fm >> 32), 30, B);
+      for (int b = 0; b < 5; ++b) bad += A[b] != B[b];
+    }
+  return bad;
+}
+
 // The T^(4g) windows (lgca::xs_pow4_windows, lg_xs_pow4) against 4g
 // sequential steps, and a stream's window tables assembled group by group
 // (lg_tab_group) against lg_build_tab, on n pseudo-random states. Returns the

@@ -221,6 +221,16 @@ def test_single_run_draw(hs):
     assert hs.hs_draw_run_check(0x2A11, 20000) == 0
 
 
+@pytest.mark.parametrize("w,h", [(4096, 64), (300, 40), (100, 30), (64, 16), (129, 20)])
+def test_lead_jumps(hs, w, h):
+    """Jumps that start a word's chain as many draws early as its low half
+    has cells without a draw (one run: box chords, the row's tail): the
+    plain chain's digits moved into place equal the masked draw's, and the
+    chain ends where the high half's own jump starts."""
+    hs.hs_lead_check.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64]
+    assert hs.hs_lead_check(w, h, 0x1EAD + w) == 0
+
+
 def test_pow4_windows_reach_group_starts(hs):
     """State 4g of a stream through the T^(4g) windows equals 4g sequential
     xorshift steps, and tables assembled group by group from those states
