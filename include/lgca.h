@@ -194,9 +194,12 @@ int lg_set_temporal_depth(lg_t* g, uint32_t steps, uint32_t rows_per_tile);
  * its first sample) saves more step time than the band launch costs (about
  * 12 us). Results are identical. lg_get_resident returns 1 (and the band
  * height and count) when resident bands are in use for calls of enough
- * steps, else 0. If the runtime declines the cooperative launch, the lattice
- * runs the other pipelines from then on (until lg_set_resident is called
- * again). A wait
+ * steps, else 0. The process's first band launch on a device is
+ * cooperative; if the runtime declines it (it does while another process
+ * that has made one is alive), the lattice runs the other pipelines from
+ * then on (until lg_set_resident is called again). Later band launches are
+ * plain and the library runs them one after another on the device (a launch
+ * from another stream waits for the previous one's stream). A wait
  * between bands that runs out (it cannot hang the device: every wait is
  * bounded) makes the call fail with LG_EDEVICE and the lattice must be
  * reloaded. */

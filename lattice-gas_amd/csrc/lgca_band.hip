@@ -15,8 +15,10 @@
 // stored and loaded as agent-scope atomics (coherent across the GPU's XCDs;
 // see lg_band_wait for the ordering). Every poll is bounded: a wait that runs out records the launch's
 // epoch in an error word and every later wait of that launch returns at
-// once, so the kernel always ends and the host reports the failure. The
-// launch is cooperative, so every band's workgroup is resident at once.
+// once, so the kernel always ends and the host reports the failure. Every
+// band's workgroup must be resident at once: the process's first band launch
+// on a device is cooperative, the later ones plain and ordered one after the
+// other (BandOrder below).
 //
 // Flags count steps from a per-launch epoch that only grows (2^21 apart),
 // so a launch never reads an older launch's counts as its own; halo rows sit
@@ -26,6 +28,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "lgca_internal.h"
@@ -636,6 +639,38 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
 
 namespace lgca_internal {
 
+// Band launches of this process, per device. The grid needs every band's
+// workgroup resident at once. Before the device's first band launch the
+// process makes a cooperative one (of no steps), which the runtime declines
+// while another process that has made one is alive: the process that gets it
+// runs band launches from then on and the other takes the other pipelines.
+// The band launches themselves are plain (the cooperative launch's runtime
+// work costs ~10-15 us a call) and are ordered one after the other on the
+// device: a launch from another stream than the last one waits
+// for that stream's work, so no two band grids of the process compete for
+// CUs. Other kernels that share the device end on their own; band
+// workgroups that wait for CUs meanwhile only delay their neighbours' waits,
+// which are bounded.
+namespace {
+struct BandOrder {
+  std::mutex mu;
+  bool claimed[LG_MAX_DEVICES] = {};
+  hipStream_t last[LG_MAX_DEVICES] = {};  // stream of the latest band launch
+  hipEvent_t mark[LG_MAX_DEVICES] = {};   // recorded there when the stream changes
+};
+BandOrder& band_order() {
+  static BandOrder o;
+  return o;
+}
+}  // namespace
+
+void band_forget_stream(const lg_t* g) {
+  if (g->device < 0 || g->device >= LG_MAX_DEVICES) return;
+  BandOrder& o = band_order();
+  std::lock_guard<std::mutex> lk(o.mu);
+  if (o.last[g->device] == g->stream) o.last[g->device] = nullptr;
+}
+
 bool band_eligible(const lg_t* g, uint32_t* rows, uint32_t* nb) {
   const uint32_t ncu = (uint32_t)std::max(g->ncu, 1);
   if (g->G.wpr > 64) return false;
@@ -782,8 +817,29 @@ int launch_band(lg_t* g, uint32_t n, bool sample, float alpha) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     g->band_lds_raised = true;
   }
-  const hipError_t e = hipLaunchCooperativeKernel(kernels[g->G.wpr == 64][sample][inflow], grid,
-                                                  block, args, lds, g->stream);
+  const void* kf = kernels[g->G.wpr == 64][sample][inflow];
+  if (g->device < 0 || g->device >= LG_MAX_DEVICES) return fail(LG_EINVAL, "device index out of range");
+  BandOrder& o = band_order();
+  std::lock_guard<std::mutex> lk(o.mu);
+  const int dev = g->device;
+  hipError_t e = hipSuccess;
+  if (!o.claimed[dev]) {
+    // the claim: a cooperative launch of no steps (it only copies the state
+    // to the next plane set, which this launch's steps then overwrite), so
+    // that the call's own launch is already a plain one: the first plain
+    // launch on a queue pays the runtime's set-up of its scratch memory
+    LgBandArgs C = A;
+    C.n = 0;
+    void* cargs[] = {&C};
+    e = hipLaunchCooperativeKernel(kernels[g->G.wpr == 64][0][inflow], grid, block, cargs, lds,
+                                   g->stream);
+    if (e == hipSuccess) o.claimed[dev] = true;
+  } else if (o.last[dev] && o.last[dev] != g->stream) {
+    if (!o.mark[dev]) HIP_TRY(hipEventCreateWithFlags(&o.mark[dev], hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(o.mark[dev], o.last[dev]));
+    HIP_TRY(hipStreamWaitEvent(g->stream, o.mark[dev], 0));
+  }
+  if (e == hipSuccess) e = hipLaunchKernel(kf, grid, block, args, lds, g->stream);
   if (e != hipSuccess) {
     // Refused (the runtime can decline a cooperative launch, e.g. while
     // another process holds the device's cooperative-launch resources):
@@ -792,6 +848,7 @@ int launch_band(lg_t* g, uint32_t n, bool sample, float alpha) {
     g->band_refused = true;
     return kBandRefused;
   }
+  o.last[dev] = g->stream;
   g->band_check = true;
   return LG_OK;
 }

@@ -946,7 +946,6 @@ static __device__ __align__(256) float scratch[LG_NOALLOC_SCRATCH_FLOATS];
 
 namespace {
 
-constexpr int LG_MAX_DEVICES = 64;
 std::mutex g_scratch_mu;
 float* g_scratch_ptr[LG_MAX_DEVICES] = {};
 

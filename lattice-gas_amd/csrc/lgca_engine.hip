@@ -579,6 +579,7 @@ void equilibrium_thresholds(float density, float ux, float uy, bool rest, uint32
 void destroy_lattice(lg_lattice* g) {
   (void)hipStreamSynchronize(g->stream);
   if (g->dstream) (void)hipStreamSynchronize(g->dstream);
+  band_forget_stream(g);
   g->a.release();
   g->b.release();
   g->solid.release();

@@ -100,6 +100,8 @@ double band_gain_us(const lg_lattice* g, uint32_t rows);
 #define LG_BAND_SPIN_LIMIT (1u << 20)
 // Steps per resident launch at most (its step counters run below 2^21).
 #define LG_BAND_MAX_CHUNK (1u << 16)
+// Devices the process-wide band launch order keeps track of.
+#define LG_MAX_DEVICES 64
 
 // Arguments of one resident-band launch (lgca_band.hip).
 struct LgBandArgs {
@@ -176,6 +178,9 @@ bool band_sample_fits(const lg_lattice* g);
 constexpr int kBandRefused = 1;
 int launch_band(lg_lattice* g, uint32_t n, bool sample = false, float alpha = 0.0f);
 int check_band(lg_lattice* g);
+// A handle's stream is going away (its work has ended): band launches of
+// other handles no longer order themselves after it.
+void band_forget_stream(const lg_lattice* g);
 
 }  // namespace lgca_internal
 
