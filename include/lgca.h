@@ -56,6 +56,15 @@ void lg_destroy(lg_t* g);
 /* Message of the last failed call on this thread ("" if none). */
 const char* lg_last_error(void);
 
+/* New: give back to the runtime the device and page-locked memory the
+ * library keeps for reuse. Buffers of destroyed lattices and batches are
+ * kept, by size class, and handed to later lattices and batches, because
+ * hipFree waits for the whole device (a serving loop that frees while other
+ * runs step would stall). The cache is also emptied when an allocation
+ * fails, before it is tried again. *freed (may be NULL) receives the bytes
+ * given back. Call it with no other call of the library running. */
+int lg_trim_memory(size_t* freed);
+
 /* GpuLattice::device_name (src/gpu.rs:517-519). */
 int lg_device_name(lg_t* g, char* buf, size_t len);
 
@@ -280,21 +289,27 @@ int lg_batch_advance(lg_batch_t* b, uint64_t n);
  * with fewer steps left stop early); limits has one entry per member. */
 int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits);
 /* lg_batch_advance_limited without the wait (limits may be NULL: n for
- * all). It first waits for work queued earlier for the members (their own
- * calls, async advances of batches), then queues the steps and returns; the
- * members' step counters already count them. A member's own calls that touch
- * the device, and other batches' calls, wait for them first; lg_poll of a
- * member includes them. lg_batch_poll reports whether the batch's queued work
- * has ended (1) or not (0), lg_batch_wait waits for it. */
+ * all). It queues the steps behind the work queued earlier for the members
+ * (their own calls, async advances of this or other batches) -- the batch's
+ * queue waits for that on the device, the host does not -- and returns; the
+ * members' step counters already count them. A serving loop can so queue a
+ * quantum on a batch of another membership before the last one has ended. A
+ * member's own calls that touch the device wait for them first (on the
+ * host), and other batches' calls on the device; lg_poll of a member
+ * includes them. lg_batch_poll reports whether the batch's queued work has
+ * ended (1) or not (0), lg_batch_wait waits for it. */
 int lg_batch_advance_async(lg_batch_t* b, uint64_t n, const uint64_t* limits);
 int lg_batch_poll(lg_batch_t* b);
 int lg_batch_wait(lg_batch_t* b);
 /* lg_batch_advance_async followed, in the same queue, by lg_batch_snapshot's
- * sample of every member's field (n = 0: the sample alone); the result stays
- * on the device until lg_batch_read_snapshot, which waits for the batch and
- * copies it out in lg_batch_snapshot's layout (nfloats as there). Any other
- * advance, snapshot, sample or autotune call of the batch in between discards
- * it (LG_ESTATE); lg_batch_poll and lg_batch_wait keep it. */
+ * sample of every member's field (n = 0: the sample alone), copied on to
+ * page-locked memory in the same queue. Up to two such snapshots wait to be
+ * read (a third sampling advance fails with LG_ESTATE): lg_batch_read_snapshot
+ * waits for the oldest one and copies it out in lg_batch_snapshot's layout
+ * (nfloats as there), so a loop can queue the next quantum before it reads
+ * the last one. A synchronous advance, snapshot, sample or autotune call of
+ * the batch discards unread snapshots; lg_batch_advance_async, lg_batch_poll
+ * and lg_batch_wait keep them. */
 int lg_batch_advance_sampling_async(lg_batch_t* b, uint64_t n, const uint64_t* limits,
                                     float alpha);
 int lg_batch_read_snapshot(lg_batch_t* b, float* out, size_t nfloats);

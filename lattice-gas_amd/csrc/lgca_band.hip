@@ -692,7 +692,7 @@ int ensure_band_buffers(lg_t* g, uint32_t nb) {
     HIP_TRY(hipMemsetAsync(g->band_err_dev.p, 0, 8, g->stream));
   }
   if (!g->band_err_host) {
-    HIP_TRY(hipHostMalloc((void**)&g->band_err_host, 8, hipHostMallocDefault));
+    HIP_TRY(host_alloc((void**)&g->band_err_host, 8));
     *g->band_err_host = 0;
   }
   if (!g->band_pw.p) {

@@ -224,12 +224,23 @@ struct lg_batch {
   hipGraphExec_t exec = nullptr;
   uint32_t graph_chunk = 0, graph_tile = 0, graph_grid = 0, graph_key = 0, graph_waves = 0;
   int ncu = 256;
-  // host staging of the last uploads, alive until the stream has consumed them
-  std::vector<LgDesc> host_desc;
-  std::vector<uint64_t> advanced;  // per member, steps taken by the last call
-  DevBuf<float> staging;           // snapshots of every member's field
+  // Descriptor uploads from two page-locked host slots in turn: a slot is
+  // filled again only once the copy from it has run (its event), so an
+  // upload never waits for the advance queued before it.
+  LgDesc* hdesc[2] = {nullptr, nullptr};
+  hipEvent_t desc_ev[2] = {nullptr, nullptr};
+  bool desc_rec[2] = {false, false};
+  uint32_t dslot = 0;                // slot of the latest upload
+  std::vector<uint64_t> advanced;    // per member, steps taken by the last call
+  // Snapshots of sampling advances: two slots (device staging, page-locked
+  // copy, event after the copy), read back oldest first.
+  DevBuf<float> staging[2];
+  float* pin[2] = {nullptr, nullptr};
+  size_t pin_bytes[2] = {0, 0};
+  hipEvent_t snap_ev[2] = {nullptr, nullptr};
+  uint32_t stage_head = 0, stage_count = 0;
   size_t staging_floats = 0;
-  bool staged = false;             // staging holds the snapshot of the last sampling advance
+  hipEvent_t own_ev = nullptr;       // marks a member's own stream for the batch to wait on
   uint32_t max_field_blocks = 0;
   uint32_t calls = 0;              // advance calls on this membership
   std::vector<uint2> host_tiles128, host_tiles256;
@@ -250,14 +261,11 @@ uint32_t pipeline_key(const lg_batch* b) {
   return b->passes * 100 + b->units * 10 + (b->passes == 2 ? b->draw_units : 0);
 }
 
-// Fill the host copy of the descriptors (no device work, no allocation: the
-// vector keeps the capacity reserved at batch creation). The last upload
-// from host_desc must have been consumed (the batch stream synchronised).
-void fill_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
-  std::vector<LgDesc>& h = b->host_desc;
-  h.assign(b->members.size(), LgDesc{});
+// Fill host descriptor slot h (no device work, no allocation: the slots are
+// page-locked buffers made at batch creation).
+void fill_desc(lg_batch* b, LgDesc* h, const uint64_t* limits, uint64_t n) {
   size_t off = 0;
-  for (size_t i = 0; i < h.size(); ++i) {
+  for (size_t i = 0; i < b->members.size(); ++i) {
     lg_t* g = b->members[i];
     LgDesc& d = h[i];
     std::memset(&d, 0, sizeof(d));
@@ -295,18 +303,28 @@ void fill_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
   b->staging_floats = off;
 }
 
+// Queue the descriptors of a call on the batch stream, from the host slot
+// not used by the latest upload (waiting only for the copy from that slot
+// two uploads ago). Allocates nothing.
+int queue_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
+  const uint32_t slot = b->dslot ^ 1u;
+  if (b->desc_rec[slot]) HIP_TRY(hipEventSynchronize(b->desc_ev[slot]));
+  fill_desc(b, b->hdesc[slot], limits, n);
+  HIP_TRY(hipMemcpyAsync(b->desc.p, b->hdesc[slot], b->members.size() * sizeof(LgDesc),
+                         hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipEventRecord(b->desc_ev[slot], b->stream));
+  b->desc_rec[slot] = true;
+  b->dslot = slot;
+  return LG_OK;
+}
+
 int upload_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
   for (lg_t* g : b->members) {
     int rc = ensure_tables(g);
     if (rc) return rc;
     if ((rc = ensure_draw_buffers(g))) return rc;  // the pipeline may change within a call
   }
-  // the last upload from host_desc may still be in flight after an async advance
-  HIP_TRY(hipStreamSynchronize(b->stream));
-  fill_desc(b, limits, n);
-  HIP_TRY(hipMemcpyAsync(b->desc.p, b->host_desc.data(), b->host_desc.size() * sizeof(LgDesc),
-                         hipMemcpyHostToDevice, b->stream));
-  return LG_OK;
+  return queue_desc(b, limits, n);
 }
 
 // Tiles of TILE units: for the fused step, every member's units; for the
@@ -549,6 +567,52 @@ static int settle_members(lg_batch* b) {
   return LG_OK;
 }
 
+// The same order kept on the device instead, for a call that queues work and
+// returns: the batch stream waits for the async batch advance that last
+// stepped each member (one wait per distinct advance: usually the previous
+// batch's) and for work still queued on a member's own stream. The host does
+// not wait, so a serving loop can queue the next quantum, on a batch of
+// another membership, before the last one has ended.
+static int order_members(lg_batch* b) {
+  hipEvent_t waited[8];
+  int nw = 0;
+  for (lg_t* g : b->members) {
+    if (g->poisoned)
+      return fail(LG_ESTATE, "the lattice's state is unknown after a failed batch call; reload it");
+    if (g->after && g->after != b->done) {
+      bool seen = false;
+      for (int i = 0; i < nw; ++i) seen = seen || waited[i] == g->after;
+      if (!seen) {
+        HIP_TRY(hipStreamWaitEvent(b->stream, g->after, 0));
+        if (nw < 8) waited[nw++] = g->after;
+      }
+    }
+    const hipError_t q = hipStreamQuery(g->stream);
+    if (q == hipErrorNotReady) {
+      HIP_TRY(hipEventRecord(b->own_ev, g->stream));
+      HIP_TRY(hipStreamWaitEvent(b->stream, b->own_ev, 0));
+    } else if (q != hipSuccess) {
+      return fail(LG_EDEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(q));
+    }
+  }
+  return LG_OK;
+}
+
+// The batch's page-locked slots and events (its stream has drained).
+static void release_host_side(lg_batch* b) {
+  for (int q = 0; q < 2; ++q) {
+    if (b->hdesc[q]) host_release(b->hdesc[q]);
+    if (b->pin[q]) host_release(b->pin[q]);
+    if (b->desc_ev[q]) (void)hipEventDestroy(b->desc_ev[q]);
+    if (b->snap_ev[q]) (void)hipEventDestroy(b->snap_ev[q]);
+    b->hdesc[q] = nullptr;
+    b->pin[q] = nullptr;
+    b->desc_ev[q] = b->snap_ev[q] = nullptr;
+  }
+  if (b->own_ev) (void)hipEventDestroy(b->own_ev);
+  b->own_ev = nullptr;
+}
+
 extern "C" {
 
 lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
@@ -559,6 +623,7 @@ lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
   auto* b = new lg_batch();
   auto bail = [&](const std::string& m, int code = LG_EDEVICE) -> lg_batch_t* {
     fail(code, m);
+    release_host_side(b);
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -601,7 +666,17 @@ lg_batch_t* lg_batch_create(lg_t* const* members, size_t n) {
   // against 920 GCUPS; served batches: 591-597 against 572-595).
   b->tile = b->passes == 1 ? 128u : 256u;
   if (b->desc.alloc(n)) return bail("hipMalloc failed for descriptors");
-  b->host_desc.reserve(n);
+  for (int q = 0; q < 2; ++q) {
+    if (host_alloc((void**)&b->hdesc[q], n * sizeof(LgDesc)) != hipSuccess) {
+      b->hdesc[q] = nullptr;
+      return bail("page-locked allocation failed for descriptors");
+    }
+    if (hipEventCreateWithFlags(&b->desc_ev[q], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->snap_ev[q], hipEventDisableTiming) != hipSuccess)
+      return bail("hipEventCreate failed");
+  }
+  if (hipEventCreateWithFlags(&b->own_ev, hipEventDisableTiming) != hipSuccess)
+    return bail("hipEventCreate failed");
   if (ensure_tiles(b)) return bail(g_err);
   if (hipStreamSynchronize(b->stream) != hipSuccess) return bail("hipStreamSynchronize failed");
   for (lg_t* g : b->members) g->batch_refs += 1;
@@ -615,6 +690,9 @@ void lg_batch_destroy(lg_batch_t* b) {
   b->desc.release();
   b->tiles128.release();
   b->tiles256.release();
+  b->staging[0].release();
+  b->staging[1].release();
+  release_host_side(b);
   for (lg_t* g : b->members)
     if (g->after == b->done) g->after = nullptr;  // ended: the stream was synchronised
   if (b->done) (void)hipEventDestroy(b->done);
@@ -626,7 +704,7 @@ void lg_batch_destroy(lg_batch_t* b) {
 
 int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits) {
   if (!b) return fail(LG_EINVAL, "null batch");
-  b->staged = false;
+  b->stage_count = 0;  // unread snapshots are discarded
   if (int rc = members_alive(b)) return rc;
   if (n == 0) return LG_OK;
   if (int rc = settle_members(b)) return rc;
@@ -645,45 +723,68 @@ int lg_batch_advance_limited(lg_batch_t* b, uint64_t n, const uint64_t* limits) 
 int lg_batch_advance(lg_batch_t* b, uint64_t n) { return lg_batch_advance_limited(b, n, nullptr); }
 
 // Queue an advance, and with `sample` one sample of every member's field at
-// its end into the staging buffer (the descriptors the advance leaves on the
-// device name each member's current planes).
+// its end into the next snapshot slot (the descriptors the advance leaves on
+// the device name each member's current planes), copied on to the slot's
+// page-locked buffer on the batch stream. Nothing waits on the host: the
+// batch stream waits on the device for the members' earlier work
+// (order_members), and the descriptors go up from the host slot the last
+// call did not use, so a second advance can be queued behind this one.
 static int advance_async(lg_batch* b, uint64_t n, const uint64_t* limits, bool sample,
                          float alpha) {
-  b->staged = false;
   if (int rc = members_alive(b)) return rc;
   if (n == 0 && !sample) return LG_OK;
-  // work queued earlier on the members' own streams, or by an earlier async
-  // advance of this or another batch, ends first (the host waits for it)
-  if (int rc = settle_members(b)) return rc;
+  if (sample && b->stage_count == 2)
+    return fail(LG_ESTATE, "two snapshots wait to be read (lg_batch_read_snapshot) first");
+  if (int rc = order_members(b)) return rc;
   int rc = upload_desc(b, limits, n);
   if (rc) return rc;
   // everything that can fail without device work queued comes first: the
-  // staging buffer is sized by upload_desc and allocated before any step
-  if (sample && b->staging_floats && b->staging.n < b->staging_floats)
-    HIP_TRY(b->staging.alloc(b->staging_floats));
+  // snapshot slot is sized by upload_desc and allocated before any step
+  const uint32_t slot = (b->stage_head + b->stage_count) & 1u;
+  const size_t bytes = b->staging_floats * 4;
+  if (sample && bytes) {
+    if (b->staging[slot].n < b->staging_floats) HIP_TRY(b->staging[slot].alloc(b->staging_floats));
+    if (b->pin_bytes[slot] < bytes) {
+      if (b->pin[slot]) host_release(b->pin[slot]);
+      b->pin[slot] = nullptr;
+      b->pin_bytes[slot] = 0;
+      HIP_TRY(host_alloc((void**)&b->pin[slot], bytes));
+      b->pin_bytes[slot] = bytes;
+    }
+  }
   if (n) {
     rc = run_steps(b, n, b->calls > 0);
     if (rc) return abandon_queued(b, rc);
   }
-  if (sample && b->staging_floats) {
+  if (sample && bytes) {
     uint32_t maxnb = 0;
     for (lg_t* g : b->members)
       if (g->has_field) maxnb = std::max(maxnb, g->bw * g->bh);
     dim3 grid(blocks_for(maxnb, 4), (uint32_t)b->members.size());
     hipLaunchKernelGGL(lg_snapshot_kernel, grid, dim3(256), 0, b->stream, b->desc.p, alpha,
-                       b->staging.p);
+                       b->staging[slot].p);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(b->pin[slot], b->staging[slot].p, bytes, hipMemcpyDeviceToHost, b->stream);
     if (e != hipSuccess)
-      return abandon_queued(b, fail(LG_EDEVICE, std::string("snapshot launch: ") + hipGetErrorString(e)));
+      return abandon_queued(b, fail(LG_EDEVICE, std::string("snapshot: ") + hipGetErrorString(e)));
   }
-  // a member's own calls, and its next batch, wait for this advance first
-  // (settle): no wait is queued on the members' streams, which would hold up
-  // unrelated work sharing their hardware queues
-  HIP_TRY(hipEventRecord(b->done, b->stream));
+  if (sample) {
+    hipError_t e = hipEventRecord(b->snap_ev[slot], b->stream);
+    if (e != hipSuccess)
+      return abandon_queued(b, fail(LG_EDEVICE, std::string("hipEventRecord: ") + hipGetErrorString(e)));
+    b->stage_count += 1;
+  }
+  // a member's own calls wait for this advance first (settle), and a later
+  // batch's stream on the device (order_members): no wait is queued on the
+  // members' streams, which would hold up unrelated work sharing their
+  // hardware queues
+  hipError_t e = hipEventRecord(b->done, b->stream);
+  if (e != hipSuccess)
+    return abandon_queued(b, fail(LG_EDEVICE, std::string("hipEventRecord: ") + hipGetErrorString(e)));
   for (lg_t* g : b->members) g->after = b->done;
   commit_members(b, limits, n);
   if (n) b->calls += 1;
-  b->staged = sample;
   return LG_OK;
 }
 
@@ -700,11 +801,14 @@ int lg_batch_advance_sampling_async(lg_batch_t* b, uint64_t n, const uint64_t* l
 
 int lg_batch_read_snapshot(lg_batch_t* b, float* out, size_t nfloats) {
   if (!b || (!out && nfloats)) return fail(LG_EINVAL, "null argument");
-  if (!b->staged) return fail(LG_ESTATE, "no sampling advance since the last batch call");
+  if (!b->stage_count) return fail(LG_ESTATE, "no unread snapshot of a sampling advance");
   if (nfloats != b->staging_floats)
     return fail(LG_EINVAL, "output size differs from the members' fields (4 x blocks each)");
-  if (nfloats) HIP_TRY(hipMemcpyAsync(out, b->staging.p, nfloats * 4, hipMemcpyDeviceToHost, b->stream));
-  HIP_TRY(hipStreamSynchronize(b->stream));
+  const uint32_t slot = b->stage_head;
+  HIP_TRY(hipEventSynchronize(b->snap_ev[slot]));
+  if (nfloats) std::memcpy(out, b->pin[slot], nfloats * 4);
+  b->stage_head ^= 1u;
+  b->stage_count -= 1;
   return LG_OK;
 }
 
@@ -742,7 +846,7 @@ extern "C" {
 
 int lg_batch_snapshot(lg_batch_t* b, float alpha, float* out, size_t nfloats) {
   if (!b || !out) return fail(LG_EINVAL, "null argument");
-  b->staged = false;
+  b->stage_count = 0;  // unread snapshots are discarded
   if (int rc = members_alive(b)) return rc;
   if (int rc = settle_members(b)) return rc;
   int rc = upload_desc(b, nullptr, 0);
@@ -750,8 +854,8 @@ int lg_batch_snapshot(lg_batch_t* b, float alpha, float* out, size_t nfloats) {
   if (nfloats != b->staging_floats)
     return fail(LG_EINVAL, "output size differs from the members' fields (4 x blocks each)");
   if (nfloats == 0) return LG_OK;
-  if (b->staging.n < nfloats) HIP_TRY(b->staging.alloc(nfloats));
-  return snapshot_into(b, alpha, b->staging.p, out, nfloats);
+  if (b->staging[0].n < nfloats) HIP_TRY(b->staging[0].alloc(nfloats));
+  return snapshot_into(b, alpha, b->staging[0].p, out, nfloats);
 }
 
 size_t lg_batch_snapshot_size(lg_batch_t* b) {
@@ -837,7 +941,7 @@ int lg_batch_get_shape(lg_batch_t* b, uint32_t* tile, uint32_t* grid, uint32_t* 
 int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps) {
   if (!b || !used_steps) return fail(LG_EINVAL, "null argument");
   *used_steps = 0;
-  b->staged = false;
+  b->stage_count = 0;  // unread snapshots are discarded
   if (int rc = members_alive(b)) return rc;
   if (int rc = settle_members(b)) return rc;
   int rc = upload_desc(b, nullptr, ~0ull);
@@ -917,7 +1021,7 @@ int lg_batch_autotune(lg_batch_t* b, uint64_t budget_steps, uint64_t* used_steps
 // on the members' own streams, and wait.
 int lg_batch_sample(lg_batch_t* b, float alpha) {
   if (!b) return fail(LG_EINVAL, "null batch");
-  b->staged = false;
+  b->stage_count = 0;  // unread snapshots are discarded
   if (int rc = members_alive(b)) return rc;
   for (lg_t* g : b->members) {
     int rc = lg_sample_field_async(g, alpha);
@@ -977,21 +1081,18 @@ size_t lg_batch_snapshot_noalloc_max(void) { return LG_NOALLOC_SCRATCH_FLOATS; }
 
 int lg_batch_snapshot_noalloc(lg_batch_t* b, float alpha, float* out, size_t nfloats) {
   if (!b || !out) return fail(LG_EINVAL, "null argument");
-  b->staged = false;
+  b->stage_count = 0;  // unread snapshots are discarded
   if (int rc = members_alive(b)) return rc;
   float* staging = nullptr;
   if (int rc = scratch_on(b->members[0]->device, &staging)) return rc;
   if (int rc = settle_members(b)) return rc;
-  HIP_TRY(hipStreamSynchronize(b->stream));  // host_desc's last upload consumed
-  fill_desc(b, nullptr, 0);
+  if (int rc = queue_desc(b, nullptr, 0)) return rc;
   if (nfloats != b->staging_floats)
     return fail(LG_EINVAL, "output size differs from the members' fields (4 x blocks each)");
   if (nfloats > LG_NOALLOC_SCRATCH_FLOATS)
     return fail(LG_EINVAL, "the members' fields exceed the allocation-free snapshot's " +
                                std::to_string(LG_NOALLOC_SCRATCH_FLOATS) + " floats");
   if (nfloats == 0) return LG_OK;
-  HIP_TRY(hipMemcpyAsync(b->desc.p, b->host_desc.data(), b->host_desc.size() * sizeof(LgDesc),
-                         hipMemcpyHostToDevice, b->stream));
   return snapshot_into(b, alpha, staging, out, nfloats);
 }
 

@@ -17,8 +17,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "lgca_internal.h"
@@ -547,6 +550,117 @@ namespace lgca_internal {
 
 thread_local std::string g_err;
 
+namespace {
+
+struct MemCache {
+  std::mutex mu;
+  struct Block {
+    int dev;       // -1: page-locked host memory
+    size_t bytes;  // the size class
+  };
+  std::unordered_map<void*, Block> live;
+  std::map<std::pair<int, size_t>, std::vector<void*>> free;  // (device, class) -> blocks
+  size_t cached = 0;                                          // bytes on the free lists
+};
+MemCache& mem_cache() {
+  static MemCache* c = new MemCache();  // never destroyed: handles may outlive static teardown
+  return *c;
+}
+// Cached bytes above which a released block is freed at once instead.
+constexpr size_t kCacheMax = 16ull << 30;
+
+size_t size_class(size_t bytes) {
+  if (bytes <= 256) return 256;
+  if (bytes < (1u << 20)) {
+    size_t c = 512;
+    while (c < bytes) c <<= 1;
+    return c;
+  }
+  const size_t m = 2u << 20;
+  return (bytes + m - 1) / m * m;
+}
+
+void raw_free(int dev, void* p) {
+  if (dev < 0)
+    (void)hipHostFree(p);
+  else
+    (void)hipFree(p);
+}
+
+// Empty the free lists of `dev` (every device and the host when dev is -2).
+size_t trim_locked(MemCache& c, int dev) {
+  size_t got = 0;
+  for (auto it = c.free.begin(); it != c.free.end();) {
+    if (dev != -2 && it->first.first != dev) {
+      ++it;
+      continue;
+    }
+    for (void* p : it->second) {
+      raw_free(it->first.first, p);
+      got += it->first.second;
+    }
+    c.cached -= it->first.second * it->second.size();
+    it = c.free.erase(it);
+  }
+  return got;
+}
+
+hipError_t cached_alloc(void** p, size_t bytes, bool host) {
+  int dev = -1;
+  if (!host) {
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+  }
+  const size_t cls = size_class(bytes);
+  MemCache& c = mem_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.free.find({dev, cls});
+  if (it != c.free.end() && !it->second.empty()) {
+    *p = it->second.back();
+    it->second.pop_back();
+    c.cached -= cls;
+    c.live[*p] = {dev, cls};
+    return hipSuccess;
+  }
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    hipError_t e = host ? hipHostMalloc(p, cls, hipHostMallocDefault) : hipMalloc(p, cls);
+    if (e == hipSuccess) {
+      c.live[*p] = {dev, cls};
+      return hipSuccess;
+    }
+    (void)hipGetLastError();
+    if (attempt == 0 && trim_locked(c, dev) == 0) return e;  // nothing cached to give back
+  }
+  return hipErrorOutOfMemory;
+}
+
+void cached_release(void* p) {
+  MemCache& c = mem_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.live.find(p);
+  if (it == c.live.end()) return;
+  const MemCache::Block b = it->second;
+  c.live.erase(it);
+  if (c.cached + b.bytes > kCacheMax) {
+    raw_free(b.dev, p);
+    return;
+  }
+  c.free[{b.dev, b.bytes}].push_back(p);
+  c.cached += b.bytes;
+}
+
+}  // namespace
+
+hipError_t dev_alloc(void** p, size_t bytes) { return cached_alloc(p, bytes, false); }
+void dev_release(void* p) { cached_release(p); }
+hipError_t host_alloc(void** p, size_t bytes) { return cached_alloc(p, bytes, true); }
+void host_release(void* p) { cached_release(p); }
+size_t trim_caches() {
+  MemCache& c = mem_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  return trim_locked(c, -2);
+}
+
 // Equilibrium thresholds exactly as src/lattice.rs:59-82 computes them: each
 // f32 operation rounded on its own, in source order.
 uint32_t threshold_f32(float p) {
@@ -596,7 +710,7 @@ void destroy_lattice(lg_lattice* g) {
   g->band_part.release();
   g->band_bcnt.release();
   g->band_field.release();
-  if (g->band_err_host) (void)hipHostFree(g->band_err_host);
+  if (g->band_err_host) host_release(g->band_err_host);
   g->band_err_host = nullptr;
   g->fux.release();
   g->fuy.release();
@@ -1016,6 +1130,12 @@ int lgca_internal::ensure_tables(lg_t* g) {
 extern "C" {
 
 const char* lg_last_error(void) { return g_err.c_str(); }
+
+int lg_trim_memory(size_t* freed) {
+  const size_t got = trim_caches();
+  if (freed) *freed = got;
+  return LG_OK;
+}
 
 lg_t* lg_create(size_t w, size_t h, int rest, uint64_t seed, size_t bands) {
   if (h % 2 != 0) {

@@ -22,6 +22,24 @@ namespace lgca_internal {
       return lgca_internal::fail(LG_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Device memory for the library's buffers goes through a per-device cache:
+// a released block joins a free list of its size class (powers of two up to
+// 1 MiB, then multiples of 2 MiB) and the next request of that class takes
+// it back. hipFree waits for the whole device, so handles that come and go
+// while other work runs (a serving loop's lattices and batches) would
+// otherwise stall it. A block is released only after every stream that used
+// it has drained (a handle's or batch's own stream; a lattice a batch holds
+// is freed by the batch's destroy), so reuse never overlaps its last use.
+// When hipMalloc fails the cache is emptied and the request tried again;
+// lg_trim_memory empties it on request.
+hipError_t dev_alloc(void** p, size_t bytes);
+void dev_release(void* p);
+// Page-locked host memory, cached the same way (hipHostFree waits too).
+hipError_t host_alloc(void** p, size_t bytes);
+void host_release(void* p);
+// Free every cached block now; returns the bytes given back.
+size_t trim_caches();
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
@@ -29,7 +47,7 @@ struct DevBuf {
   hipError_t alloc(size_t count) {
     release();
     if (count == 0) return hipSuccess;
-    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    hipError_t e = dev_alloc((void**)&p, count * sizeof(T));
     if (e == hipSuccess)
       n = count;
     else
@@ -37,7 +55,7 @@ struct DevBuf {
     return e;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) dev_release(p);
     p = nullptr;
     n = 0;
   }

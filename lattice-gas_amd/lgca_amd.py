@@ -42,6 +42,7 @@ def load_library(path: str | None = None):
         "lg_create": (vp, [sz, sz, i32, u64, sz]),
         "lg_destroy": (None, [vp]),
         "lg_last_error": (C.c_char_p, []),
+        "lg_trim_memory": (i32, [P(sz)]),
         "lg_device_name": (i32, [vp, C.c_char_p, sz]),
         "lg_set_inlet": (i32, [vp, sz, P(C.c_uint32)]),
         "lg_set_inlet_flow": (i32, [vp, sz, f32, f32, f32]),
@@ -498,6 +499,14 @@ class Batch:
         used = C.c_uint64()
         _check(load_library().lg_batch_autotune(self._p, int(budget_steps), C.byref(used)))
         return int(used.value)
+
+
+def trim_memory() -> int:
+    """Give the device and page-locked memory the library keeps for reuse
+    back to the runtime (lg_trim_memory); returns the bytes given back."""
+    v = C.c_size_t()
+    _check(load_library().lg_trim_memory(C.byref(v)))
+    return int(v.value)
 
 
 def measure_transport(density: float, rest: bool, threads: int, seed: int, size: int = 256):

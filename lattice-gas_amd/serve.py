@@ -140,7 +140,7 @@ def _summary(trace, results, t_first, t_last, mode):
 
 def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
                   hbm_budget: int = 200 * GIB, keep_cells: bool = False, shape=None,
-                  alone_max: int = 2):
+                  alone_max: int = 2, depth: int = 2):
     """Continuous batching over the trace; returns (summary, results).
 
     Time is cut into quanta of `every` steps. At each quantum boundary the
@@ -150,135 +150,135 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
     A run joins at a boundary, so its snapshots fall every `every` of its own
     steps and at its end, exactly as in a run of its own.
 
-    The advance and the snapshot behind it are queued without waiting
-    (Batch.advance_sampling_async), and while the device runs them the host
-    frees the lattices retired at the last
-    boundary (a device free waits for the device, so this takes the rest of
-    the advance) and then builds the runs that have arrived; those join at the
-    next boundary. Both touch only lattices outside the batch, and a retired
-    lattice's bytes count against the budget until it is freed.
+    Quanta are queued `depth` deep (Batch.advance_sampling_async): the next
+    quantum is queued -- on a new batch when the membership changes, whose
+    queue waits for the last one on the device -- before the host waits for
+    the oldest one's snapshot, so the device has the next quantum while the
+    host reads snapshots, retires runs and builds arrivals. A run's steps
+    count when they are queued; it is retired when its last quantum is read.
+    Finished runs' lattices are destroyed at once (their memory goes back to
+    the library's cache; a batch still holding one frees it with the batch),
+    and their bytes leave the budget.
 
-    With at most `alone_max` runs on the device (the start and the tail of a
-    stream), runs whose lattices take resident bands advance on their own
-    launches instead of in a batch: for so few lattices those beat the batch
-    kernel (4096 x 2304: 10.3 us per step resident against ~17 us in a batch
-    of one)."""
+    With at most `alone_max` runs on the device and nothing queued (the start
+    and the tail of a stream), runs whose lattices take resident bands
+    advance on their own launches instead of in a batch: for so few lattices
+    those beat the batch kernel (4096 x 2304: ~8.5 us per step resident
+    against ~17 us in a batch of one)."""
     pending = deque(sorted(trace, key=lambda r: r.arrival))
     results = {r.rid: Result(r.rid, r.arrival) for r in trace}
-    active = []             # dicts: req, g, done, bytes
-    ready = []              # built, not yet in the batch
-    retired = []            # (lattice, bytes) of finished runs, not yet freed
-    used = 0                # bytes of every built lattice not yet freed
-    batch = None
+    runs = []               # dicts: req, g, sched (steps queued), done (steps read back), bytes
+    used = 0                # bytes of the runs' lattices
+    inflight = deque()      # queued quanta: (batch, runs, limits)
+    cur, cur_ids = None, None   # batch of the latest queued quantum and its members
     t0 = time.perf_counter()
     clock = lambda: time.perf_counter() - t0
     quanta = 0
     peak = 0
     spent = {"admit": 0.0, "batch_setup": 0.0, "advance": 0.0, "snapshot": 0.0,
-             "retire": 0.0, "idle": 0.0,            # wall seconds by activity
-             "free_during_advance": 0.0, "admit_during_advance": 0.0}
-
-    def timed(key, fn, *a):
-        t = time.perf_counter()
-        r = fn(*a)
-        spent[key] += time.perf_counter() - t
-        return r
+             "retire": 0.0, "idle": 0.0}  # wall seconds by activity
 
     def admit_arrived():
         nonlocal used
+        t = time.perf_counter()
         while pending and pending[0].arrival <= clock():
             r = pending[0]
             fp = footprint(r, block)
-            if used + fp > hbm_budget and (active or ready or retired):
+            if used + fp > hbm_budget and runs:
                 break                                # wait for memory to free up
             pending.popleft()
             g = build(r, block)
             used += fp
             results[r.rid].start = clock()
-            ready.append({"req": r, "g": g, "done": 0, "bytes": fp})
+            runs.append({"req": r, "g": g, "sched": 0, "done": 0, "bytes": fp})
+        spent["admit"] += time.perf_counter() - t
 
-    def free_retired():
-        nonlocal used
-        while retired:
-            g, fp = retired.pop()
-            g.close()
-            used -= fp
+    def close_batch(b):
+        if b is not None and b is not cur and all(q[0] is not b for q in inflight):
+            b.close()
 
-    while pending or active or ready:
-        timed("admit", admit_arrived)
-        changed = bool(ready)
-        active.extend(ready)
-        ready.clear()
-        if not active:
-            timed("retire", free_retired)
-            if pending:
-                timed("idle", time.sleep, max(0.0, min(pending[0].arrival - clock(), 0.01)))
-            continue
-        peak = max(peak, len(active))
-        limits = [a["req"].steps - a["done"] for a in active]
-        if len(active) <= alone_max and all(a["g"].resident() is not None for a in active):
-            # a few runs left (the start and the tail of a stream): each one's
-            # own resident-band launches beat the batch kernel over so few
-            # lattices; same steps, same snapshot (alpha 1: the field is the
-            # coarse-grain of the quantum's last state)
-            if batch is not None:
-                batch.close()
-                batch = None
-            ta = time.perf_counter()
-            for a, lim in zip(active, limits):
-                a["g"].advance_async(min(every, lim), every, 1.0)
-            try:
-                timed("free_during_advance", free_retired)
-                timed("admit_during_advance", admit_arrived)
-            finally:
-                for a in active:
+    while pending or runs or inflight:
+        admit_arrived()
+        nxt = [a for a in runs if a["sched"] < a["req"].steps]
+        peak = max(peak, len(runs))
+        if nxt and len(inflight) < depth:
+            limits = [min(every, a["req"].steps - a["sched"]) for a in nxt]
+            if (not inflight and len(nxt) <= alone_max
+                    and all(a["g"].resident() is not None for a in nxt)):
+                # a few runs left (the start and the tail of a stream): each
+                # one's own resident-band launches beat the batch kernel over
+                # so few lattices; same steps, same snapshot (alpha 1: the
+                # field is the coarse-grain of the quantum's last state)
+                old, cur, cur_ids = cur, None, None
+                close_batch(old)
+                ta = time.perf_counter()
+                for a, lim in zip(nxt, limits):
+                    a["g"].advance_async(lim, every, 1.0)
+                for a in nxt:
                     a["g"].wait()
-            spent["advance"] += time.perf_counter() - ta
-            quanta += 1
-            snaps = [timed("snapshot", a["g"].read_field) for a in active]
-        else:
-            snaps = None
-        if snaps is None and (changed or batch is None):
-            ts = time.perf_counter()
-            if batch is not None:
-                batch.close()
-            batch = Batch([a["g"] for a in active])
-            if shape:
-                batch.set_shape(*shape)
-            spent["batch_setup"] += time.perf_counter() - ts
+                spent["advance"] += time.perf_counter() - ta
+                quanta += 1
+                ts = time.perf_counter()
+                snaps = [a["g"].read_field() for a in nxt]
+                spent["snapshot"] += time.perf_counter() - ts
+                for a, lim in zip(nxt, limits):
+                    a["sched"] += lim
+                inflight.append((None, nxt, limits, snaps))
+            else:
+                ids = [a["req"].rid for a in nxt]
+                if ids != cur_ids:
+                    ts = time.perf_counter()
+                    old, cur = cur, Batch([a["g"] for a in nxt])
+                    cur_ids = ids
+                    if shape:
+                        cur.set_shape(*shape)
+                    close_batch(old)
+                    spent["batch_setup"] += time.perf_counter() - ts
+                ta = time.perf_counter()
+                cur.advance_sampling_async(every, limits, 1.0)
+                spent["advance"] += time.perf_counter() - ta
+                quanta += 1
+                for a, lim in zip(nxt, limits):
+                    a["sched"] += lim
+                inflight.append((cur, nxt, limits, None))
+                if len(inflight) < depth:
+                    continue                         # queue the next quantum before reading
+        if not inflight:
+            if pending:
+                t = time.perf_counter()
+                time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
+                spent["idle"] += time.perf_counter() - t
+            continue
+        b, members, limits, snaps = inflight.popleft()
         if snaps is None:
-            ta = time.perf_counter()
-            batch.advance_sampling_async(every, limits, 1.0)
-            try:
-                timed("free_during_advance", free_retired)
-                timed("admit_during_advance", admit_arrived)
-            finally:
-                batch.wait()            # the batch must be idle before anything else
-            spent["advance"] += time.perf_counter() - ta
-            quanta += 1
-            snaps = timed("snapshot", batch.read_snapshot)
-        for a, lim, snap in zip(active, limits, snaps):
-            a["done"] += min(every, lim)
-            results[a["req"].rid].snapshots.append((a["done"],) + snap)
-        finished = [a for a in active if a["done"] >= a["req"].steps]
+            ts = time.perf_counter()
+            snaps = b.read_snapshot()                # waits for this quantum only
+            spent["snapshot"] += time.perf_counter() - ts
         tr = time.perf_counter()
+        for a, lim, snap in zip(members, limits, snaps):
+            a["done"] += lim
+            results[a["req"].rid].snapshots.append((a["done"],) + tuple(snap))
+        finished = [a for a in members if a["done"] >= a["req"].steps]
+        for a in finished:
+            res = results[a["req"].rid]
+            if keep_cells:
+                res.final_cells = a["g"].store()
+            res.done = clock()
+            a["g"].close()                           # a batch that holds it frees it later
+            used -= a["bytes"]
         if finished:
-            for a in finished:
-                res = results[a["req"].rid]
-                if keep_cells:
-                    res.final_cells = a["g"].store()
-                res.done = clock()
-                retired.append((a["g"], a["bytes"]))
-            active = [a for a in active if a["done"] < a["req"].steps]
-            if batch is not None:
-                batch.close()
-                batch = None
+            gone = {id(a) for a in finished}
+            runs = [a for a in runs if id(a) not in gone]
+        close_batch(b)
         spent["retire"] += time.perf_counter() - tr
-    free_retired()
+    close_batch(cur)
+    if cur is not None:
+        cur.close()
     t_last = max(res.done for res in results.values())
     s = _summary(trace, results, 0.0, t_last, "batched")
     s["quanta"] = quanta
     s["peak_resident"] = peak
+    s["queue_depth"] = depth
     s["wall_s_by_activity"] = {k: round(v, 4) for k, v in spent.items()}
     return s, results
 
@@ -303,11 +303,13 @@ def serve_plain(trace: list[Request], every: int = 100, block: int = 32, keep_ce
         while done < r.steps:
             q = min(every, r.steps - done)
             t = time.perf_counter()
-            g.advance(q)
+            # the quantum and its sample in one call, as the reference's run
+            # loop drives GpuLattice::advance_sampling (src/sim.rs:119-143);
+            # resident bands take the sample in the same launch
+            g.advance_sampling(q, every, 1.0)
             spent["advance"] += time.perf_counter() - t
             done += q
             t = time.perf_counter()
-            g.sample_field(1.0)
             res.snapshots.append(_snapshot(g, done))
             spent["snapshot"] += time.perf_counter() - t
         if keep_cells:

@@ -112,6 +112,78 @@ def test_batch_sampling_advance_equals_advance_then_snapshot():
         g.close()
 
 
+def test_two_sampling_advances_queued_before_reading():
+    """Two sampling advances queued back to back (and a plain async advance
+    between them) are read back oldest first, each equal to an advance and a
+    snapshot taken in turn; a third unread one is refused."""
+    from lgca_amd import LgcaError
+    specs = [(256, 128, 1, 3), (192, 64, 2, 1), (640, 512, 6, 7)]
+    a, b = [lattice(*s) for s in specs], [lattice(*s) for s in specs]
+    for g in a + b:
+        g.attach_field(8, 8)
+    ba, bb = Batch(a), Batch(b)
+    limits = [30, 12, 30]
+    want = []
+    for n, lim in ((30, limits), (20, None)):
+        ba.advance_limited(n, lim if lim is not None else [n] * 3)
+        want.append(ba.snapshot(1.0))
+    ba.advance(5)
+    bb.advance_sampling_async(30, limits, 1.0)
+    bb.advance_sampling_async(20, None, 1.0)
+    with pytest.raises(LgcaError, match="two snapshots"):
+        bb.advance_sampling_async(1, None, 1.0)
+    bb.advance_async(5)                      # keeps the unread snapshots
+    for w in want:
+        got = bb.read_snapshot()
+        for x, y in zip(w, got):
+            assert all(np.array_equal(u, v) for u, v in zip(x, y))
+    with pytest.raises(LgcaError, match="unread snapshot"):
+        bb.read_snapshot()
+    for x, y in zip(a, b):
+        assert x.steps == y.steps and np.array_equal(x.store(), y.store())
+    for bt in (ba, bb):
+        bt.close()
+    for g in a + b:
+        g.close()
+
+
+def test_quantum_on_another_batch_queued_before_the_last_ends():
+    """A serving loop's order: a quantum queued on batch A, the next queued at
+    once on batch B of another membership that shares members with A (B's
+    queue waits for A on the device), then both snapshots read; every
+    member, and every snapshot, equals its own run's."""
+    specs = [(256, 128, 1, 3), (192, 64, 2, 1), (100, 36, 3, 5), (384, 256, 4, 2)]
+    solo, mem = [lattice(*s) for s in specs], [lattice(*s) for s in specs]
+    for g in solo + mem:
+        g.attach_field(4, 4)
+    for rep in range(2):
+        ba = Batch(mem[:3])
+        bb = Batch(mem[1:])
+        ba.advance_sampling_async(40, [40, 17, 40], 1.0)
+        bb.advance_sampling_async(40, None, 1.0)   # members 1, 2 follow A's steps
+        sa, sb = ba.read_snapshot(), bb.read_snapshot()
+        for g, lim in zip(solo[:3], (40, 17, 40)):
+            g.advance(lim)
+        snaps_a = []
+        for g in solo[:3]:
+            g.sample_field(1.0)
+            snaps_a.append(g.read_field())
+        for g in solo[1:]:
+            g.advance(40)
+        snaps_b = []
+        for g in solo[1:]:
+            g.sample_field(1.0)
+            snaps_b.append(g.read_field())
+        for want, got in zip(snaps_a + snaps_b, list(sa) + list(sb)):
+            assert all(np.array_equal(u, v) for u, v in zip(want, got)), f"round {rep}"
+        ba.close()
+        bb.close()
+        for s, m in zip(solo, mem):
+            assert s.steps == m.steps and np.array_equal(s.store(), m.store()), f"round {rep}"
+    for g in solo + mem:
+        g.close()
+
+
 def test_destroy_with_async_work_queued():
     """Destroying a batch or a lattice right after an async advance (no wait)
     lets the queued steps finish first: the members hold the advanced state."""

@@ -37,13 +37,16 @@ def test_footprint_counts_planes_tables_and_field():
 class _Lattice:
     """Stand-in for a resident lattice: counts its own steps. With `bands`
     set it says it runs resident bands, so that a run with few others on the
-    device advances on its own (advance_async / wait / read_field)."""
+    device advances on its own (advance_async / wait / read_field). `last`
+    is the batch advance that stepped it last (a thread), which the next one
+    waits for, as a batch's queue waits on the device for the members'
+    earlier work."""
     live = {}
     bands = False
     alone_calls = 0
 
     def __init__(self, r):
-        self.r, self.steps, self.closed, self.queued = r, 0, False, 0
+        self.r, self.steps, self.closed, self.queued, self.last = r, 0, False, 0, None
         _Lattice.live[r.rid] = self
 
     def resident(self):
@@ -51,10 +54,12 @@ class _Lattice:
 
     def advance_async(self, n, every, alpha):
         assert not self.closed and 0 < n <= every and alpha == 1.0
+        assert self.last is None or not self.last.is_alive(), "own call before the batch's steps"
         self.queued = n
         _Lattice.alone_calls += 1
 
     def wait(self):
+        time.sleep(0.001)               # the quantum's steps on the device
         self.steps += self.queued
         self.queued = 0
 
@@ -63,53 +68,63 @@ class _Lattice:
 
     def close(self):
         assert not self.closed, "a lattice freed twice"
-        time.sleep(0.003)               # a device free waits for the device
         self.closed = True
         del _Lattice.live[self.r.rid]
 
 
 class _Batch:
-    """Stand-in for lgca_amd.Batch: an async advance runs on a thread of its
-    own and sleeps, so that the host work the server overlaps with it really
-    runs beside it."""
+    """Stand-in for lgca_amd.Batch: each sampling advance runs on a thread of
+    its own, which first waits for the advances that stepped its members last
+    (queued on this or another batch) and sleeps, so that the host work the
+    server overlaps with queued quanta really runs beside them; at most two
+    snapshots wait to be read, oldest first."""
 
     def __init__(self, members):
         assert all(not m.closed for m in members)
-        self.members, self.closed, self.run, self.errors = members, False, None, []
+        self.members, self.closed, self.queue, self.errors = members, False, [], []
 
-    def _advance(self, n, limits):
+    def _advance(self, before, n, limits, out):
+        for t in before:
+            t.join()
         time.sleep(0.002)
         for m, lim in zip(self.members, limits):
             if m.closed:
-                self.errors.append("a member freed during the advance")
+                self.errors.append("a member freed before its steps")
             m.steps += min(n, lim)
+        out.extend((np.full(1, m.steps, np.float32),) * 4 for m in self.members)
 
     def advance_sampling_async(self, n, limits, alpha):
-        assert not self.closed and self.run is None and len(limits) == len(self.members)
-        self.run = threading.Thread(target=self._advance, args=(n, list(limits)))
-        self.run.start()
-
-    def wait(self):
-        self.run.join()
-        self.run = None
-        assert not self.errors, self.errors
+        assert not self.closed and len(limits) == len(self.members)
+        assert len(self.queue) < 2, "a third unread snapshot"
+        before = {id(m.last): m.last for m in self.members if m.last is not None}.values()
+        out = []
+        t = threading.Thread(target=self._advance, args=(list(before), n, list(limits), out))
+        for m in self.members:
+            m.last = t
+        t.start()
+        self.queue.append((t, out))
 
     def read_snapshot(self):
-        assert self.run is None
-        return [(np.full(1, m.steps, np.float32),) * 4 for m in self.members]
+        t, out = self.queue.pop(0)
+        t.join()
+        assert not self.errors, self.errors
+        return out
 
     def close(self):
-        assert not self.closed and self.run is None
+        assert not self.closed
+        for t, _ in self.queue:
+            t.join()
+        assert not self.errors, self.errors
         self.closed = True
 
 
 @pytest.mark.parametrize("bands", [False, True])
 def test_batched_scheduler_snapshots_budget_and_frees(monkeypatch, bands):
-    """The continuous-batching loop with stand-ins for the device objects:
-    every run's snapshots fall at its own multiples of `every` and its last
-    step, every lattice is freed exactly once and never while its batch
-    advances, and the bytes of lattices not yet freed stay within the budget
-    whenever a run is admitted beside others."""
+    """The continuous-batching loop with stand-ins for the device objects,
+    quanta queued two deep: every run's snapshots fall at its own multiples of
+    `every` and its last step, every lattice is freed exactly once and never
+    before its last steps, and the bytes of lattices not yet freed stay
+    within the budget whenever a run is admitted beside others."""
     rng = np.random.default_rng(5)
     trace = [serve.Request(i, 0.002 * i, 64, 64, int(rng.integers(1, 9)) * 25 + int(rng.integers(0, 25)),
                            i + 1, 4, 0.2, 10.0, 4) for i in range(40)]
