@@ -15,6 +15,7 @@
 // The polynomials depend only on the obstacle layout (which cells draw), so
 // they are built once per load and reused by every step.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 #include <vector>
@@ -29,10 +30,7 @@ inline uint64_t xs_step(uint64_t x) {
 }
 
 // Low 64 coefficients of P(z) = z^64 + sum_{i<64} p_i z^i.
-inline uint64_t charpoly_low() {
-  static uint64_t cached = 0;
-  static bool done = false;
-  if (done) return cached;
+inline uint64_t charpoly_find() {
   // Bit 0 of the state sequence from an arbitrary non-zero start.
   const int N = 256;
   uint8_t a[N];
@@ -66,8 +64,10 @@ inline uint64_t charpoly_low() {
   if (L == 64)
     for (int j = 1; j <= 64; ++j)
       if (C[j]) p |= 1ull << (64 - j);
-  cached = p;
-  done = true;
+  return p;
+}
+inline uint64_t charpoly_low() {
+  static const uint64_t p = charpoly_find();  // thread-safe one-time initialisation
   return p;
 }
 

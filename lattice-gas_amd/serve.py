@@ -22,6 +22,8 @@ arrival) and aggregate throughput (cell updates over the makespan).
 from __future__ import annotations
 
 import math
+import queue
+import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -103,6 +105,52 @@ def build(r: Request, block: int) -> GpuLattice:
     return g
 
 
+class _Builder:
+    """Builds admitted requests' lattices on a thread of its own, one at a
+    time, so that the serving loop never waits for a build: a build's set-up
+    calls wait for their own stream, which can share a hardware queue with
+    the batch's queued quanta and then waits for those too. ctypes releases
+    the interpreter lock during each call, so the loop runs meanwhile."""
+
+    def __init__(self, block: int):
+        self.block = block
+        self.todo, self.built = queue.Queue(), queue.Queue()
+        self.thread = threading.Thread(target=self._run, daemon=True)
+        self.thread.start()
+
+    def _run(self):
+        while True:
+            r = self.todo.get()
+            if r is None:
+                return
+            try:
+                self.built.put((r, build(r, self.block), None))
+            except Exception as e:  # reported by the loop
+                self.built.put((r, None, e))
+
+    def submit(self, r: Request):
+        self.todo.put(r)
+
+    def take(self, wait: float = 0.0):
+        """Lattices built since the last call ((request, lattice) pairs),
+        waiting up to `wait` seconds for the first one."""
+        out = []
+        try:
+            out.append(self.built.get(timeout=wait) if wait > 0 else self.built.get_nowait())
+            while True:
+                out.append(self.built.get_nowait())
+        except queue.Empty:
+            pass
+        for r, g, e in out:
+            if e is not None:
+                raise e
+        return [(r, g) for r, g, _ in out]
+
+    def close(self):
+        self.todo.put(None)
+        self.thread.join()
+
+
 def footprint(r: Request, block: int) -> int:
     """Device bytes of one resident run: two plane sets, the solid plane,
     jump polynomials (two per word, and the lead jump), word order, the two digit and inflow slots of the
@@ -154,8 +202,10 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
     quantum is queued -- on a new batch when the membership changes, whose
     queue waits for the last one on the device -- before the host waits for
     the oldest one's snapshot, so the device has the next quantum while the
-    host reads snapshots, retires runs and builds arrivals. A run's steps
-    count when they are queued; it is retired when its last quantum is read.
+    host reads snapshots and retires runs. Arrivals are admitted while the
+    HBM budget allows and built on a thread of their own (_Builder); a built
+    run joins the next quantum queued. A run's steps count when they are
+    queued; it is retired when its last quantum is read.
     Finished runs' lattices are destroyed at once (their memory goes back to
     the library's cache; a batch still holding one frees it with the batch),
     and their bytes leave the budget.
@@ -178,26 +228,32 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
     spent = {"admit": 0.0, "batch_setup": 0.0, "advance": 0.0, "snapshot": 0.0,
              "retire": 0.0, "idle": 0.0}  # wall seconds by activity
 
-    def admit_arrived():
-        nonlocal used
+    builder = _Builder(block)
+    building = 0            # admitted requests not yet built
+
+    def admit_arrived(wait=0.0):
+        nonlocal used, building
         t = time.perf_counter()
         while pending and pending[0].arrival <= clock():
             r = pending[0]
             fp = footprint(r, block)
-            if used + fp > hbm_budget and runs:
+            if used + fp > hbm_budget and (runs or building):
                 break                                # wait for memory to free up
             pending.popleft()
-            g = build(r, block)
             used += fp
             results[r.rid].start = clock()
-            runs.append({"req": r, "g": g, "sched": 0, "done": 0, "bytes": fp})
+            builder.submit(r)
+            building += 1
+        for r, g in builder.take(wait if building else 0.0):
+            building -= 1
+            runs.append({"req": r, "g": g, "sched": 0, "done": 0, "bytes": footprint(r, block)})
         spent["admit"] += time.perf_counter() - t
 
     def close_batch(b):
         if b is not None and b is not cur and all(q[0] is not b for q in inflight):
             b.close()
 
-    while pending or runs or inflight:
+    while pending or runs or inflight or building:
         admit_arrived()
         nxt = [a for a in runs if a["sched"] < a["req"].steps]
         peak = max(peak, len(runs))
@@ -244,7 +300,9 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
                 if len(inflight) < depth:
                     continue                         # queue the next quantum before reading
         if not inflight:
-            if pending:
+            if building:                             # nothing to step until a build is done
+                admit_arrived(0.002)
+            elif pending:
                 t = time.perf_counter()
                 time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
                 spent["idle"] += time.perf_counter() - t
@@ -271,7 +329,7 @@ def serve_batched(trace: list[Request], every: int = 100, block: int = 32,
             runs = [a for a in runs if id(a) not in gone]
         close_batch(b)
         spent["retire"] += time.perf_counter() - tr
-    close_batch(cur)
+    builder.close()
     if cur is not None:
         cur.close()
     t_last = max(res.done for res in results.values())
