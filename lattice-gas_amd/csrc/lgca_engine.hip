@@ -951,23 +951,19 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
   uint64_t left = n, since = 0;
   const uint32_t per = g->G.nbands + 1;
   if (timed) g->tev_used = 0;
-  const uint32_t passes = step_passes(g);
-  const bool ahead = passes == 2 && g->ahead > 0;
-  if (ahead) {
-    int rc = ensure_ahead_stream(g);
-    if (!rc) rc = ensure_draw_buffers(g);
-    if (rc) return rc;
-    // the draw stream starts with this call's work on the main stream
-    HIP_TRY(hipEventRecord(g->ev_start, g->stream));
-    HIP_TRY(hipStreamWaitEvent(g->dstream, g->ev_start, 0));
-  }
+  uint32_t passes = step_passes(g);
   uint32_t brows = 0, bnb = 0;
-  // a call whose first chunk is too short to repay the band launch (one
-  // step per call at 2048 x 1280: 50 us resident against 25) takes the
-  // other pipelines unless the caller asked for resident bands
+  // a call whose first chunk is too short to repay the band launch takes
+  // the other pipelines unless the caller asked for resident bands
   const uint64_t first = std::min<uint64_t>(n, every > 0 ? every : n);
-  if (resident_active(g, &brows, &bnb) &&
-      (g->resident == 1 || (double)first * band_gain_us(g, brows) >= LG_BAND_LAUNCH_US)) {
+  const bool resident = resident_active(g, &brows, &bnb);
+  if (resident && g->resident != 1 && first == 1 && g->G.inlet_cols == 0) {
+    // a single step without inflow: the fused step's call (a tables launch
+    // and one step launch) is the shortest (2048 x 1280: 26 us against 29
+    // resident and 31 two-pass)
+    passes = 1;
+  } else if (resident &&
+             (g->resident == 1 || (double)first * band_gain_us(g, brows) >= LG_BAND_LAUNCH_US)) {
     // resident bands: one launch per chunk (it makes its own stream tables);
     // if the runtime declines the launch, the rest of the call takes the
     // other pipelines below
@@ -1003,6 +999,15 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
       }
     }
     if (left == 0) return LG_OK;
+  }
+  const bool ahead = passes == 2 && g->ahead > 0;
+  if (ahead) {
+    int rc = ensure_ahead_stream(g);
+    if (!rc) rc = ensure_draw_buffers(g);
+    if (rc) return rc;
+    // the draw stream starts with this call's work on the main stream
+    HIP_TRY(hipEventRecord(g->ev_start, g->stream));
+    HIP_TRY(hipStreamWaitEvent(g->dstream, g->ev_start, 0));
   }
   while (left > 0) {
     const uint32_t extra = (passes == 2 && !ahead) ? 1u : 0u;  // see the two-pass branch
@@ -1102,8 +1107,8 @@ bool lgca_internal::resident_active(const lg_t* g, uint32_t* rows, uint32_t* nb)
 }
 
 double lgca_internal::band_gain_us(const lg_t* g, uint32_t rows) {
-  const double resident = 3.6 + 0.72 * rows;
-  const double two_pass = 5.2 + 0.06e-3 * (double)g->G.total + (g->G.inlet_cols > 0 ? 1.5 : 0.0);
+  const double resident = 4.0 + 0.47 * rows;
+  const double two_pass = 4.8 + 0.055e-3 * (double)g->G.total + (g->G.inlet_cols > 0 ? 1.3 : 0.0);
   return two_pass - resident;
 }
 

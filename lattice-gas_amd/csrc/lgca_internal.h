@@ -100,19 +100,20 @@ void resolve_pipeline(lg_lattice* g);
 // Resident bands: rows per band at most (one wave per row), and the polls a
 // neighbour wait makes before it gives up (each sleeps ~64 clocks: ~0.1 s).
 #define LG_BAND_MAX_ROWS 11u
-// The size rule's resident bands, from measured step costs: a step of B-row
-// bands costs about 3.6 + 0.72 B us whatever the row width, the two-pass
-// step about 5.2 us + 0.06 us per 1000 words, 1.5 us more with inflow, and
-// a band launch about LG_BAND_LAUNCH_US more than the other pipelines'
-// launches of the same steps (its first step's tables and draws are not
-// overlapped: a 5-step launch at 4096 x 2304 takes 71.6 us, 10.3 us per step
-// in longer ones; the two-pass step's call has its draw-only launch); a call takes resident bands when its first
-// chunk's steps save more than that (band_gain_us). (Measured, 100 steps
-// per call: 2048 x 1280 6.5 against 6.9 us, with inflow 6.9 against 8.9;
-// 1024 x 1024 with inflow 6.7 against 7.7; 256 x 256 4.3 against 5.5;
-// 4096 x 2304 with inflow 10.3 against 14.3; 1024 x 2304 with inflow,
-// 9-row bands of 16 words, 9.8 against 9.0; 512 x 1280 6.4 against 5.7.)
-#define LG_BAND_LAUNCH_US 12.0
+// The size rule's resident bands, from measured step costs (wall-clock per
+// step in calls of 10 and 100 steps on MI355X): a step of B-row bands costs
+// about 4.0 + 0.47 B us whatever the row width (at most: 4-row to 9-row
+// bands measured 5.6 to 8.3 us), the two-pass step about 4.8 us + 0.055 us
+// per 1000 words, 1.3 us more with inflow; a band call costs no more than a
+// two-pass call of the same steps (2048 x 1280: 23.6 against 24.1 us
+// besides the steps; 4096 x 2304 with inflow 28.6 against 32.1), so a call
+// takes resident bands where its first chunk's steps save more than
+// LG_BAND_LAUNCH_US (band_gain_us). (Measured per step, resident against
+// two-pass: 2048 x 1280 5.6 against 6.5, with inflow 5.8 against 7.9;
+// 1024 x 1024 with inflow 5.9 against 7.2; 256 x 256 4.5 against 5.5;
+// 4096 x 2304 with inflow 8.3 against 14.0; 1024 x 2304 with inflow, 9-row
+// bands of 16 words, 8.2 against 8.2; 512 x 1280 5.6 against 5.4.)
+#define LG_BAND_LAUNCH_US 1.0
 // Step time saved by resident bands of `rows` rows (us; negative: lost).
 double band_gain_us(const lg_lattice* g, uint32_t rows);
 #define LG_BAND_SPIN_LIMIT (1u << 20)

@@ -104,3 +104,32 @@ def test_compiled_header_consumer_runs_the_golden_case():
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
     assert "harness: ok" in r.stdout
+
+
+def test_memory_cache_reuses_and_trims(oracle):
+    """Buffers of destroyed lattices go to the library's cache and come back to
+    the next lattices (whose runs stay exact); lg_trim_memory gives the cache
+    back to the runtime, and lattices made after it still run exactly."""
+    import lgca_amd
+    lgca_amd.trim_memory()
+    for rnd in range(3):
+        gs = []
+        for k in range(4):
+            w, h = 256 + 64 * k, 128
+            lat = oracle.Lattice(w, h, 0.22, (0.3, 0.0), 8, True, 2, 90 + k)
+            lat.init_equilibrium()
+            g = GpuLattice(w, h, True, 90 + k, 2)
+            g.set_inlet(8, 0.22, (0.3, 0.0))
+            g.load(lat.cells)
+            gs.append((lat, g))
+        for lat, g in gs:
+            lat.advance(23)
+            g.advance(23)
+            assert np.array_equal(g.store(), lat.cells), f"round {rnd}, {lat.w} x {lat.h}"
+        for _, g in gs:
+            g.close()
+        freed = lgca_amd.trim_memory() if rnd == 1 else None
+        if rnd == 1:
+            assert freed > 0
+    assert lgca_amd.trim_memory() > 0
+    assert lgca_amd.trim_memory() == 0
