@@ -682,7 +682,7 @@ bool band_eligible(const lg_t* g, uint32_t* rows, uint32_t* nb) {
 }
 
 int ensure_band_buffers(lg_t* g, uint32_t nb) {
-  if (g->band_nb >= nb && g->band_halo.p) return LG_OK;
+  if (g->band_nb >= nb && g->band_halo.p && g->lead_version == g->tables_version) return LG_OK;
   HIP_TRY(hipStreamSynchronize(g->stream));
   HIP_TRY(g->band_halo.alloc((size_t)2 * 2 * nb * 128));
   HIP_TRY(g->band_flags.alloc((size_t)2 * nb));
@@ -694,6 +694,15 @@ int ensure_band_buffers(lg_t* g, uint32_t nb) {
   if (!g->band_err_host) {
     HIP_TRY(host_alloc((void**)&g->band_err_host, 8));
     *g->band_err_host = 0;
+  }
+  if (g->lead_version != g->tables_version) {
+    std::vector<uint64_t> solid, cj(g->cjump.n), lead;
+    HIP_TRY(hipMemcpy(cj.data(), g->cjump.p, cj.size() * 8, hipMemcpyDeviceToHost));
+    if (int rc = download_solid(g, solid)) return rc;
+    g->lead_ok = lgca::build_lead_jumps(g->G.w, g->G.h, solid, cj, lead);
+    HIP_TRY(g->lead.alloc(lead.size()));
+    HIP_TRY(hipMemcpy(g->lead.p, lead.data(), lead.size() * 8, hipMemcpyHostToDevice));
+    g->lead_version = g->tables_version;
   }
   if (!g->band_pw.p) {
     std::vector<uint64_t> pw;

@@ -758,12 +758,9 @@ int rebuild_tables(lg_t* g, const std::vector<uint64_t>& solid_host) {
     HIP_TRY(g->icell.alloc(ic.size()));
     HIP_TRY(hipMemcpyAsync(g->icell.p, ic.data(), ic.size() * 8, hipMemcpyHostToDevice, g->stream));
   }
-  std::vector<uint64_t> lead;
-  g->lead_ok = lgca::build_lead_jumps(g->G.w, g->G.h, solid_host, cj, lead);
-  HIP_TRY(g->lead.alloc(lead.size()));
-  HIP_TRY(hipMemcpyAsync(g->lead.p, lead.data(), lead.size() * 8, hipMemcpyHostToDevice, g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->tables_dirty = false;
+  g->tables_version += 1;  // the resident bands' lead jumps follow on their next launch
   return LG_OK;
 }
 
@@ -778,12 +775,16 @@ std::vector<uint64_t> solid_plane(const LgGeom& G, const uint8_t* cells) {
   return s;
 }
 
-int download_solid(lg_t* g, std::vector<uint64_t>& s) {
+}  // namespace
+
+int lgca_internal::download_solid(lg_t* g, std::vector<uint64_t>& s) {
   s.resize((size_t)g->G.total);
   HIP_TRY(hipMemcpyAsync(s.data(), g->solid.p, s.size() * 8, hipMemcpyDeviceToHost, g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
   return LG_OK;
 }
+
+namespace {
 
 int add_solid(lg_t* g, int kind, float a, float b, float c, float d) {
   hipLaunchKernelGGL(lg_solid_kernel, dim3(blocks_for(g->G.total)), dim3(256), 0, g->stream, g->G,

@@ -176,6 +176,8 @@ void destroy_lattice(lg_lattice* g);
 // Rebuild the static jump and order tables of a lattice if its obstacle
 // layout or inflow width changed since they were last built.
 int ensure_tables(lg_lattice* g);
+// The solid plane on the host (waits for the handle's stream).
+int download_solid(lg_lattice* g, std::vector<uint64_t>& s);
 
 // Allocate the two digit/inflow slots of the two-pass step if needed.
 int ensure_draw_buffers(lg_lattice* g);
@@ -223,9 +225,11 @@ struct lg_lattice {
   DevBuf<uint64_t> a, b, solid, cjump, ijump, basis;
   DevBuf<uint64_t> icell;  // per-cell inflow jumps, inflow widths <= 64 (resident bands)
   // resident bands: per-word lead jumps, and whether every half of the
-  // lattice has at most one run of cells without a draw (lgca::build_lead_jumps)
+  // lattice has at most one run of cells without a draw (lgca::build_lead_jumps),
+  // built at the first band launch after the tables (tables_version) change
   DevBuf<uint64_t> lead;
   bool lead_ok = false;
+  uint64_t tables_version = 0, lead_version = ~0ull;
   DevBuf<uint32_t> row_band, order;
   uint32_t units = 1;  // 1: a lane steps a 64-cell word; 2: a 32-cell half
   // passes per step: 1 = one fused kernel per step; 2 = one launch per step

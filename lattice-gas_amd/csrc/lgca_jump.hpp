@@ -258,8 +258,14 @@ inline bool build_lead_jumps(uint32_t w, uint32_t h, const std::vector<uint64_t>
                              const std::vector<uint64_t>& cjump, std::vector<uint64_t>& lead) {
   const uint64_t p = charpoly_low();
   const uint32_t wpr = (w + 63) / 64;
-  uint64_t zinv[33];
-  for (uint32_t k = 0; k <= 32; ++k) zinv[k] = k ? poly_pow(~0ull - k, p) : 1ull;
+  struct Inv {
+    uint64_t z[33];
+    Inv() {
+      for (uint32_t k = 0; k <= 32; ++k) z[k] = k ? poly_pow(~0ull - k, charpoly_low()) : 1ull;
+    }
+  };
+  static const Inv inv;  // z^-k for k <= 32, built once per process (thread-safe)
+  const uint64_t* zinv = inv.z;
   auto one_run = [](uint32_t fh) {
     const uint32_t z = ~fh;
     return ((z + (z & (0u - z))) & z) == 0u;
