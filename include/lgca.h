@@ -139,7 +139,10 @@ int lg_set_steps(lg_t* g, uint64_t steps);
 
 /* Launch shape of the step: each lane steps 64 cells (a whole u64 word, the
  * default) or 32 (half a word, twice the lanes). The results are identical
- * either way. */
+ * either way. Until set, calls of one or two steps without inflow columns on
+ * lattices below 2^18 words per plane take the fused step with 32-cell lanes
+ * (the shortest call), unless lg_set_passes or lg_set_resident(g, 1) chose
+ * the pipeline. */
 int lg_set_work_unit(lg_t* g, uint32_t cells_per_lane);
 
 /* Work per step launch: 2 splits the step into a draw pass (the random
@@ -154,8 +157,9 @@ int lg_set_passes(lg_t* g, uint32_t passes);
 /* (lg_set_passes(g, 0) also returns the draw unit and the draw-ahead below to
  * their size rule.) */
 
-/* The pipeline the next step will use: kernels per step (1 or 2), cells per
- * lane of the collide (or fused) and draw work, draw-ahead steps. */
+/* The pipeline the steps of a long call will use when it does not run
+ * resident bands (lg_get_resident): kernels per step (1 or 2), cells per lane
+ * of the collide (or fused) and draw work, draw-ahead steps. */
 int lg_get_pipeline(lg_t* g, uint32_t* passes, uint32_t* cells_per_lane,
                     uint32_t* draw_cells_per_lane, uint32_t* draw_ahead);
 
@@ -188,8 +192,7 @@ int lg_set_temporal_depth(lg_t* g, uint32_t steps, uint32_t rows_per_tile);
 
 /* New. Resident bands (on = 1; 2, the default, = on unless lg_set_passes,
  * lg_set_draw_ahead or lg_set_temporal_depth chose another pipeline; 0 =
- * off): each call's steps run in one cooperative
- * launch per chunk (a chunk ends at a sample or at 65536 steps; a chunk that
+ * off): each call's steps run in one launch per chunk (a chunk ends at a sample or at 65536 steps; a chunk that
  * ends at a sample takes it in the same launch where a band's block sums fit
  * its LDS), in which one
  * workgroup per band of rows keeps its rows in registers for the whole
@@ -200,8 +203,8 @@ int lg_set_temporal_depth(lg_t* g, uint32_t steps, uint32_t rows_per_tile);
  * one band per CU; other lattices keep the other pipelines. The size rule
  * (2) takes it where, by measured step costs, bands of B rows step faster
  * than the two-pass step, for calls whose first chunk (the call, or up to
- * its first sample) saves more step time than the band launch costs (about
- * 12 us). Results are identical. lg_get_resident returns 1 (and the band
+ * its first sample) saves more step time than a band call costs over a
+ * two-pass call (about 4 us). Results are identical. lg_get_resident returns 1 (and the band
  * height and count) when resident bands are in use for calls of enough
  * steps, else 0. The process's first band launch on a device is
  * cooperative; if the runtime declines it (it does while another process

@@ -278,6 +278,7 @@ void fill_desc(lg_batch* b, LgDesc* h, const uint64_t* limits, uint64_t n) {
     d.ijump = g->ijump.p;
     d.order = g->order.p;
     d.tabs = g->basis.p;
+    g->tab_n = 0;  // the batch's tables kernel overwrites them
     d.kb = g->kbuf.p;
     d.ib = g->ibuf.p;
     d.seed = g->seed;

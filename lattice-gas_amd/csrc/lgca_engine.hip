@@ -8,8 +8,8 @@
 // the digits drawn one launch earlier, bounce back on solid sites, overwrite
 // the inflow cells, mask the ragged tail. The alternatives (one fused kernel
 // per step; draws on a second stream) give the same cells. The per-step
-// stream jump tables come from lg_tables_kernel, one launch per chunk of
-// steps.
+// stream jump tables come from lg_tables_kernel, launched for the next
+// basis_steps steps whenever a call's steps leave the ones it built last.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -899,7 +899,7 @@ int ensure_ahead_stream(lg_t* g) {
 // Draw-ahead: launch the draws of steps [s0, s0 + nd) of the chunk whose
 // tables are in basis into the next slot group on dstream, then their
 // collides on the main stream.
-int enqueue_ahead_group(lg_t* g, uint32_t s0, uint32_t nd) {
+int enqueue_ahead_group(lg_t* g, const uint64_t* tabs, uint32_t s0, uint32_t nd) {
   const uint32_t per = g->G.nbands + 1;
   const size_t nhalf = (size_t)2 * g->G.total, nih = (size_t)2 * g->G.iw * g->G.h;
   const uint32_t q = (uint32_t)(g->dgroup++ & 1u);
@@ -907,7 +907,7 @@ int enqueue_ahead_group(lg_t* g, uint32_t s0, uint32_t nd) {
   uint32_t* ib = g->ibuf.p + (size_t)q * g->ahead * 7 * nih;
   HIP_TRY(hipStreamWaitEvent(g->dstream, g->ev_used[q], 0));
   const dim3 dg(blocks_for((size_t)g->G.total * g->draw_units), nd);
-  const uint64_t* tabs0 = g->basis.p + (size_t)s0 * per * LG_WTAB;
+  const uint64_t* tabs0 = tabs + (size_t)s0 * per * LG_WTAB;
   if (g->draw_units == 2)
     hipLaunchKernelGGL(lg_draw_kernel<2>, dg, dim3(256), 0, g->dstream, g->G, g->solid.p,
                        g->cjump.p, g->row_band.p, g->ijump.p, g->order.p, tabs0, per, kb, ib);
@@ -944,6 +944,24 @@ int enqueue_ahead_group(lg_t* g, uint32_t s0, uint32_t nd) {
   return LG_OK;
 }
 
+// The stream tables of steps [t, t + nsteps) (nsteps <= basis_steps): from
+// the ones `basis` holds when they cover those steps, else the tables of the
+// next basis_steps steps from t, launched on `stream` first. Tables depend
+// only on the seed and the step, so they stay good across calls; work that
+// reads them is ordered before a later refill by the streams' order (a refill
+// on the draw stream waits for the main stream's earlier work, and the main
+// stream's later work waits for the draws).
+const uint64_t* stream_tables(lg_t* g, uint64_t t, uint32_t nsteps, hipStream_t stream) {
+  const uint32_t per = g->G.nbands + 1;
+  if (g->tab_n == 0 || t < g->tab_t0 || t + nsteps > g->tab_t0 + g->tab_n) {
+    hipLaunchKernelGGL(lg_tables_kernel, dim3(blocks_for((size_t)g->basis_steps * per * 16)),
+                       dim3(256), 0, stream, g->seed, t, g->basis_steps, g->G.nbands, g->basis.p);
+    g->tab_t0 = t;
+    g->tab_n = g->basis_steps;
+  }
+  return g->basis.p + (size_t)(t - g->tab_t0) * per * LG_WTAB;
+}
+
 // Enqueue `n` steps (and samples every `every`), no synchronisation.
 int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) {
   int trc = ensure_tables(g);
@@ -958,11 +976,15 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
   // the other pipelines unless the caller asked for resident bands
   const uint64_t first = std::min<uint64_t>(n, every > 0 ? every : n);
   const bool resident = resident_active(g, &brows, &bnb);
-  if (resident && g->resident != 1 && first == 1 && g->G.inlet_cols == 0) {
-    // a single step without inflow: the fused step's call (a tables launch
-    // and one step launch) is the shortest (2048 x 1280: 26 us against 29
-    // resident and 31 two-pass)
+  uint32_t units = g->units;  // cells per lane of the fused step
+  if (g->resident != 1 && first <= 2 && g->G.inlet_cols == 0 && g->passes == 0 &&
+      g->tdepth < 2 && g->auto_ahead && g->G.total < LG_AUTO_FUSED_WORDS) {
+    // one or two steps without inflow: the fused step's call is the
+    // shortest, with 32-cell lanes unless the caller set the lane width
+    // (2048 x 1280, one step: 17.7 us against 18.7 with 64-cell lanes, 20.3
+    // two-pass, 29 resident; two steps: 24.6, 26.5, 28.2, 33.0)
     passes = 1;
+    if (g->auto_units) units = 2;
   } else if (resident &&
              (g->resident == 1 || (double)first * band_gain_us(g, brows) >= LG_BAND_LAUNCH_US)) {
     // resident bands: one launch per chunk (it makes its own stream tables);
@@ -1014,10 +1036,8 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
     const uint32_t extra = (passes == 2 && !ahead) ? 1u : 0u;  // see the two-pass branch
     uint64_t chunk = std::min<uint64_t>(left, g->basis_steps - extra);
     if (every > 0) chunk = std::min<uint64_t>(chunk, every - since);
-    const uint32_t ntab = (uint32_t)chunk + extra;
-    hipLaunchKernelGGL(lg_tables_kernel, dim3(blocks_for((size_t)ntab * per * 16)), dim3(256), 0,
-                       ahead ? g->dstream : g->stream, g->seed, g->steps, ntab, g->G.nbands,
-                       g->basis.p);
+    const uint64_t* tabs =
+        stream_tables(g, g->steps, (uint32_t)chunk + extra, ahead ? g->dstream : g->stream);
     HIP_TRY(hipGetLastError());
     if (timed && ahead) {
       // one interval for the whole call: the draw stream runs ahead of the
@@ -1041,22 +1061,22 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
     }
     if (ahead) {
       for (uint64_t s0 = 0; s0 < chunk; s0 += g->ahead) {
-        int rc = enqueue_ahead_group(g, (uint32_t)s0,
+        int rc = enqueue_ahead_group(g, tabs, (uint32_t)s0,
                                      (uint32_t)std::min<uint64_t>(g->ahead, chunk - s0));
         if (rc) return rc;
       }
     } else if (passes == 1) {
       for (uint64_t k = 0; k < chunk; ++k) {
-        if (g->units == 2)
+        if (units == 2)
           hipLaunchKernelGGL(lg_step_kernel_h, dim3(blocks_for((size_t)g->G.total * 2)), dim3(256),
                              0, g->stream, g->G, g->a.p, g->b.p, g->solid.p, g->cjump.p,
                              g->row_band.p, g->ijump.p, g->order.p,
-                             g->basis.p + (size_t)k * per * LG_WTAB);
+                             tabs + (size_t)k * per * LG_WTAB);
         else
           hipLaunchKernelGGL(lg_step_kernel_w, dim3(blocks_for(g->G.total)), dim3(256), 0,
                              g->stream, g->G, g->a.p, g->b.p, g->solid.p, g->cjump.p,
                              g->row_band.p, g->ijump.p, g->order.p,
-                             g->basis.p + (size_t)k * per * LG_WTAB);
+                             tabs + (size_t)k * per * LG_WTAB);
         std::swap(g->a.p, g->b.p);
         g->steps += 1;
       }
@@ -1066,12 +1086,12 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
       int rc = ensure_draw_buffers(g);
       if (rc) return rc;
       if (left == n) {  // the call's first step: draw it on its own
-        launch_step2(g, 1, false, true, g->basis.p);
+        launch_step2(g, 1, false, true, tabs);
         g->kslot = 0;
       }
       for (uint64_t k = 0; k < chunk; ++k) {
         const bool more = k + 1 < chunk || left > chunk;
-        launch_step2(g, g->kslot, true, more, g->basis.p + (size_t)(k + 1) * per * LG_WTAB);
+        launch_step2(g, g->kslot, true, more, tabs + (size_t)(k + 1) * per * LG_WTAB);
         g->kslot ^= 1u;
         std::swap(g->a.p, g->b.p);
         g->steps += 1;
@@ -1295,6 +1315,7 @@ int lg_restart(lg_t* g, const uint8_t* cells, size_t n, uint64_t seed) {
   if (rc) return rc;
   g->seed = seed;
   g->steps = 0;
+  g->tab_n = 0;  // tables of the old seed
   return LG_OK;
 }
 
@@ -1467,6 +1488,7 @@ int lg_set_work_unit(lg_t* g, uint32_t cells_per_lane) {
   if (cells_per_lane != 64 && cells_per_lane != 32)
     return fail(LG_EINVAL, "cells per lane must be 64 or 32");
   g->units = cells_per_lane == 64 ? 1u : 2u;
+  g->auto_units = false;
   return LG_OK;
 }
 

@@ -104,16 +104,17 @@ void resolve_pipeline(lg_lattice* g);
 // step in calls of 10 and 100 steps on MI355X): a step of B-row bands costs
 // about 4.0 + 0.47 B us whatever the row width (at most: 4-row to 9-row
 // bands measured 5.6 to 8.3 us), the two-pass step about 4.8 us + 0.055 us
-// per 1000 words, 1.3 us more with inflow; a band call costs no more than a
-// two-pass call of the same steps (2048 x 1280: 23.6 against 24.1 us
-// besides the steps; 4096 x 2304 with inflow 28.6 against 32.1), so a call
-// takes resident bands where its first chunk's steps save more than
-// LG_BAND_LAUNCH_US (band_gain_us). (Measured per step, resident against
+// per 1000 words, 1.3 us more with inflow; besides its steps a band call
+// costs about 23 us, a two-pass call about 15 (2048 x 1280, its stream
+// tables already built), so a call takes resident bands where its first
+// chunk's steps save more than LG_BAND_LAUNCH_US (band_gain_us; the model
+// puts the gain at 2048 x 1280 near 0.7 us a step, measured 0.9-1.6, and
+// the calls cross over at 5-6 steps). (Measured per step, resident against
 // two-pass: 2048 x 1280 5.6 against 6.5, with inflow 5.8 against 7.9;
 // 1024 x 1024 with inflow 5.9 against 7.2; 256 x 256 4.5 against 5.5;
 // 4096 x 2304 with inflow 8.3 against 14.0; 1024 x 2304 with inflow, 9-row
 // bands of 16 words, 8.2 against 8.2; 512 x 1280 5.6 against 5.4.)
-#define LG_BAND_LAUNCH_US 1.0
+#define LG_BAND_LAUNCH_US 4.0
 // Step time saved by resident bands of `rows` rows (us; negative: lost).
 double band_gain_us(const lg_lattice* g, uint32_t rows);
 #define LG_BAND_SPIN_LIMIT (1u << 20)
@@ -232,6 +233,7 @@ struct lg_lattice {
   uint64_t tables_version = 0, lead_version = ~0ull;
   DevBuf<uint32_t> row_band, order;
   uint32_t units = 1;  // 1: a lane steps a 64-cell word; 2: a 32-cell half
+  bool auto_units = true;  // calls of one or two steps may take 32-cell lanes
   // passes per step: 1 = one fused kernel per step; 2 = one launch per step
   // that collides step t and draws step t + 1's random digits into the other
   // slot of kbuf/ibuf (kslot: the slot holding the current step's digits);
@@ -283,6 +285,11 @@ struct lg_lattice {
   DevBuf<uint32_t> kbuf, ibuf;
   DevBuf<uint8_t> bytes;
   uint32_t basis_steps = 0;  // capacity of `basis` in steps
+  // `basis` holds the stream tables of steps [tab_t0, tab_t0 + tab_n): calls
+  // whose steps fall inside launch no tables kernel (stream_tables); a batch
+  // that writes its members' tables, and a restart, clear tab_n
+  uint64_t tab_t0 = 0;
+  uint32_t tab_n = 0;
   // field
   bool has_field = false;
   uint32_t bx = 0, by = 0, bw = 0, bh = 0;
