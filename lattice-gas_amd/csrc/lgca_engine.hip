@@ -944,6 +944,19 @@ int enqueue_ahead_group(lg_t* g, const uint64_t* tabs, uint32_t s0, uint32_t nd)
   return LG_OK;
 }
 
+// At least `count` timing events for lg_advance_timed. They release at
+// device scope: recorded around launches they then add no write-back of the
+// caches to system scope (the call's results reach the host through copies
+// that synchronise by themselves).
+int timing_events(lg_t* g, size_t count) {
+  while (g->tev.size() < count) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
+    g->tev.push_back(e);
+  }
+  return LG_OK;
+}
+
 // The stream tables of steps [t, t + nsteps) (nsteps <= basis_steps): from
 // the ones `basis` holds when they cover those steps, else the tables of the
 // next basis_steps steps from t, launched on `stream` first. Tables depend
@@ -994,11 +1007,7 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
       uint64_t chunk = std::min<uint64_t>(left, LG_BAND_MAX_CHUNK);
       if (every > 0) chunk = std::min<uint64_t>(chunk, every - since);
       if (timed) {
-        while (g->tev.size() < 2 * (g->tev_used + 1)) {
-          hipEvent_t e;
-          HIP_TRY(hipEventCreate(&e));
-          g->tev.push_back(e);
-        }
+        if (int rc = timing_events(g, 2 * (g->tev_used + 1))) return rc;
         HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used], g->stream));
       }
       // a chunk followed by a sample coarse-grains its final state in the
@@ -1043,20 +1052,12 @@ int enqueue_steps(lg_t* g, uint64_t n, uint64_t every, float alpha, bool timed) 
       // one interval for the whole call: the draw stream runs ahead of the
       // main stream's step boundaries
       if (g->tev_used == 0) {
-        while (g->tev.size() < 2) {
-          hipEvent_t e;
-          HIP_TRY(hipEventCreate(&e));
-          g->tev.push_back(e);
-        }
+        if (int rc = timing_events(g, 2)) return rc;
         HIP_TRY(hipEventRecord(g->tev[0], g->stream));
         g->tev_used = 1;
       }
     } else if (timed) {
-      while (g->tev.size() < 2 * (g->tev_used + 1)) {
-        hipEvent_t e;
-        HIP_TRY(hipEventCreate(&e));
-        g->tev.push_back(e);
-      }
+      if (int rc = timing_events(g, 2 * (g->tev_used + 1))) return rc;
       HIP_TRY(hipEventRecord(g->tev[2 * g->tev_used], g->stream));
     }
     if (ahead) {
