@@ -224,12 +224,13 @@ __device__ __forceinline__ void lg_band_tables(const lgca_internal::LgBandArgs& 
 // Step t's inflow cells of the band's rows, drawn by the table wave (inflow
 // of at most 64 cells, lanes per row `cpr` a power of two): lane q of chunk c
 // takes cell q % cpr of band row c * (64 / cpr) + q / cpr from its own jump
-// (icell; zero for a solid cell, which takes no draw) against the inflow
-// stream's table itab; each row's seven state planes of cells 0 .. cols - 1
+// (icl[c * 64 + q], the cell's icell entry kept in LDS for the launch; zero
+// for a solid cell, which takes no draw) against the inflow stream's table
+// itab; each row's seven state planes of cells 0 .. cols - 1
 // go to out[row * 8 + d], which the row's collide of step t applies.
-__device__ __forceinline__ void lg_band_inflow(const LgGeom& G, const uint64_t* __restrict icell,
-                                               uint32_t y0, uint32_t Bb, uint32_t cpr,
-                                               uint32_t lane, const uint64_t* __restrict itab,
+__device__ __forceinline__ void lg_band_inflow(const LgGeom& G, const uint64_t* __restrict icl,
+                                               uint32_t Bb, uint32_t cpr, uint32_t lane,
+                                               const uint64_t* __restrict itab,
                                                uint64_t* __restrict out) {
   const uint32_t cols = G.inlet_cols, rpc = 64u / cpr;
   const uint64_t fieldmask = cpr >= 64 ? ~0ull : (1ull << cpr) - 1ull;
@@ -237,7 +238,7 @@ __device__ __forceinline__ void lg_band_inflow(const LgGeom& G, const uint64_t* 
     const uint32_t r = r0 + lane / cpr, x = lane % cpr;
     uint32_t s = 0;
     if (r < Bb && x < cols) {
-      const uint64_t c = icell[(size_t)(y0 + r) * cols + x];
+      const uint64_t c = icl[(r0 / rpc) * 64 + lane];
       if (c) {
         uint64_t st = lg_jump_tab(c, itab);
         s = lg_eq_cell(G, st);
@@ -346,8 +347,9 @@ __device__ __forceinline__ void lg_band_sample(const lgca_internal::LgBandField&
 // A.src into A.dst. LDS: two exchange slots of the rows' planes 1, 2, 4, 5
 // (rows x 4 KB), two slots of stream tables (A.nstreams x 4 KB), the helper
 // waves' digits (split bands), the T^(4g) windows (32 KB), two slots of
-// the table wave's inflow planes (rows x 64 B) and each row's digits for its
-// next collide (rows x 2.5 KB): at most 157 KB (11 rows, 12 streams).
+// the table wave's inflow planes (rows x 64 B), each row's digits for its
+// next collide (rows x 2.5 KB) and (INFLOW 0) the inflow cells' jumps
+// (1.5 KB): at most 159 KB (11 rows, 12 streams).
 // SAMPLE: the launch ends with the fused coarse-graining (an instance of its
 // own, so launches without it keep the step loop's register allocation).
 // INFLOW (how the inflow cells draw): 0, the table wave draws every inflow
@@ -377,6 +379,8 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
   uint64_t* const ilds = pw_lds + 16 * 256;
   // [rows][2 halves][LG_KDIGITS][64 lanes]: each row's digits for its next collide
   uint32_t* const kr = reinterpret_cast<uint32_t*>(ilds + (size_t)2 * NW * 8);
+  // INFLOW 0: the table wave's inflow cells' jumps ([LG_BAND_ICHUNKS][64])
+  uint64_t* const icl = reinterpret_cast<uint64_t*>(kr + (size_t)NW * 2 * LG_KDIGITS * 64);
   const uint32_t b = blockIdx.x, y0 = b * A.rows;
   const uint32_t Bb = min(A.rows, G.h - y0);  // rows of this band
   // the band row it holds (helpers: none, but y below falls on row 0, whose
@@ -385,6 +389,11 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
                      : split                ? lg_band_row_split(w, NW)
                                             : lg_band_row_of_wave(w, NW);
   const bool row_live = r < Bb;
+  // the table wave's work (the next step's stream tables, the step's inflow
+  // cells) is one wave's serial chain; left at the default priority it
+  // issues after the older row waves of its SIMD and is the last to reach
+  // each pass's barrier, so it issues first
+  if (table_wave) __builtin_amdgcn_s_setprio(3);
   const uint32_t y = y0 + (row_live ? r : 0u);
   const bool live = lane < G.wpr;
   const uint32_t j = live ? lane : 0u;
@@ -422,6 +431,15 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
   LG_UNROLL
   for (int d = 0; d < 7; ++d) I[d] = 0;
   for (uint32_t q = threadIdx.x; q < 16 * 256; q += blockDim.x) pw_lds[q] = A.pw[q];
+  if (INFLOW == 0 && table_wave && G.inlet_cols > 0) {
+    // the inflow cells' jumps, read once (the table wave alone reads them,
+    // each lane its own entries)
+    const uint32_t cols = G.inlet_cols, rpc = 64u / A.icpr;
+    for (uint32_t c = 0; c * rpc < Bb; ++c) {
+      const uint32_t rr = c * rpc + lane / A.icpr, x = lane % A.icpr;
+      icl[c * 64 + lane] = (rr < Bb && x < cols) ? A.icell[(size_t)(y0 + rr) * cols + x] : 0ull;
+    }
+  }
   __syncthreads();
   if (table_wave) lg_band_tables(A, rb0, nsb, A.t0, lane, pw_lds, tabs_lds);
   if (row_live) {
@@ -538,7 +556,7 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
       lg_band_tables(A, rb0, nsb, A.t0 + i + 1, lane, pw_lds,
                      tabs_lds + (size_t)((i + 1) & 1u) * A.nstreams * LG_WTAB);
     if (INFLOW == 0 && table_wave && i < A.n && G.inlet_cols > 0)
-      lg_band_inflow(G, A.icell, y0, Bb, A.icpr, lane,
+      lg_band_inflow(G, icl, Bb, A.icpr, lane,
                      tabs_lds + (size_t)(i & 1u) * A.nstreams * LG_WTAB + (size_t)nsb * LG_WTAB,
                      ilds + (size_t)(i & 1u) * NW * 8);
     // step i's draws
@@ -802,14 +820,16 @@ int launch_band(lg_t* g, uint32_t n, bool sample, float alpha) {
   uint32_t cpr = 1;
   while (cpr < cols && cpr < 64) cpr <<= 1;
   A.icpr = cpr;
-  const int inflow = cols > 64 ? 2 : ((rows + 64 / cpr - 1) / (64 / cpr) <= 3 ? 0 : 1);
+  const int inflow =
+      cols > 64 ? 2 : ((rows + 64 / cpr - 1) / (64 / cpr) <= LG_BAND_ICHUNKS ? 0 : 1);
   void* args[] = {&A};
   const dim3 grid(nb), block(64 * (rows + 1 + 2 * A.split));
   // [rows][stream tables][helpers' digits][T^(4g) windows][inflow planes]
   // [rows' digits]
   const size_t lds = (size_t)(rows + A.nstreams) * 4096 +
                      (A.split ? 2 * 2 * LG_KDIGITS * 64 * 4 : 0) + 16 * 256 * 8 +
-                     (size_t)2 * rows * 8 * 8 + (size_t)rows * 2 * LG_KDIGITS * 64 * 4;
+                     (size_t)2 * rows * 8 * 8 + (size_t)rows * 2 * LG_KDIGITS * 64 * 4 +
+                     (inflow == 0 ? LG_BAND_ICHUNKS * 64 * 8 : 0);
   // [rows of 64 words][sampling][inflow]
   const void* const kernels[2][2][3] = {
       {{(const void*)lg_band_kernel<false, false, 0>, (const void*)lg_band_kernel<false, false, 1>,

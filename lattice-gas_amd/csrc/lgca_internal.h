@@ -100,6 +100,10 @@ void resolve_pipeline(lg_lattice* g);
 // Resident bands: rows per band at most (one wave per row), and the polls a
 // neighbour wait makes before it gives up (each sleeps ~64 clocks: ~0.1 s).
 #define LG_BAND_MAX_ROWS 11u
+// Most passes of the table wave over its lanes for a band's inflow cells
+// (INFLOW 0): 64 / cpr rows a pass, cpr the inflow width rounded up to a
+// power of two
+#define LG_BAND_ICHUNKS 3u
 // The size rule's resident bands, from measured step costs (wall-clock per
 // step in calls of 10 and 100 steps on MI355X): a step of B-row bands costs
 // about 4.0 + 0.47 B us whatever the row width (at most: 4-row to 9-row
