@@ -132,6 +132,12 @@ __device__ __forceinline__ void lg_band_digits(uint64_t fm, uint64_t cj0,
                                                uint32_t lane, uint32_t* __restrict kout,
                                                bool lead) {
   uint32_t Kl[NK], Kh[NK];
+#if defined(__HIP_DEVICE_COMPILE__)
+  // the fluid mask is the same every step: left to itself the compiler
+  // hoists the masked chains' per-cell lane masks out of the step loop, 64
+  // scalar register pairs that then spill and push other values out too
+  asm volatile("" : "+v"(fm));
+#endif
   uint64_t st = lg_jump_tab(cj0, bt);
   const uint32_t f0 = (uint32_t)fm, f1 = (uint32_t)(fm >> 32);
   if (lead) {
@@ -608,7 +614,10 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
       LG_UNROLL
       for (int q = 0; q < LG_KDIGITS; ++q) Kx[q] = 0;
       if (live) {
-        const uint32_t fh = hh ? (uint32_t)(fm >> 32) : (uint32_t)fm;
+        uint32_t fh = hh ? (uint32_t)(fm >> 32) : (uint32_t)fm;
+#if defined(__HIP_DEVICE_COMPILE__)
+        asm volatile("" : "+v"(fh));  // as in lg_band_digits
+#endif
         const uint64_t st = lg_jump_tab(c0, tab);
         // each half starts at its own jump: a half with one run of cells
         // without a draw needs no end state (lg_draw_half_run)
