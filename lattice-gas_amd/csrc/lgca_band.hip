@@ -414,9 +414,9 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
   const uint32_t rb0 = A.row_band[y0], nsb = A.row_band[y0 + Bb - 1] - rb0 + 1;
   const size_t hstride = (size_t)A.nb * 128;  // one halo kind of one slot: nb x 2 planes x 64
 
-  uint64_t P[7];  // the row's state, loaded once the windows are in LDS
+  uint64_t P[7];
   LG_UNROLL
-  for (int d = 0; d < 7; ++d) P[d] = 0;
+  for (int d = 0; d < 7; ++d) P[d] = A.src[(size_t)d * T + word];
   const uint64_t sm = A.solid[word];
   const uint64_t fm = live ? ~sm & tail : 0ull;
   // a helper's half starts at its own jump; a row's word at its lead jump
@@ -447,14 +447,14 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
     }
   }
   __syncthreads();
-  // the state's loads, issued after the windows' copy has waited for its
-  // own: the first collide (pass 1) needs them, so they land while pass 0
-  // draws
-  if (row_live) {
-    LG_UNROLL
-    for (int d = 0; d < 7; ++d) P[d] = A.src[(size_t)d * T + word];
-  }
   if (table_wave) lg_band_tables(A, rb0, nsb, A.t0, lane, pw_lds, tabs_lds);
+  if (row_live) {
+    uint64_t* s0 = xch + (size_t)r * 256;
+    s0[0 * 64 + lane] = P[1];
+    s0[1 * 64 + lane] = P[2];
+    s0[2 * 64 + lane] = P[4];
+    s0[3 * 64 + lane] = P[5];
+  }
   __syncthreads();
 
   // Pass i collides step i - 1 (i > 0) and draws step i (i < n): one copy
@@ -642,13 +642,6 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
       uint32_t* o = kx + ((size_t)(i & 1u) * 2 + hh) * LG_KDIGITS * 64;
       LG_UNROLL
       for (int q = 0; q < LG_KDIGITS; ++q) o[q * 64 + lane] = Kx[q];
-    }
-    if (i == 0 && row_live) {  // the rows' initial planes for pass 1's collide
-      uint64_t* s0 = xch + (size_t)r * 256;
-      s0[0 * 64 + lane] = P[1];
-      s0[1 * 64 + lane] = P[2];
-      s0[2 * 64 + lane] = P[4];
-      s0[3 * 64 + lane] = P[5];
     }
     __syncthreads();
   }
