@@ -125,7 +125,7 @@ __device__ __forceinline__ uint64_t lg_band_right(uint64_t x, int addr) {
 // Otherwise the whole wave takes the branch-free masked chain, which
 // continues through the masked cells, so the upper half starts where the
 // lower one ended without a second jump.
-template <int NK>
+template <int NK, bool OPAQUE_FM>
 __device__ __forceinline__ void lg_band_digits(uint64_t fm, uint64_t cj0,
                                                const uint64_t* __restrict cjhi,
                                                const uint64_t* __restrict bt, uint32_t stride,
@@ -133,10 +133,12 @@ __device__ __forceinline__ void lg_band_digits(uint64_t fm, uint64_t cj0,
                                                bool lead) {
   uint32_t Kl[NK], Kh[NK];
 #if defined(__HIP_DEVICE_COMPILE__)
-  // the fluid mask is the same every step: left to itself the compiler
-  // hoists the masked chains' per-cell lane masks out of the step loop, 64
-  // scalar register pairs that then spill and push other values out too
-  asm volatile("" : "+v"(fm));
+  // OPAQUE_FM: the fluid mask is the same every step, and left to itself the
+  // compiler hoists the masked chains' per-cell lane masks out of the step
+  // loop, 64 scalar register pairs that then spill and push other values out
+  // too. Measured per instance: in the sampling instances hiding the mask
+  // from the optimiser shortens the step; in the others it lengthens it.
+  if (OPAQUE_FM) asm volatile("" : "+v"(fm));
 #endif
   uint64_t st = lg_jump_tab(cj0, bt);
   const uint32_t f0 = (uint32_t)fm, f1 = (uint32_t)(fm >> 32);
@@ -578,13 +580,13 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
       if (live && !drawless) {
         const uint64_t* cjhi = A.cjump + 2 * word + 1;
         if (G.rest)
-          lg_band_digits<LG_COLLIDE_REST_KBITS>(fm, c0, cjhi,
+          lg_band_digits<LG_COLLIDE_REST_KBITS, SAMPLE>(fm, c0, cjhi,
                                                 tab + (size_t)(stream - rb0) * LG_WTAB,
                                                 LG_COLLIDE_REST_STRIDE, lane,
                                                 kr + (size_t)r * 2 * LG_KDIGITS * 64,
                                                 A.lead_ok != 0);
         else
-          lg_band_digits<LG_COLLIDE_NOREST_KBITS>(fm, c0, cjhi,
+          lg_band_digits<LG_COLLIDE_NOREST_KBITS, SAMPLE>(fm, c0, cjhi,
                                                   tab + (size_t)(stream - rb0) * LG_WTAB,
                                                   LG_COLLIDE_NOREST_STRIDE, lane,
                                                   kr + (size_t)r * 2 * LG_KDIGITS * 64,
@@ -616,7 +618,7 @@ __global__ void __launch_bounds__(64 * (LG_BAND_MAX_ROWS + 1))
       if (live) {
         uint32_t fh = hh ? (uint32_t)(fm >> 32) : (uint32_t)fm;
 #if defined(__HIP_DEVICE_COMPILE__)
-        asm volatile("" : "+v"(fh));  // as in lg_band_digits
+        if (SAMPLE) asm volatile("" : "+v"(fh));  // as in lg_band_digits
 #endif
         const uint64_t st = lg_jump_tab(c0, tab);
         // each half starts at its own jump: a half with one run of cells
