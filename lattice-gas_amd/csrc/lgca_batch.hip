@@ -19,7 +19,11 @@
 // work is the same device code with the member's own geometry and streams.
 #include <chrono>
 #include <cstring>
+#include <atomic>
 #include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
 
 #include "lgca_internal.h"
 
@@ -319,12 +323,50 @@ int queue_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
   return LG_OK;
 }
 
-int upload_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
-  for (lg_t* g : b->members) {
-    int rc = ensure_tables(g);
-    if (rc) return rc;
-    if ((rc = ensure_draw_buffers(g))) return rc;  // the pipeline may change within a call
+// The members' stream-jump tables, built where their obstacles or inflow
+// changed since the last build: on several host threads when there are many
+// (each build is host work of about a millisecond at 2048 x 2048, and a new
+// batch of hundreds of members would otherwise build them one after another
+// inside its first call). Each member's build uses only its own stream.
+int ensure_member_tables(lg_batch* b) {
+  std::vector<lg_t*> todo;
+  for (lg_t* g : b->members)
+    if (g->tables_dirty) todo.push_back(g);
+  const size_t nthreads = std::min<size_t>(
+      todo.size() / 4, std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency())));
+  if (nthreads < 2) {
+    for (lg_t* g : todo)
+      if (int rc = ensure_tables(g)) return rc;
+    return LG_OK;
   }
+  std::atomic<size_t> next{0};
+  std::vector<int> rcs(nthreads, LG_OK);
+  std::vector<std::string> msgs(nthreads);
+  std::vector<std::thread> pool;
+  for (size_t t = 0; t < nthreads; ++t)
+    pool.emplace_back([&, t] {
+      for (size_t i; (i = next.fetch_add(1)) < todo.size();) {
+        // the member's device (a thread's current device starts at 0)
+        int rc = hipSetDevice(todo[i]->device) == hipSuccess
+                     ? ensure_tables(todo[i])
+                     : fail(LG_EDEVICE, "hipSetDevice failed for a batch member's device");
+        if (rc) {
+          rcs[t] = rc;
+          msgs[t] = g_err;  // the worker's own error text
+          return;
+        }
+      }
+    });
+  for (std::thread& th : pool) th.join();
+  for (size_t t = 0; t < nthreads; ++t)
+    if (rcs[t]) return fail(rcs[t], msgs[t]);
+  return LG_OK;
+}
+
+int upload_desc(lg_batch* b, const uint64_t* limits, uint64_t n) {
+  if (int rc = ensure_member_tables(b)) return rc;
+  for (lg_t* g : b->members)
+    if (int rc = ensure_draw_buffers(g)) return rc;  // the pipeline may change within a call
   return queue_desc(b, limits, n);
 }
 
