@@ -102,6 +102,7 @@ def build(r: Request, block: int) -> GpuLattice:
     g.add_solid_box(*W.plate_box(r.w, r.h, r.plate_x, r.plate_rows))
     g.init_equilibrium(r.density, 0.0, 0.0)
     g.attach_field(block, block)
+    g.advance(0)  # builds the stream-jump tables now, not in the first quantum's call
     return g
 
 
